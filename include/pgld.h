@@ -1,0 +1,174 @@
+/* libpgld -- MI355X (gfx950) LD pair-statistics library, C ABI.
+ *
+ * Conventions
+ *  - Every buffer argument is a device pointer owned by the caller unless the
+ *    comment says "host".  The library owns only band-plan memory.
+ *  - Every call is asynchronous on the given hipStream_t (passed as void*);
+ *    NULL means the default stream.
+ *  - Every call returns an int status: 0 = OK, negative = invalid argument
+ *    (-1), input outside the value domain (-2), capacity exceeded (-3) or
+ *    unsupported (-4); positive = the hipError_t of a failed HIP call.  The
+ *    message of the last failure on the calling thread is pgld_last_error().
+ *    Nothing aborts and no C++ exception crosses this boundary.
+ *  - Sizes and offsets are int64; variant and row indices are int32, as in
+ *    the reference's CuPy arrays.
+ *  - Matrices are row-major int8: haplotypes (n_rows = haplotypes, n_var),
+ *    values {-1 missing, 0, 1}; genotypes (n_rows = individuals, n_var),
+ *    values {-1, 0, 1, 2}.
+ *  - Not thread-safe per plan: use one band plan per host thread or stream.
+ *
+ * The reference (kr-colab/pg_gpu) has no C ABI: its operator boundary is the
+ * CuPy-level Python functions named in each comment, which launch NVRTC
+ * kernels through _launch (haplotype_kernels.py:11-13,
+ * genotype_kernels.py:18-20).  Each entry point below replaces one of them.
+ */
+#ifndef PGLD_H
+#define PGLD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGLD_FAMILY_HAP 0
+#define PGLD_FAMILY_GENO 1
+
+#define PGLD_STAT_DD 0
+#define PGLD_STAT_DZ 1
+#define PGLD_STAT_PI2 2      /* fused-kernel pi2 form (haplotype_kernels.py:167-304) */
+#define PGLD_STAT_PI2_SYM 3  /* CuPy pi2 form; differs only for pi2(i,i,k,k) */
+#define PGLD_STAT_SIGMA_D2 4 /* genotype sigma_D^2 (genotype_kernels.py:91-169) */
+
+#define PGLD_KEY_BIN_ABS 0 /* bin of |coord[j]-coord[i]| against edges  */
+#define PGLD_KEY_BIN 1     /* bin of coord[j]-coord[i] against edges       */
+#define PGLD_KEY_WINDOW 2  /* floor((coord[i]-origin)/width), left SNP     */
+
+/* One weighted term of a named statistic: weight * type(pops...).  Mirrors
+ * one (weight, stat_type, pop_indices) entry of generate_stat_specs
+ * (ld_pipeline.py:388-457).  32 bytes. */
+typedef struct pgld_term {
+  int32_t type;    /* PGLD_STAT_* */
+  int32_t pops[4]; /* population indices; unused slots ignored */
+  int32_t pad_;
+  double weight;
+} pgld_term;
+
+int pgld_version(void);
+const char* pgld_last_error(void);
+
+/* Value scan over selected rows: flags[0] = 1 if any value < 0, flags[1] = 1
+ * if any value lies outside [-1, max_value].  rows == NULL selects all rows.
+ * Replaces the has_missing syncs of ld_pipeline.py:158 and :253. */
+int pgld_scan_rows(const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                   int64_t n_sel, int32_t max_value, int32_t* flags, void* stream);
+
+/* Pair enumeration of iter_pairs_within_distance (ld_pipeline.py:57-122).
+ * per[v] = #{j > v : pos[j] - pos[v] <= max_dist}; positions sorted. */
+int pgld_pairs_per_variant(const double* pos, int64_t n, double max_dist, int64_t* per, void* stream);
+/* Expand the pairs of anchors v_lo .. v_lo+n_anchor-1.  excl[k] is the
+ * exclusive prefix sum of per[] over that anchor range (length n_anchor),
+ * n_pairs the total.  Pairs are ordered by (i, j). */
+int pgld_pairs_fill(const int64_t* excl, int64_t v_lo, int64_t n_anchor, int64_t n_pairs,
+                    int32_t* idx_i, int32_t* idx_j, void* stream);
+
+/* Plain per-pair 4-way haplotype counts [n11, n10, n01, n00] (int32, n x 4)
+ * and n_valid (int32, n; may be NULL) over the selected rows.  One thread per
+ * pair, sample columns staged through LDS.  Replaces compute_counts_for_pairs
+ * (ld_pipeline.py:130-175). */
+int pgld_counts_pairs_hap(const int8_t* hap, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                          int64_t n_sel, const int32_t* idx_i, const int32_t* idx_j, int64_t n_pairs,
+                          int32_t* counts, int32_t* n_valid, void* stream);
+
+/* Plain per-pair 9-way genotype counts n00, n01, ..., n22 (int32, n x 9) and
+ * n_valid.  Replaces _GENO_COUNTS_KERN / compute_genotype_counts_for_pairs
+ * (ld_pipeline.py:183-263). */
+int pgld_counts_pairs_geno(const int8_t* geno, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                           int64_t n_sel, const int32_t* idx_i, const int32_t* idx_j, int64_t n_pairs,
+                           int32_t* counts, int32_t* n_valid, void* stream);
+
+/* Per-pair statistics (float64, n_pairs x n_stats) from per-population
+ * counts.  counts is a HOST array of n_pops device pointers, each (n_pairs x
+ * 4) for haplotypes or (n_pairs x 9) for genotypes.  terms / stat_offsets
+ * (device) list, for statistic s, terms[stat_offsets[s] .. stat_offsets[s+1]).
+ * Replaces ld_statistics.dd/dz/pi2 (ld_statistics.py:13-129),
+ * compute_multi_pop_statistics_batch_hap (haplotype_kernels.py:584-640),
+ * compute_multi_pop_statistics_batch_geno (genotype_kernels.py:881-936) and
+ * the sigma_D^2 kernel launch of sigma_d2_geno (ld_statistics_genotype.py:
+ * 1593-1597).  n_pops <= 4. */
+int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* const* counts, int64_t n_pairs,
+                     const pgld_term* terms, const int32_t* stat_offsets, int32_t n_stats, double* out,
+                     void* stream);
+
+/* r (which=0), r^2 (1) or D' (2) per pair from haplotype counts; n_valid may
+ * be NULL (n = sum of counts).  Replaces ld_statistics.r/r_squared/d_prime
+ * (ld_statistics.py:167-246). */
+int pgld_r_family(const int32_t* counts, const int32_t* n_valid, int64_t n_pairs, int32_t which,
+                  double* out, void* stream);
+
+/* Key of every pair: PGLD_KEY_BIN_ABS / PGLD_KEY_BIN use edges (device,
+ * n_keys + 1, ascending) with [e_k, e_{k+1}) semantics of digitize - 1
+ * (moments_ld.py:310-314); PGLD_KEY_WINDOW uses origin / width.  Keys
+ * outside [0, n_keys) are written as -1. */
+int pgld_pair_keys(const double* coord, const int32_t* idx_i, const int32_t* idx_j, int64_t n_pairs,
+                   int32_t mode, const double* edges, int32_t n_keys, double origin, double width,
+                   int32_t* keys, void* stream);
+
+/* sums[key, c] += sum of values[p, c] over pairs with keys[p] == key (float64
+ * atomics; keys < 0 skipped).  Replaces the cp.add.at scatter of
+ * moments_ld.py:335-339 and haplotype_matrix.py:1683-1689. */
+int pgld_keyed_sums(const double* values, const int32_t* keys, int64_t n_pairs, int32_t n_cols,
+                    int32_t n_keys, double* sums, void* stream);
+
+/* ---- tiled int8-MFMA path ---------------------------------------------- */
+
+/* Indicator planes for the MFMA path: SNP-major, sample-contiguous, K padded
+ * with zeros to k_pad (a multiple of 64).  Haplotypes: 2 planes [h, v] with
+ * h = (x == 1), v = (x >= 0); genotypes: 3 planes [v, e1, e2] with
+ * e1 = (g == 1), e2 = (g == 2).  planes has n_planes * n_var * k_pad bytes;
+ * marg (int32, n_planes * n_var) receives each plane's per-SNP sum; flags as
+ * in pgld_scan_rows.  rows selects one population. */
+int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var,
+                     const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
+                     int32_t* flags, void* stream);
+
+typedef struct pgld_band_plan pgld_band_plan;
+
+/* Build the tile schedule of the distance band {(i, j): i < j,
+ * pos[j] - pos[i] <= max_dist} over SNP tiles of 64, and the key slots each
+ * tile can touch.  pos and edges are HOST arrays (edges: n_keys + 1 values
+ * for the bin modes, NULL for windows).  The plan holds device copies. */
+int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int32_t key_mode,
+                          const double* edges, int32_t n_keys, double origin, double width,
+                          pgld_band_plan** plan);
+int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n_pairs, int64_t* n_slots);
+int pgld_band_plan_destroy(pgld_band_plan* plan);
+
+/* Fused band pass: int8 MFMA counts for every pair of the band, every
+ * population, then the moments statistics of ld_names(n_pops) (spec-averaged
+ * as generate_stat_specs, ld_pipeline.py:388-457; haplotypes with two
+ * populations use the fast-path forms of ld_pipeline.py:271-322), then
+ * float64 sums per key, reduced in a fixed order.  planes / marg / k_pad /
+ * n_samples are HOST arrays of n_pops entries (device pointers for planes
+ * and marg).  sums (device, n_keys x n_stats) is overwritten.  Replaces the
+ * chunk loop of moments_ld._compute_ld_sums (moments_ld.py:248-343) and
+ * HaplotypeMatrix.compute_ld_statistics_gpu_* (haplotype_matrix.py:
+ * 1592-1866). */
+int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
+                   const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                   const int32_t* n_samples, int32_t with_missing, double* sums, int64_t* n_pairs_out,
+                   void* stream);
+
+/* Counts of every band pair through the MFMA path, written per pair in the
+ * plan's pair order (i ascending, then j): counts (int32, n_pairs x 4 or x 9)
+ * for ONE population.  Used to check the tiled counts bit-exactly against
+ * the plain kernel. */
+int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes,
+                     const int32_t* marg, int64_t k_pad, int32_t n_samples, int32_t with_missing,
+                     const int64_t* pair_base, int32_t* counts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGLD_H */

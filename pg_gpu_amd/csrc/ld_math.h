@@ -1,0 +1,359 @@
+// Per-pair LD statistics from exact integer counts.
+//
+// Counts arrive as exact integers (int32 from the counting kernels) and are
+// promoted to double; every numerator below is a polynomial with integer or
+// dyadic (1/2, 1/4, 1/16) coefficients, so for populations of up to a few
+// thousand samples it is evaluated exactly in FP64 and each statistic carries
+// a single rounding: the final division.
+//
+// Haplotype formulas follow ld_statistics.py:1116-1617 and
+// haplotype_kernels.py:69-304; genotype formulas follow
+// genotype_kernels.py:27-492.  Invalid denominators give 0 (DD/Dz/pi2) or
+// NaN (r, r^2, D', sigma_D^2) as the reference does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "geno_poly.inc"
+
+namespace pgld {
+
+#define PGLD_HD __host__ __device__ __forceinline__
+
+// ---- statistic / case codes shared with the host -------------------------
+enum StatType : int32_t {
+  ST_DD = 0,
+  ST_DZ = 1,
+  ST_PI2 = 2,       // fused-kernel form (haplotype_kernels.py:167-304)
+  ST_PI2_SYM = 3,   // CuPy form: pi2(i,i,k,k) symmetrised (ld_statistics.py:1337-1349)
+  ST_SIGMA_D2 = 4,  // genotype sigma_D^2 (genotype_kernels.py:91-169)
+};
+
+PGLD_HD double safe_div(double num, double den) { return den > 0.0 ? num / den : 0.0; }
+
+// ---------------------------------------------------------------------------
+// Haplotype per-population values (ld_pipeline.py:465-482)
+// ---------------------------------------------------------------------------
+struct HapP {
+  double c1, c2, c3, c4, n, D, pA, qA, pB, qB;
+};
+
+PGLD_HD HapP hap_pop(double c1, double c2, double c3, double c4) {
+  HapP p;
+  p.c1 = c1; p.c2 = c2; p.c3 = c3; p.c4 = c4;
+  p.n = c1 + c2 + c3 + c4;
+  p.D = c2 * c3 - c1 * c4;
+  p.pA = c1 + c2; p.qA = c3 + c4;
+  p.pB = c1 + c3; p.qB = c2 + c4;
+  return p;
+}
+
+PGLD_HD double f4(double n) { return n * (n - 1.0) * (n - 2.0) * (n - 3.0); }
+PGLD_HD double f3(double n) { return n * (n - 1.0) * (n - 2.0); }
+PGLD_HD double f2(double n) { return n * (n - 1.0); }
+
+PGLD_HD double hap_dd_single(const HapP& a) {
+  double num = a.c1 * (a.c1 - 1.0) * a.c4 * (a.c4 - 1.0) + a.c2 * (a.c2 - 1.0) * a.c3 * (a.c3 - 1.0) -
+               2.0 * a.c1 * a.c2 * a.c3 * a.c4;
+  return safe_div(num, f4(a.n));
+}
+PGLD_HD double hap_dd_between(const HapP& a, const HapP& b) {
+  return safe_div(a.D * b.D, f2(a.n) * f2(b.n));
+}
+PGLD_HD double hap_dz_iii(const HapP& a) {
+  double diff = a.c1 * a.c4 - a.c2 * a.c3;
+  double num = diff * (a.qA - a.pA) * (a.qB - a.pB) + diff * ((a.c2 + a.c3) - (a.c1 + a.c4)) +
+               2.0 * (a.c2 * a.c3 + a.c1 * a.c4);
+  return safe_div(num, f4(a.n));
+}
+// Dz(i,i,k): a = pop i, b = pop k
+PGLD_HD double hap_dz_iij(const HapP& a, const HapP& b) {
+  double num = (a.qA - a.pA) * (-a.D) * (-b.c1 + b.c2 - b.c3 + b.c4);
+  return safe_div(num, b.n * f3(a.n));
+}
+// Dz(i,j,i): a = pop i, b = pop j
+PGLD_HD double hap_dz_iji(const HapP& a, const HapP& b) {
+  double num = (-a.c1 + a.c2 - a.c3 + a.c4) * (-a.D) * (b.qA - b.pA);
+  return safe_div(num, b.n * f3(a.n));
+}
+// Dz(i,j,j): a = pop i, b = pop j
+PGLD_HD double hap_dz_ijj(const HapP& a, const HapP& b) {
+  double num = (-a.D) * ((-b.c1 + b.c2 + b.c3 - b.c4) + (-b.c1 + b.c2 - b.c3 + b.c4) * (b.qA - b.pA));
+  return safe_div(num, f2(a.n) * f2(b.n));
+}
+PGLD_HD double hap_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
+  double num = -(a.D * (b.pA - b.qA) * (c.c1 - c.c2 + c.c3 - c.c4));
+  return safe_div(num, f2(a.n) * b.n * c.n);
+}
+PGLD_HD double hap_pi2_iiii(const HapP& a) {
+  double num = a.pA * a.pB * a.qB * a.qA - a.c1 * a.c4 * (-1.0 + a.c1 + 3.0 * a.c2 + 3.0 * a.c3 + a.c4) -
+               a.c2 * a.c3 * (-1.0 + 3.0 * a.c1 + a.c2 + a.c3 + 3.0 * a.c4);
+  return safe_div(num, f4(a.n));
+}
+// three samples of pop t, one of pop s (haplotype_kernels.py:268-289)
+PGLD_HD double hap_pi2_triple(const HapP& t, const HapP& s) {
+  double num = (-(t.pA * t.c4 * s.pB) - (t.c2 * t.qA * s.pB) + (t.pA * (t.c2 + t.c4) * t.qA * s.pB) +
+                (t.pA * t.qA * (-2.0 * s.c2 - 2.0 * s.c4)) + (t.pA * t.c4 * s.qB) + (t.c2 * t.qA * s.qB) +
+                (t.pA * (t.c1 + t.c3) * t.qA * s.qB)) * 0.5;
+  return safe_div(num, s.n * f3(t.n));
+}
+PGLD_HD double hap_pi2_iikk(const HapP& a, const HapP& b) {
+  return safe_div(a.pA * a.qA * (b.pB * b.qB), f2(a.n) * f2(b.n));
+}
+PGLD_HD double hap_pi2_iikk_sym(const HapP& a, const HapP& b) {
+  double num = 0.5 * (a.pA * a.qA * b.pB * b.qB + b.pA * b.qA * a.pB * a.qB);
+  return safe_div(num, f2(a.n) * f2(b.n));
+}
+// pi2(i,i,k,l)
+PGLD_HD double hap_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
+  return safe_div(a.pA * a.qA * (b.pB * c.qB + b.qB * c.pB), 2.0 * f2(a.n) * b.n * c.n);
+}
+// pi2(i,j,k,k)
+PGLD_HD double hap_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
+  return safe_div(c.pB * c.qB * (a.pA * b.qA + a.qA * b.pA), 2.0 * f2(c.n) * a.n * b.n);
+}
+PGLD_HD double hap_pi2_ijij(const HapP& A, const HapP& B) {
+  const double a1 = A.c1, a2 = A.c2, a3 = A.c3, a4 = A.c4;
+  const double b1 = B.c1, b2 = B.c2, b3 = B.c3, b4 = B.c4;
+  double quart = A.qB * A.qA * B.pA * B.pB + A.pB * A.qA * B.pA * B.qB + A.pA * A.qB * B.pB * B.qA +
+                 A.pA * A.pB * B.qB * B.qA;
+  // cubic correction, grouped by the b-count it multiplies
+  double cub = b1 * (-(a2 * a3) + a4 - a2 * a4 - a3 * a4 - a4 * a4 - a4 * b1) +
+               b2 * (a3 - a1 * a3 - a3 * a3 - a1 * a4 - a3 * a4 - a3 * b1 - a4 * b1 - a3 * b2) +
+               b3 * (a2 - a1 * a2 - a2 * a2 - a1 * a4 - a2 * a4 - a2 * b1 - a4 * b1 - a1 * b2 - a4 * b2 -
+                     a2 * b3) +
+               b4 * (a1 - a1 * a1 - a1 * a2 - a1 * a3 - a2 * a3 - a2 * b1 - a3 * b1 - a1 * b2 - a3 * b2 -
+                     a1 * b3 - a2 * b3 - a1 * b4);
+  return safe_div((quart + cub) * 0.25, f2(A.n) * f2(B.n));
+}
+// one population shared between the two locus pairs (haplotype_kernels.py:218-241)
+PGLD_HD double hap_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
+  const double s1 = S.c1, s2 = S.c2, s3 = S.c3, s4 = S.c4;
+  const double xa = X.pA, xq = X.qA;
+  const double b1 = Y.c1, b2 = Y.c2, b3 = Y.c3, b4 = Y.c4;
+  double inner = (-1.0 + s3) * b1 + s3 * b2 - b3 + s3 * b3 + s3 * b4 + s1 * (b2 + b4);
+  double num = s4 * s4 * xa * Y.pB + s2 * s2 * xq * Y.pB + (-1.0 + s1 + s3) * (s3 * xa + s1 * xq) * Y.qB +
+               s4 * (s1 * xq * Y.pB + xa * inner) +
+               s2 * (s4 * (xa + xq) * Y.pB + s3 * (xa * Y.pB + xq * Y.qB) +
+                     xq * ((-1.0 + s1) * b1 - b3 + s1 * (b2 + b3 + b4)));
+  return safe_div(num * 0.25, f2(S.n) * X.n * Y.n);
+}
+PGLD_HD double hap_pi2_alldiff(const HapP& a, const HapP& b, const HapP& c, const HapP& d) {
+  double HA = a.pA * b.qA + a.qA * b.pA;
+  double HB = c.pB * d.qB + c.qB * d.pB;
+  return safe_div(HA * HB, 4.0 * a.n * b.n * c.n * d.n);
+}
+
+// ---------------------------------------------------------------------------
+// Genotype per-population values.  Counts in moments order g[0..8] =
+// g1..g9 = n22, n21, n20, n12, n11, n10, n02, n01, n00
+// (genotype_kernels.py:424-435).
+// ---------------------------------------------------------------------------
+struct GenoP {
+  double g[9];
+  double n;
+  double pA, qA, pB, qB;      // allele counts at the two loci
+  double xAB, xAb, xaB, xab;  // same-individual joint allele counts
+};
+
+PGLD_HD GenoP geno_pop(const double* cnt_n00_to_n22) {
+  GenoP p;
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    p.g[t] = cnt_n00_to_n22[8 - t];
+    s += p.g[t];
+  }
+  p.n = s;
+  const double* g = p.g;
+  p.pA = g[0] + g[1] + g[2] + 0.5 * (g[3] + g[4] + g[5]);
+  p.qA = 0.5 * (g[3] + g[4] + g[5]) + g[6] + g[7] + g[8];
+  p.pB = g[0] + 0.5 * g[1] + g[3] + 0.5 * g[4] + g[6] + 0.5 * g[7];
+  p.qB = 0.5 * g[1] + g[2] + 0.5 * g[4] + g[5] + 0.5 * g[7] + g[8];
+  p.xAB = g[0] + 0.5 * g[1] + 0.5 * g[3] + 0.25 * g[4];
+  p.xAb = 0.5 * g[1] + g[2] + 0.25 * g[4] + 0.5 * g[5];
+  p.xaB = 0.5 * g[3] + 0.25 * g[4] + g[6] + 0.5 * g[7];
+  p.xab = 0.25 * g[4] + 0.5 * g[5] + 0.5 * g[7] + g[8];
+  return p;
+}
+
+// D from genotype frequencies (_PopDataGeno.D_geno, genotype_kernels.py:441-446)
+PGLD_HD double geno_D(const GenoP& p) { return p.xAB * p.xab - p.xAb * p.xaB; }
+
+PGLD_HD double geno_dd_num(const GenoP& p) {
+  double Dg = geno_D(p);
+  return 4.0 * (geno_poly_dd(p.g) * (1.0 / 16.0) + Dg * Dg);
+}
+PGLD_HD double geno_pi2_num(const GenoP& p) {
+  return p.pA * p.pB * p.qB * p.qA + geno_poly_pi2corr(p.g) * (1.0 / 16.0);
+}
+PGLD_HD double geno_dd_single(const GenoP& p) {
+  return p.n < 4.0 ? 0.0 : geno_dd_num(p) / f4(p.n);
+}
+PGLD_HD double geno_dd_between(const GenoP& a, const GenoP& b) {
+  return safe_div(4.0 * geno_D(a) * geno_D(b), f2(a.n) * f2(b.n));
+}
+PGLD_HD double geno_Acoef(const GenoP& p) {
+  const double* g = p.g;
+  return -g[0] + g[2] - g[3] + g[5] - g[6] + g[8];
+}
+PGLD_HD double geno_Bcoef(const GenoP& p) {
+  const double* g = p.g;
+  return -g[0] - g[1] - g[2] + g[6] + g[7] + g[8];
+}
+PGLD_HD double geno_dz_iii(const GenoP& p) {
+  if (p.n < 4.0) return 0.0;
+  double num = 0.25 * geno_poly_dzq(p.g) + geno_Acoef(p) * geno_Bcoef(p) * geno_D(p);
+  return 2.0 * num / f4(p.n);
+}
+// Dz precompute features (genotype_kernels.py:176-197)
+PGLD_HD double geno_Ucoef(const GenoP& p) {
+  const double* g = p.g;
+  double MA = 2. * g[1] * g[6] + 4. * g[2] * g[6] + 2. * g[4] * g[6] + 4. * g[5] * g[6] - 2. * g[0] * g[7] +
+              2. * g[2] * g[7] - 2. * g[3] * g[7] + 2. * g[5] * g[7] - 4. * g[0] * g[8] - 2. * g[1] * g[8] -
+              4. * g[3] * g[8] - 2. * g[4] * g[8];
+  return geno_D(p) * (1. + geno_Bcoef(p)) + 0.25 * MA;
+}
+PGLD_HD double geno_Vcoef(const GenoP& p) {
+  const double* g = p.g;
+  double MB = 2. * g[2] * g[3] + 2. * g[2] * g[4] - 2. * g[0] * g[5] - 2. * g[1] * g[5] + 4. * g[2] * g[6] +
+              2. * g[5] * g[6] + 4. * g[2] * g[7] + 2. * g[5] * g[7] - 4. * g[0] * g[8] - 4. * g[1] * g[8] -
+              2. * g[3] * g[8] - 2. * g[4] * g[8];
+  return geno_D(p) * (1. + geno_Acoef(p)) + 0.25 * MB;
+}
+PGLD_HD double geno_Wjj(const GenoP& p) {
+  const double* g = p.g;
+  return (-g[0] + g[2] + g[6] - g[8]) + geno_Acoef(p) * geno_Bcoef(p);
+}
+// The reference's multi-pop Dz kernels divide without a guard
+// (genotype_kernels.py:199-233); a zero denominator gives inf/NaN there too.
+PGLD_HD double geno_dz_iij(const GenoP& a, const GenoP& b) {
+  return 2.0 * geno_Ucoef(a) * geno_Acoef(b) / (b.n * f3(a.n));
+}
+PGLD_HD double geno_dz_iji(const GenoP& a, const GenoP& b) {
+  return 2.0 * geno_Vcoef(a) * geno_Bcoef(b) / (b.n * f3(a.n));
+}
+PGLD_HD double geno_dz_ijj(const GenoP& a, const GenoP& b) {
+  return 2.0 * geno_D(a) * geno_Wjj(b) / (f2(b.n) * f2(a.n));
+}
+PGLD_HD double geno_dz_diff(const GenoP& a, const GenoP& b, const GenoP& c) {
+  return 2.0 * geno_D(a) * geno_Bcoef(b) * geno_Acoef(c) / (f2(a.n) * b.n * c.n);
+}
+PGLD_HD double geno_pi2_iiii(const GenoP& p) {
+  return p.n < 4.0 ? 0.0 : geno_pi2_num(p) / f4(p.n);
+}
+PGLD_HD double nn1(const GenoP& p) {
+  double v = 4.0 * p.n * (p.n - 1.0);
+  return v > 1.0 ? v : 1.0;
+}
+PGLD_HD double geno_BL(const GenoP& p) { return (4.0 * p.pA * p.qA - (p.g[3] + p.g[4] + p.g[5])) / nn1(p); }
+PGLD_HD double geno_BR(const GenoP& p) { return (4.0 * p.pB * p.qB - (p.g[1] + p.g[4] + p.g[7])) / nn1(p); }
+// triple coefficients (genotype_kernels.py:473-489)
+PGLD_HD void geno_triple_A(const GenoP& p, double& AQ3, double& AS3) {
+  const double* g = p.g;
+  double P = p.pA, R = p.qA, Q = p.pB, S = p.qB;
+  double L0 = g[4] + 2 * g[5] + 2 * g[7] + 4 * g[8];
+  double L1 = g[4] + 2 * g[5];
+  double negR3 = 3 * g[3] + 3 * g[4] + 3 * g[5] + 4 * g[6] + 4 * g[7] + 4 * g[8];
+  double negR5 = 5 * g[3] + 5 * g[4] + 5 * g[5] + 8 * g[6] + 8 * g[7] + 8 * g[8];
+  double beta5A = -2 * g[1] - 4 * g[2] + 4 * g[3] + g[4] - 2 * g[5] + 8 * g[6] + 4 * g[7] + 4;
+  double beta6A = -g[4] - 2 * g[5] - 4 * g[6] - 4 * g[7] - 4 * g[8];
+  AQ3 = -(S * negR3) / 8 + P * S * R / 2 + (R * L0) / 8 + L1 / 8 - (P * L0) / 8;
+  AS3 = -(Q * negR5) / 8 + P * Q * R / 2 + (R * beta5A) / 8 + beta6A / 8 + (P * L0) / 8;
+}
+PGLD_HD void geno_triple_B(const GenoP& p, double& BP3, double& BR3) {
+  const double* g = p.g;
+  double P = p.pA, R = p.qA, Q = p.pB, S = p.qB;
+  double L0 = g[4] + 2 * g[5] + 2 * g[7] + 4 * g[8];
+  double M1 = g[4] + 2 * g[7];
+  double negL3 = 3 * g[1] + 4 * g[2] + 3 * g[4] + 4 * g[5] + 3 * g[7] + 4 * g[8];
+  double negL5 = 5 * g[1] + 8 * g[2] + 5 * g[4] + 8 * g[5] + 5 * g[7] + 8 * g[8];
+  double beta5B = 4 * g[1] + 8 * g[2] - 2 * g[3] + g[4] + 4 * g[5] - 4 * g[6] - 2 * g[7] + 4;
+  double beta6B = -4 * g[2] - g[4] - 4 * g[5] - 2 * g[7] - 4 * g[8];
+  BP3 = -(R * negL3) / 8 + Q * S * R / 2 + (S * L0) / 8 + M1 / 8 - (Q * L0) / 8;
+  BR3 = -(P * negL5) / 8 + P * Q * S / 2 + (S * beta5B) / 8 + beta6B / 8 + (Q * L0) / 8;
+}
+// pi2(t,t;t,s) and pi2(t,t;s,t): single pop in the B-locus pair
+PGLD_HD double geno_pi2_iiij(const GenoP& t, const GenoP& s) {
+  double AQ3, AS3;
+  geno_triple_A(t, AQ3, AS3);
+  return safe_div(AQ3 * s.pB + AS3 * s.qB, f3(t.n) * s.n);
+}
+// pi2(t,s;t,t) and pi2(s,t;t,t): single pop in the A-locus pair
+PGLD_HD double geno_pi2_ijii(const GenoP& t, const GenoP& s) {
+  double BP3, BR3;
+  geno_triple_B(t, BP3, BR3);
+  return safe_div(BP3 * s.pA + BR3 * s.qA, f3(t.n) * s.n);
+}
+PGLD_HD double geno_pi2_iikk(const GenoP& a, const GenoP& b) { return geno_BL(a) * geno_BR(b); }
+PGLD_HD double geno_pi2_iikl(const GenoP& a, const GenoP& b, const GenoP& c) {
+  double qb = b.pB / b.n, sb = b.qB / b.n, qc = c.pB / c.n, sc = c.qB / c.n;
+  return geno_BL(a) * 0.5 * (qb * sc + sb * qc);
+}
+PGLD_HD double geno_pi2_ijkk(const GenoP& a, const GenoP& b, const GenoP& c) {
+  double pa = a.pA / a.n, ra = a.qA / a.n, pb = b.pA / b.n, rb = b.qA / b.n;
+  return 0.5 * (pa * rb + ra * pb) * geno_BR(c);
+}
+PGLD_HD double geno_pi2_ijij(const GenoP& A, const GenoP& B) {
+  double Pi = A.pA, Qi = A.pB, Ri = A.qA, Si = A.qB;
+  double Pj = B.pA, Qj = B.pB, Rj = B.qA, Sj = B.qB;
+  double X01i = A.xAb, X10i = A.xaB, X11i = A.xab, Bi = A.xAB - A.xAb - A.xaB + A.xab;
+  double X01j = B.xAb, X11j = B.xab, Bj = B.xAB - B.xAb - B.xaB + B.xab;
+  double PQi = Pi * Qi, PRi = Pi * Ri, PSi = Pi * Si, QRi = Qi * Ri, RSi = Ri * Si;
+  double PQj = Pj * Qj, PSj = Pj * Sj, QRj = Qj * Rj, RSj = Rj * Sj;
+  double Tc = Ri * (1.0 + X01i - X10i), Uc = Si * (1.0 + Ri);
+  (void)X01j;
+  double num = 0.0;
+  num += -0.25 * RSi * Qj - 0.25 * X11i * PQj + 0.25 * RSi * PQj - 0.25 * Ri * PSj;
+  num += -0.25 * PQi * X11j - 0.25 * RSi * X11j + 0.25 * PSi * X11j + 0.25 * QRi * X11j;
+  num += -0.25 * PRi * Sj + 0.25 * Tc * Sj + 0.25 * X11i * PSj + 0.25 * QRi * PSj;
+  num += 0.25 * Uc * Rj - 0.25 * X01i * QRj + 0.25 * PSi * QRj;
+  num += -0.25 * PSi * Rj - 0.25 * Pi * RSj + 0.25 * X01i * RSj + 0.25 * PQi * RSj;
+  num += 0.25 * X11i * Bj + 0.25 * Bi * X11j - 1.0 * X11i * X11j;
+  return safe_div(num, f2(A.n) * f2(B.n));
+}
+// one shared population s; a = other pop of the first pair, b = of the second
+PGLD_HD double geno_pi2_shared(const GenoP& S, const GenoP& A, const GenoP& B) {
+  double d = nn1(S);
+  double C00 = (S.pA * S.pB - S.xAB) / d, C01 = (S.pA * S.qB - S.xAb) / d;
+  double C10 = (S.qA * S.pB - S.xaB) / d, C11 = (S.qA * S.qB - S.xab) / d;
+  double ra = A.qA / A.n, pa = A.pA / A.n, sb = B.qB / B.n, qb = B.pB / B.n;
+  return ra * (C00 * sb + C01 * qb) + pa * (C10 * sb + C11 * qb);
+}
+PGLD_HD double geno_pi2_alldiff(const GenoP& a, const GenoP& b, const GenoP& c, const GenoP& d) {
+  double pa = a.pA / a.n, ra = a.qA / a.n, pb = b.pA / b.n, rb = b.qA / b.n;
+  double qc = c.pB / c.n, sc = c.qB / c.n, qd = d.pB / d.n, sd = d.qB / d.n;
+  return 0.25 * (pa * rb + ra * pb) * (qc * sd + sc * qd);
+}
+PGLD_HD double geno_sigma_d2(const GenoP& p) {
+  const double nan_v = __builtin_nan("");
+  if (p.n < 4.0) return nan_v;
+  double pn = geno_pi2_num(p);
+  return pn > 0.0 ? geno_dd_num(p) / pn : nan_v;
+}
+
+// ---------------------------------------------------------------------------
+// r, r^2, D' (haplotype_kernels.py:20-62)
+// ---------------------------------------------------------------------------
+PGLD_HD void hap_r_family(double a, double b, double c, double d, double n, double& r, double& r2,
+                          double& dp) {
+  const double nan_v = __builtin_nan("");
+  double pA = (a + b) / n, pB = (a + c) / n;
+  double qA = 1.0 - pA, qB = 1.0 - pB;
+  double D = (a * d - b * c) / (n * n);
+  double den = pA * qA * pB * qB;
+  r = den > 0.0 ? D / sqrt(den) : nan_v;
+  r2 = den > 0.0 ? D * D / den : nan_v;
+  double dmax;
+  if (D >= 0.0) {
+    double v1 = pA * qB, v2 = qA * pB;
+    dmax = v1 < v2 ? v1 : v2;
+  } else {
+    double v1 = pA * pB, v2 = qA * qB;
+    dmax = v1 < v2 ? v1 : v2;
+  }
+  dp = dmax > 0.0 ? D / dmax : nan_v;
+}
+
+}  // namespace pgld
