@@ -89,16 +89,21 @@ int pgld_counts_pairs_geno(const int8_t* geno, int64_t n_rows, int64_t n_var, co
 
 /* Per-pair statistics (float64, n_pairs x n_stats) from per-population
  * counts.  counts is a HOST array of n_pops device pointers, each (n_pairs x
- * 4) for haplotypes or (n_pairs x 9) for genotypes.  terms / stat_offsets
+ * 4) for haplotypes or (n_pairs x 9) for genotypes.  n_valid is NULL or a
+ * HOST array of n_pops device pointers (entries may be NULL): for haplotypes a
+ * non-NULL entry replaces that population's sample size n (the reference's
+ * n_valid argument, ld_statistics.py:1082-1113); genotype statistics always
+ * use n = sum of the 9 counts, as the single-population genotype kernels do
+ * (genotype_kernels.py:35).  terms / stat_offsets
  * (device) list, for statistic s, terms[stat_offsets[s] .. stat_offsets[s+1]).
  * Replaces ld_statistics.dd/dz/pi2 (ld_statistics.py:13-129),
  * compute_multi_pop_statistics_batch_hap (haplotype_kernels.py:584-640),
  * compute_multi_pop_statistics_batch_geno (genotype_kernels.py:881-936) and
  * the sigma_D^2 kernel launch of sigma_d2_geno (ld_statistics_genotype.py:
  * 1593-1597).  n_pops <= 4. */
-int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* const* counts, int64_t n_pairs,
-                     const pgld_term* terms, const int32_t* stat_offsets, int32_t n_stats, double* out,
-                     void* stream);
+int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* const* counts,
+                     const int32_t* const* n_valid, int64_t n_pairs, const pgld_term* terms,
+                     const int32_t* stat_offsets, int32_t n_stats, double* out, void* stream);
 
 /* r (which=0), r^2 (1) or D' (2) per pair from haplotype counts; n_valid may
  * be NULL (n = sum of counts).  Replaces ld_statistics.r/r_squared/d_prime
@@ -123,7 +128,7 @@ int pgld_keyed_sums(const double* values, const int32_t* keys, int64_t n_pairs, 
 /* ---- tiled int8-MFMA path ---------------------------------------------- */
 
 /* Indicator planes for the MFMA path: SNP-major, sample-contiguous, K padded
- * with zeros to k_pad (a multiple of 64).  Haplotypes: 2 planes [h, v] with
+ * with zeros to k_pad (a multiple of 128).  Haplotypes: 2 planes [h, v] with
  * h = (x == 1), v = (x >= 0); genotypes: 3 planes [v, e1, e2] with
  * e1 = (g == 1), e2 = (g == 2).  planes has n_planes * n_var * k_pad bytes;
  * marg (int32, n_planes * n_var) receives each plane's per-SNP sum; flags as
@@ -136,28 +141,37 @@ typedef struct pgld_band_plan pgld_band_plan;
 
 /* Build the tile schedule of the distance band {(i, j): i < j,
  * pos[j] - pos[i] <= max_dist} over SNP tiles of 64, and the key slots each
- * tile can touch.  pos and edges are HOST arrays (edges: n_keys + 1 values
- * for the bin modes, NULL for windows).  The plan holds device copies. */
+ * tile can touch.  pos, key_coord and edges are HOST arrays: band membership
+ * uses pos; keys use key_coord (NULL = pos; must be nondecreasing, e.g. a
+ * genetic map interpolated at pos, moments_ld.py:167-174); edges holds
+ * n_keys + 1 values for the bin modes (NULL for windows).  The plan keeps
+ * device copies and its own workspace. */
 int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int32_t key_mode,
-                          const double* edges, int32_t n_keys, double origin, double width,
-                          pgld_band_plan** plan);
+                          const double* key_coord, const double* edges, int32_t n_keys, double origin,
+                          double width, pgld_band_plan** plan);
 int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n_pairs, int64_t* n_slots);
 int pgld_band_plan_destroy(pgld_band_plan* plan);
 
+#define PGLD_SET_MOMENTS 0 /* ld_names(n_pops): 3 / 15 / 45 statistics for 1 / 2 / 3 pops */
+#define PGLD_SET_SELF 1    /* DD_i_i, Dz_i_i_i, pi2_i_i_i_i per population (3 * n_pops) */
+
 /* Fused band pass: int8 MFMA counts for every pair of the band, every
- * population, then the moments statistics of ld_names(n_pops) (spec-averaged
+ * population, then the statistics of stat_set (PGLD_SET_MOMENTS: ld_names(n_pops), spec-averaged
  * as generate_stat_specs, ld_pipeline.py:388-457; haplotypes with two
  * populations use the fast-path forms of ld_pipeline.py:271-322), then
- * float64 sums per key, reduced in a fixed order.  planes / marg / k_pad /
- * n_samples are HOST arrays of n_pops entries (device pointers for planes
- * and marg).  sums (device, n_keys x n_stats) is overwritten.  Replaces the
+ * float64 sums per key, reduced in a fixed order (bit-reproducible run to
+ * run).  planes / marg / k_pad / n_samples are HOST arrays of n_pops entries
+ * (device pointers for planes and marg).  sums (device, n_keys x n_stats) is
+ * overwritten; key_pairs (device, n_keys, may be NULL) receives the number of
+ * pairs per key.  Haplotypes: moments set for n_pops <= 3, self set for
+ * n_pops <= 4; genotypes: n_pops <= 2.  Replaces the
  * chunk loop of moments_ld._compute_ld_sums (moments_ld.py:248-343) and
  * HaplotypeMatrix.compute_ld_statistics_gpu_* (haplotype_matrix.py:
  * 1592-1866). */
 int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
-                   const int32_t* n_samples, int32_t with_missing, double* sums, int64_t* n_pairs_out,
-                   void* stream);
+                   const int32_t* n_samples, int32_t with_missing, int32_t stat_set, double* sums,
+                   double* key_pairs, void* stream);
 
 /* Counts of every band pair through the MFMA path, written per pair in the
  * plan's pair order (i ascending, then j): counts (int32, n_pairs x 4 or x 9)

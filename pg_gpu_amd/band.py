@@ -1,0 +1,148 @@
+"""Tiled int8-MFMA band path: packed indicator planes, band plans and fused
+per-key LD sums (libpgld pgld_pack_planes / pgld_band_*).
+
+This replaces the chunk loop of moments_ld._compute_ld_sums
+(moments_ld.py:248-343) and HaplotypeMatrix.compute_ld_statistics_gpu_*
+(haplotype_matrix.py:1592-1866): one launch counts every pair of the distance
+band for every population with int8 MFMA, evaluates the statistics from the
+exact counts and reduces them per key (distance bin or window) in a fixed
+order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+def k_pad_for(n_samples):
+    """K padding of a population: a multiple of 128 bytes, at least 128."""
+    return max(128, ((int(n_samples) + 127) // 128) * 128)
+
+
+@dataclass
+class PackedPop:
+    """Indicator planes of one population (SNP-major, K padded)."""
+    family: int
+    planes: object       # int8 tensor (n_planes, n_var, k_pad)
+    marg: object         # int32 tensor (n_planes, n_var)
+    k_pad: int
+    n_samples: int
+    has_missing: bool
+
+
+def pack_population(data, rows=None, family="hap", k_pad=None):
+    """Pack one population's rows of a (rows, n_var) int8 matrix."""
+    fam = L.FAMILY_HAP if family == "hap" else L.FAMILY_GENO
+    g = L.to_device(data, np.int8)
+    n_rows, n_var = g.shape
+    rows_t = None
+    n_sel = n_rows
+    if rows is not None:
+        r = np.asarray(L.to_numpy(rows), dtype=np.int64)
+        if r.size and (r.min() < 0 or r.max() >= n_rows):
+            raise IndexError("population row index out of range")
+        rows_t = L.to_device(r, np.int32)
+        n_sel = int(r.size)
+    kp = k_pad_for(n_sel) if k_pad is None else int(k_pad)
+    npl = 2 if fam == L.FAMILY_HAP else 3
+    planes = L.empty((npl, n_var, kp), np.int8)
+    marg = L.empty((npl, n_var), np.int32)
+    flags = L.zeros((2,), np.int32)
+    L.call("pgld_pack_planes", fam, L.ptr(g), n_rows, n_var, L.ptr(rows_t), n_sel, kp, L.ptr(planes),
+           L.ptr(marg), L.ptr(flags), L.stream_handle())
+    f = flags.cpu().numpy()
+    if f[1]:
+        raise ValueError("input values outside {-1, 0, 1}" if fam == L.FAMILY_HAP
+                         else "genotype values outside {-1, 0, 1, 2}")
+    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]))
+
+
+class BandPlan:
+    """Tile schedule of the band {(i, j): i < j, pos[j] - pos[i] <= max_dist}
+    keyed by distance bins (``edges``) or by windows of the left SNP
+    (``origin``, ``width``, ``n_keys``)."""
+
+    def __init__(self, positions, max_dist, edges=None, *, window=None, n_keys=None, origin=0.0,
+                 key_coord=None, abs_dist=False):
+        L.lib()
+        pos = np.ascontiguousarray(L.to_numpy(positions), dtype=np.float64)
+        self.n_var = int(pos.shape[0])
+        self.max_dist = float(max_dist)
+        kc = None if key_coord is None else np.ascontiguousarray(L.to_numpy(key_coord), dtype=np.float64)
+        if window is not None:
+            mode = L.KEY_WINDOW
+            if n_keys is None:
+                span = (pos[-1] - origin) if self.n_var else 0.0
+                n_keys = int(np.floor(span / window)) + 1 if self.n_var else 1
+            e = None
+            width = float(window)
+        else:
+            if edges is None:
+                raise ValueError("BandPlan needs bin edges or a window width")
+            e = np.ascontiguousarray(edges, dtype=np.float64)
+            n_keys = len(e) - 1
+            mode = L.KEY_BIN_ABS if abs_dist else L.KEY_BIN
+            width = 1.0
+        self.n_keys = int(n_keys)
+        self.key_mode = mode
+        self._keep = (pos, kc, e)
+        h = C.c_void_p()
+        L.call("pgld_band_plan_create", pos.ctypes.data_as(C.c_void_p), self.n_var, self.max_dist, mode,
+               None if kc is None else kc.ctypes.data_as(C.c_void_p),
+               None if e is None else e.ctypes.data_as(C.c_void_p), self.n_keys, float(origin), width,
+               C.byref(h))
+        self._h = h
+        nt, npairs, ns = C.c_int64(), C.c_int64(), C.c_int64()
+        L.call("pgld_band_plan_info", h, C.byref(nt), C.byref(npairs), C.byref(ns))
+        self.n_tiles, self.n_pairs, self.n_slots = nt.value, npairs.value, ns.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().pgld_band_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    def sums(self, pops, stat_set="moments", with_missing=None, want_pairs=False):
+        """Fused band pass over packed populations.  Returns float64 tensors
+        (n_keys, n_stats) and, if asked, the pair count per key."""
+        P = len(pops)
+        fam = pops[0].family
+        if any(p.family != fam for p in pops):
+            raise ValueError("all populations must be of one family")
+        if with_missing is None:
+            with_missing = any(p.has_missing for p in pops)
+        sset = L.SET_MOMENTS if stat_set == "moments" else L.SET_SELF
+        if sset == L.SET_SELF and P > 1:
+            n_stats = 3 * P
+        else:
+            n_stats = {1: 3, 2: 15, 3: 45, 4: 105}[P]
+        sums = L.empty((self.n_keys, n_stats), np.float64)
+        kp = L.empty((self.n_keys,), np.float64) if want_pairs else None
+        planes, _k1 = L.ptr_array([p.planes for p in pops])
+        margs, _k2 = L.ptr_array([p.marg for p in pops])
+        kpad = (C.c_int64 * P)(*[p.k_pad for p in pops])
+        ns = (C.c_int32 * P)(*[p.n_samples for p in pops])
+        L.call("pgld_band_sums", self._h, fam, P, planes, margs, kpad, ns, int(bool(with_missing)), sset,
+               L.ptr(sums), L.ptr(kp), L.stream_handle())
+        return (sums, kp) if want_pairs else sums
+
+    def counts(self, pop, with_missing=None):
+        """Per-pair counts of one population through the MFMA path, in the
+        band's (i, j) pair order: int32 (n_pairs, 4) or (n_pairs, 9)."""
+        if with_missing is None:
+            with_missing = pop.has_missing
+        nc = 4 if pop.family == L.FAMILY_HAP else 9
+        out = L.empty((self.n_pairs, nc), np.int32)
+        L.call("pgld_band_counts", self._h, pop.family, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad,
+               pop.n_samples, int(bool(with_missing)), None, L.ptr(out), L.stream_handle())
+        return out

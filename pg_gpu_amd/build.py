@@ -1,0 +1,59 @@
+"""Build libpgld.so (HIP, gfx950) in-tree.
+
+    python -m pg_gpu_amd.build        # or __graft_entry__.build()
+
+Runs the two code generators under csrc/, compiles every csrc/*.hip to an
+object with hipcc in parallel, and links pg_gpu_amd/libpgld.so.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libpgld.so")
+BUILD_DIR = os.path.join(os.path.dirname(HERE), "_build")
+ARCH = os.environ.get("PGLD_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 ops are kept out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    lines = [ln for ln in r.stdout.splitlines() if "packed-fp32-ops" not in ln]
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), "\n".join(lines)))
+    return "\n".join(lines)
+
+
+def build(verbose=False):
+    for gen in ("gen_geno_poly.py", "gen_epilogue.py"):
+        _run([sys.executable, os.path.join(CSRC, gen)])
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    objs = [os.path.join(BUILD_DIR, f[:-4] + ".o") for f in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_run, [HIPCC, *FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+                for s, o in zip(srcs, objs)]
+        for f in futs:
+            out = f.result()
+            if verbose and out:
+                print(out)
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

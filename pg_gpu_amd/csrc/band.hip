@@ -1,0 +1,902 @@
+// Tiled int8-MFMA path for banded LD sums.
+//
+// Counts for a 64 x 64 tile of SNP pairs are one dense contraction over the
+// samples of a population: with 0/1 indicator planes (haplotypes: h, v;
+// genotypes: v, e1, e2) stored SNP-major and sample-contiguous, every product
+// of an A-side plane (SNPs i) and a B-side plane (SNPs j) is an int8 GEMM with
+// exact int32 accumulation (v_mfma_i32_32x32x32_i8).  The joint counts follow
+// by inclusion-exclusion (e.g. n10 = h_i.v_j - h_i.h_j), as in the reference's
+// own GEMM form _tile_counts (ld_statistics.py:281-298).  Without missing data
+// only the allele planes enter the MFMA; the valid-plane products are the
+// per-SNP plane sums and the sample count.
+//
+// Each workgroup of 4 waves owns one tile; wave w owns the 32 x 32 sub-tile
+// (rows 32*(w/2), cols 32*(w%2)).  Populations are processed one after the
+// other; all but the last park their counts in LDS as int16.  The epilogue then
+// evaluates the statistics of each band pair in FP64 from exact counts, keys
+// the pair (window of the left SNP, or distance bin), and reduces per key in a
+// fixed order: per-lane running sums, wave butterflies at key changes, one LDS
+// row per wave, a fixed-order sum over the 4 waves into a per-tile partial
+// row, and a fixed-order per-key reduction over tiles.  Results are therefore
+// identical from run to run.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "ld_math.h"
+#include "../../include/pgld.h"
+
+namespace pgld {
+#include "epilogue.inc"
+}
+
+struct pgld_band_plan {
+  int64_t n_var = 0, n_pairs = 0, n_tiles = 0, n_slots = 0;
+  double max_dist = 0.0, origin = 0.0, width = 1.0;
+  int32_t key_mode = 0, n_keys = 0, max_tile_slots = 0;
+  double* d_pos = nullptr;       // band coordinate
+  double* d_kc = nullptr;        // key coordinate
+  double* d_edges = nullptr;     // n_keys + 1 (bin modes)
+  void* d_tiles = nullptr;       // BandTile[n_tiles]
+  int64_t* d_pair_base = nullptr;  // exclusive prefix of per-variant pair counts
+  int32_t* d_key_off = nullptr;    // n_keys + 1
+  int64_t* d_key_list = nullptr;   // slots grouped by key, ascending tile order
+  double* d_partials = nullptr;    // workspace
+  size_t partials_cap = 0;         // doubles
+};
+
+namespace pgld {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int BT = 64;       // tile edge (SNPs)
+constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
+constexpr int BAND_THREADS = 256;
+constexpr int RED_LDS_BYTES = 64 * 1024;
+
+struct BandTile {
+  int32_t i0, j0, key_lo, n_slots;
+  int64_t slot_base;
+};
+
+struct PopArgs {
+  const int8_t* planes[4];
+  const int32_t* marg[4];
+  int64_t k_pad[4];
+  int32_t n_samples[4];
+};
+
+// ---------------------------------------------------------------------------
+// Plane selection per family / missing-data mode
+// ---------------------------------------------------------------------------
+template <int FAMILY, bool MISSING>
+struct Planes;
+template <>
+struct Planes<PGLD_FAMILY_HAP, true> {  // planes [h, v]; products {h, v} x {h, v}
+  static constexpr int N = 2, FIRST = 0, NPL = 2, NC = 4;
+};
+template <>
+struct Planes<PGLD_FAMILY_HAP, false> {  // h only
+  static constexpr int N = 1, FIRST = 0, NPL = 2, NC = 4;
+};
+template <>
+struct Planes<PGLD_FAMILY_GENO, true> {  // planes [v, e1, e2]; all 9 products
+  static constexpr int N = 3, FIRST = 0, NPL = 3, NC = 9;
+};
+template <>
+struct Planes<PGLD_FAMILY_GENO, false> {  // e1, e2 only
+  static constexpr int N = 2, FIRST = 1, NPL = 3, NC = 9;
+};
+
+template <int N>
+struct WaveAcc {
+  v16i c[N][N];
+};
+
+// One wave: products of its 32 A-rows and 32 B-cols over all K of one
+// population.  Lane l reads 64 contiguous bytes of row (l % 32) at K offset
+// 64 * (l / 32) per 128-byte step; A and B use the same lane -> K assignment,
+// so every K byte is paired with itself exactly once.
+template <int FAMILY, bool MISSING>
+__device__ __forceinline__ void wave_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
+                                              int ri, int rj, WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
+  using PL = Planes<FAMILY, MISSING>;
+  constexpr int N = PL::N;
+  const int lane = threadIdx.x & 63;
+  const int64_t si = min((int64_t)(ri + (lane & 31)), n_var - 1);
+  const int64_t sj = min((int64_t)(rj + (lane & 31)), n_var - 1);
+  const int64_t kq = (lane >> 5) * 64;
+  const int8_t* pa[N];
+  const int8_t* pb[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    pa[a] = planes + ((int64_t)(PL::FIRST + a) * n_var + si) * k_pad + kq;
+    pb[a] = planes + ((int64_t)(PL::FIRST + a) * n_var + sj) * k_pad + kq;
+  }
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+  for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
+    v4i A[N][4], B[N][4];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + k0 + 16 * t);
+        B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + k0 + 16 * t);
+      }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int b = 0; b < N; ++b)
+          acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a][t], B[b][t], acc.c[a][b], 0, 0, 0);
+  }
+}
+
+// Joint counts of accumulator element r (row i, col j) in the reference's
+// column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
+template <int FAMILY, bool MISSING>
+__device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING>::N>& acc, int r,
+                                            const int32_t* __restrict__ marg, int64_t n_var, int64_t i, int64_t j,
+                                            int32_t S, int* out) {
+  if constexpr (FAMILY == PGLD_FAMILY_HAP) {
+    int hh, hv, vh, vv;
+    hh = acc.c[0][0][r];
+    if constexpr (MISSING) {
+      hv = acc.c[0][1][r];
+      vh = acc.c[1][0][r];
+      vv = acc.c[1][1][r];
+    } else {
+      hv = marg[i];  // plane 0 (h) sums
+      vh = marg[j];
+      vv = S;
+    }
+    out[0] = hh;
+    out[1] = hv - hh;
+    out[2] = vh - hh;
+    out[3] = vv - hv - vh + hh;
+  } else {
+    int N[9];  // N[3a+b], a/b: 0 = valid, 1 = e1, 2 = e2
+    if constexpr (MISSING) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) N[3 * a + b] = acc.c[a][b][r];
+    } else {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) N[3 * (a + 1) + (b + 1)] = acc.c[a][b][r];
+      N[0] = S;
+      N[1] = marg[n_var + j];      // e1 sums
+      N[2] = marg[2 * n_var + j];  // e2 sums
+      N[3] = marg[n_var + i];
+      N[6] = marg[2 * n_var + i];
+    }
+    int n11 = N[4], n12 = N[5], n21 = N[7], n22 = N[8];
+    int n10 = N[3] - n11 - n12;
+    int n20 = N[6] - n21 - n22;
+    int n01 = N[1] - n11 - n21;
+    int n02 = N[2] - n12 - n22;
+    out[0] = N[0] - n10 - n20 - n01 - n02 - n11 - n12 - n21 - n22;
+    out[1] = n01; out[2] = n02;
+    out[3] = n10; out[4] = n11; out[5] = n12;
+    out[6] = n20; out[7] = n21; out[8] = n22;
+  }
+}
+
+__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ int32_t key_bin(double d, const double* edges, int32_t n_keys) {
+  if (!(d >= edges[0]) || !(d < edges[n_keys])) return -1;
+  int lo = 0, hi = n_keys;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (edges[mid] <= d) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+struct BandArgs {
+  const BandTile* tiles;
+  const double* pos;
+  const double* kc;
+  const double* edges;
+  int64_t n_var;
+  double max_dist, origin, width;
+  int32_t key_mode, n_keys;
+  double* partials;
+  int32_t* counts_out;       // counts mode
+  const int64_t* pair_base;  // counts mode
+};
+
+template <int FAMILY, int P, int SET>
+struct StatSet;
+#define PGLD_STATSET(FAM, FAMC, P, SET, SETC, NAME)                                     \
+  template <>                                                                           \
+  struct StatSet<FAMC, P, SETC> {                                                       \
+    static constexpr int N = EPI_##FAM##_##NAME##_##P##_N;                              \
+    template <class T>                                                                  \
+    __device__ static void eval(const T* v, double* out) {                              \
+      epi_##SET##_##P(v, out);                                                          \
+    }                                                                                   \
+  };
+#define PGLD_STATSETS(P)                                                                           \
+  template <> struct StatSet<PGLD_FAMILY_HAP, P, PGLD_SET_MOMENTS> {                               \
+    static constexpr int N = EPI_HAP_MOMENTS_##P##_N;                                              \
+    __device__ static void eval(const HapP* v, double* out) { epi_hap_moments_##P(v, out); }       \
+  };                                                                                               \
+  template <> struct StatSet<PGLD_FAMILY_HAP, P, PGLD_SET_SELF> {                                  \
+    static constexpr int N = EPI_HAP_SELF_##P##_N;                                                 \
+    __device__ static void eval(const HapP* v, double* out) { epi_hap_self_##P(v, out); }          \
+  };                                                                                               \
+  template <> struct StatSet<PGLD_FAMILY_GENO, P, PGLD_SET_MOMENTS> {                              \
+    static constexpr int N = EPI_GENO_MOMENTS_##P##_N;                                             \
+    __device__ static void eval(const GenoP* v, double* out) { epi_geno_moments_##P(v, out); }     \
+  };                                                                                               \
+  template <> struct StatSet<PGLD_FAMILY_GENO, P, PGLD_SET_SELF> {                                 \
+    static constexpr int N = EPI_GENO_SELF_##P##_N;                                                \
+    __device__ static void eval(const GenoP* v, double* out) { epi_geno_self_##P(v, out); }        \
+  };
+PGLD_STATSETS(1)
+PGLD_STATSETS(2)
+PGLD_STATSETS(3)
+PGLD_STATSETS(4)
+#undef PGLD_STATSETS
+#undef PGLD_STATSET
+
+template <int FAMILY>
+struct PopVal;
+template <>
+struct PopVal<PGLD_FAMILY_HAP> {
+  using T = HapP;
+  __device__ static T make(const int* c) { return hap_pop((double)c[0], (double)c[1], (double)c[2], (double)c[3]); }
+};
+template <>
+struct PopVal<PGLD_FAMILY_GENO> {
+  using T = GenoP;
+  __device__ static T make(const int* c) {
+    double d[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) d[t] = (double)c[t];
+    return geno_pop(d);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Fused band kernel: counts for all populations, statistics, keyed sums.
+// ---------------------------------------------------------------------------
+template <int FAMILY, bool MISSING, int P, int SET>
+__global__ __launch_bounds__(BAND_THREADS) void k_band_sums(BandArgs ba, PopArgs pa) {
+  using PL = Planes<FAMILY, MISSING>;
+  using SS = StatSet<FAMILY, P, SET>;
+  constexpr int NC = PL::NC;
+  constexpr int NS = SS::N + 1;  // statistics + pair count
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  // layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16]
+  __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
+
+  const BandTile tile = ba.tiles[blockIdx.x];
+  if (tile.n_slots <= 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double* red = reinterpret_cast<double*>(smem);
+  const int red_n = 4 * tile.n_slots * NS;
+  int16_t* stash = reinterpret_cast<int16_t*>(smem + ((red_n * 8 + 15) / 16) * 16);
+
+  for (int t = tid; t < red_n; t += BAND_THREADS) red[t] = 0.0;
+  if (tid < BT) {
+    int64_t gi = min((int64_t)tile.i0 + tid, ba.n_var - 1), gj = min((int64_t)tile.j0 + tid, ba.n_var - 1);
+    s_pos_i[tid] = ba.pos[gi];
+    s_pos_j[tid] = ba.pos[gj];
+    s_kc_i[tid] = ba.kc[gi];
+    s_kc_j[tid] = ba.kc[gj];
+  }
+  __syncthreads();
+
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int ri = tile.i0 + wr, rj = tile.j0 + wc;
+  // wave-uniform skip: sub-tile entirely on/below the diagonal, past the end,
+  // or farther apart than max_dist
+  const int64_t last_i = min((int64_t)ri + 31, ba.n_var - 1);
+  bool wave_live = (int64_t)rj < ba.n_var && rj + 31 > ri && (int64_t)ri < ba.n_var &&
+                   s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ba.max_dist;
+
+  WaveAcc<PL::N> acc;
+  if (wave_live) {
+#pragma unroll 1
+    for (int p = 0; p < P - 1; ++p) {
+      wave_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], ri, rj, acc);
+      const int64_t j = rj + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = ri + acc_row(r, lane);
+        int c[NC];
+        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(j, ba.n_var - 1),
+                                     pa.n_samples[p], c);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
+      }
+    }
+    wave_products<FAMILY, MISSING>(pa.planes[P - 1], ba.n_var, pa.k_pad[P - 1], ri, rj, acc);
+  }
+
+  // ---- epilogue: statistics and keyed running sums ----
+  double lacc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) lacc[s] = 0.0;
+  int ck = -1;
+  const int jl = wc + (lane & 31);
+  const int64_t j = (int64_t)tile.j0 + jl;
+  double* wred = red + w * tile.n_slots * NS;
+
+  auto flush = [&](bool flag) {
+    unsigned long long part = __ballot(flag);
+    while (part) {
+      const int leader = __ffsll((long long)part) - 1;
+      const int kk = __shfl(ck, leader);
+      const bool mine = flag && ck == kk;
+      part &= ~__ballot(mine);
+      const int slot = kk - tile.key_lo;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        double x = wave_sum(mine ? lacc[s] : 0.0);
+        if (lane == 0 && slot >= 0 && slot < tile.n_slots) wred[slot * NS + s] += x;
+      }
+    }
+    if (flag) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) lacc[s] = 0.0;
+      ck = -1;
+    }
+  };
+
+  if (wave_live) {
+#pragma unroll 1
+    for (int r = 0; r < 16; ++r) {
+      const int il = wr + acc_row(r, lane);
+      const int64_t i = (int64_t)tile.i0 + il;
+      bool valid = i < j && j < ba.n_var && (s_pos_j[jl] - s_pos_i[il]) <= ba.max_dist;
+      int key = -1;
+      if (valid) {
+        if (ba.key_mode == PGLD_KEY_WINDOW) {
+          double wk = floor((s_kc_i[il] - ba.origin) / ba.width);
+          key = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
+        } else {
+          double d = s_kc_j[jl] - s_kc_i[il];
+          if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
+          key = key_bin(d, ba.edges, ba.n_keys);
+        }
+      }
+      const bool has = valid && key >= 0;
+      const bool need = has && ck >= 0 && key != ck;
+      if (__ballot(need)) flush(need);
+      if (has) {
+        ck = key;
+        typename PopVal<FAMILY>::T V[P];
+#pragma unroll
+        for (int p = 0; p < P - 1; ++p) {
+          int c[NC];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
+          V[p] = PopVal<FAMILY>::make(c);
+        }
+        {
+          int c[NC];
+          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
+          V[P - 1] = PopVal<FAMILY>::make(c);
+        }
+        double out[SS::N];
+        SS::eval(V, out);
+#pragma unroll
+        for (int s = 0; s < SS::N; ++s) lacc[s] += out[s];
+        lacc[NS - 1] += 1.0;
+      }
+    }
+    flush(ck >= 0);
+  }
+  __syncthreads();
+  // fixed-order sum over the four wave rows into this tile's partial row
+  const int row_n = tile.n_slots * NS;
+  double* dst = ba.partials + tile.slot_base * NS;
+  for (int t = tid; t < row_n; t += BAND_THREADS)
+    dst[t] = ((red[t] + red[row_n + t]) + red[2 * row_n + t]) + red[3 * row_n + t];
+}
+
+// Counts mode: one population, per-pair counts in the band's (i, j) order.
+template <int FAMILY, bool MISSING>
+__global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopArgs pa) {
+  using PL = Planes<FAMILY, MISSING>;
+  constexpr int NC = PL::NC;
+  const BandTile tile = ba.tiles[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int ri = tile.i0 + wr, rj = tile.j0 + wc;
+  if ((int64_t)rj >= ba.n_var || (int64_t)ri >= ba.n_var || rj + 31 <= ri) return;
+  WaveAcc<PL::N> acc;
+  wave_products<FAMILY, MISSING>(pa.planes[0], ba.n_var, pa.k_pad[0], ri, rj, acc);
+  const int64_t j = rj + (lane & 31);
+  if (j >= ba.n_var) return;
+  const double pj = ba.pos[j];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int64_t i = ri + acc_row(r, lane);
+    if (i < j && pj - ba.pos[i] <= ba.max_dist) {
+      int c[NC];
+      elem_counts<FAMILY, MISSING>(acc, r, pa.marg[0], ba.n_var, i, j, pa.n_samples[0], c);
+      int32_t* row = ba.counts_out + (ba.pair_base[i] + (j - i - 1)) * NC;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) row[q] = c[q];
+    }
+  }
+}
+
+// Per-key reduction over tile partial rows, in ascending tile order.
+__global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ partials,
+                                                     const int32_t* __restrict__ key_off,
+                                                     const int64_t* __restrict__ key_list, int32_t ns_total,
+                                                     int32_t n_stats, double* __restrict__ sums,
+                                                     double* __restrict__ key_pairs) {
+  __shared__ double buf[256];
+  const int k = blockIdx.x;
+  const int lo = key_off[k], hi = key_off[k + 1];
+  for (int s = 0; s < ns_total; ++s) {
+    double x = 0.0;
+    for (int e = lo + threadIdx.x; e < hi; e += 256) x += partials[key_list[e] * ns_total + s];
+    buf[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (s < n_stats) sums[(int64_t)k * n_stats + s] = buf[0];
+      else if (key_pairs) key_pairs[k] = buf[0];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plane packing: (rows x variants) int8 -> [plane][variant][k_pad] 0/1 bytes
+// ---------------------------------------------------------------------------
+// Tile of 128 samples x 64 variants per block: rows are read as dwords
+// (4 variants) when the row stride allows, transposed through LDS, and each
+// (plane, variant) gets one full 128-byte line of sample bytes.
+constexpr int PK_S = 128, PK_V = 64;
+
+template <int FAMILY, bool WORDS>
+__global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
+                                              const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
+                                              int8_t* __restrict__ planes, int32_t* __restrict__ marg,
+                                              int32_t* __restrict__ flags) {
+  constexpr int NPL = FAMILY == PGLD_FAMILY_HAP ? 2 : 3;
+  constexpr int maxv = FAMILY == PGLD_FAMILY_HAP ? 1 : 2;
+  __shared__ uint32_t tile[PK_S][PK_V / 4 + 1];  // [sample][variant quad], padded
+  __shared__ int msum[NPL][PK_V];
+  const int tid = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * PK_V, k0 = (int64_t)blockIdx.y * PK_S;
+  if (tid < NPL * PK_V) msum[tid / PK_V][tid % PK_V] = 0;
+  int any_neg = 0, any_bad = 0;
+  // 16 threads per row (one dword = 4 variants each), 16 rows per pass
+  const int q = tid & 15, rr = tid >> 4;
+  for (int s = rr; s < PK_S; s += 16) {
+    uint32_t word = 0xffffffffu;  // padding samples read as missing
+    if (k0 + s < n_sel) {
+      const int64_t base = (int64_t)(rows ? rows[k0 + s] : k0 + s) * n_var + v0 + 4 * q;
+      if (WORDS && v0 + 4 * q + 3 < n_var) {
+        word = *reinterpret_cast<const uint32_t*>(data + base);
+      } else {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          uint32_t byte = 0xffu;
+          if (v0 + 4 * q + b < n_var) byte = (uint8_t)data[base + b];
+          word = (word & ~(0xffu << (8 * b))) | (byte << (8 * b));
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (v0 + 4 * q + b < n_var) {
+          const int x = (int8_t)(word >> (8 * b));
+          any_neg |= x < 0;
+          any_bad |= (x < -1) | (x > maxv);
+        }
+      }
+    }
+    tile[s][q] = word;
+  }
+  __syncthreads();
+  // 8 lanes per (variant): each builds 16 sample bytes of every plane
+  for (int task = tid; task < PK_V * 8; task += 256) {
+    const int v = task >> 3, ch = (task & 7) * 16;
+    if (v0 + v >= n_var) continue;
+    uint32_t words[NPL][4];
+    int cnt[NPL];
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+      cnt[p] = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) words[p][t] = 0;
+    }
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const int x = (int8_t)(tile[ch + b][v >> 2] >> (8 * (v & 3)));
+      uint32_t bits[NPL];
+      if constexpr (FAMILY == PGLD_FAMILY_HAP) {
+        bits[0] = x == 1;
+        bits[1] = x >= 0;
+      } else {
+        bits[0] = x >= 0;
+        bits[1] = x == 1;
+        bits[2] = x == 2;
+      }
+#pragma unroll
+      for (int p = 0; p < NPL; ++p) {
+        words[p][b >> 2] |= bits[p] << (8 * (b & 3));
+        cnt[p] += bits[p];
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+      int4 val;
+      val.x = (int)words[p][0];
+      val.y = (int)words[p][1];
+      val.z = (int)words[p][2];
+      val.w = (int)words[p][3];
+      *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + ch) = val;
+      if (cnt[p]) atomicAdd(&msum[p][v], cnt[p]);
+    }
+  }
+  __syncthreads();
+  if (tid < NPL * PK_V) {
+    const int p = tid / PK_V, vv = tid % PK_V;
+    if (v0 + vv < n_var && msum[p][vv]) atomicAdd(&marg[(int64_t)p * n_var + v0 + vv], msum[p][vv]);
+  }
+  unsigned long long mneg = __ballot(any_neg), mbad = __ballot(any_bad);
+  if ((tid & 63) == 0) {
+    if (mneg) atomicOr(&flags[0], 1);
+    if (mbad) atomicOr(&flags[1], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int64_t upper_index(const double* pos, int64_t n, int64_t from, double ub) {
+  return std::upper_bound(pos + from, pos + n, ub) - pos;
+}
+
+static int bin_host(double d, const std::vector<double>& e) {
+  int n = (int)e.size() - 1;
+  if (!(d >= e[0]) || !(d < e[n])) return -1;
+  return (int)(std::upper_bound(e.begin(), e.end(), d) - e.begin()) - 1;
+}
+
+template <int FAMILY, bool MISSING, int P, int SET>
+static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)plan->n_tiles), dim3(BAND_THREADS), lds,
+                     s, ba, pa);
+}
+
+typedef void (*SumsLauncher)(const pgld_band_plan*, const BandArgs&, const PopArgs&, size_t, hipStream_t);
+
+template <int FAMILY, bool MISSING>
+static SumsLauncher pick_launcher(int P, int set) {
+  if constexpr (FAMILY == PGLD_FAMILY_HAP) {
+    if (set == PGLD_SET_MOMENTS) {
+      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_MOMENTS>;
+    } else {
+      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_SELF>;
+      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_SELF>;
+    }
+  } else {
+    if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
+    if (P == 2) {
+      if (set == PGLD_SET_MOMENTS) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS>;
+      return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF>;
+    }
+  }
+  return nullptr;
+}
+
+static int n_stats_of(int family, int P, int set) {
+  if (set == PGLD_SET_SELF && P > 1) return 3 * P;
+  return P == 1 ? 3 : (P == 2 ? 15 : (P == 3 ? 45 : 105));
+}
+
+}  // namespace pgld
+
+using namespace pgld;
+
+extern "C" {
+
+int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                     int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream) {
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_pack_planes: bad family");
+  if (!rows) n_sel = n_rows;
+  PGLD_REQUIRE(n_var >= 0 && n_sel >= 0, "pgld_pack_planes: negative size");
+  PGLD_REQUIRE(k_pad % 128 == 0 && k_pad >= n_sel, "pgld_pack_planes: k_pad must be a multiple of 128 >= n_sel");
+  PGLD_REQUIRE(n_sel <= 32767, "pgld_pack_planes: at most 32767 samples per population");
+  PGLD_REQUIRE(data && planes && marg && flags, "pgld_pack_planes: null pointer");
+  const int npl = family == PGLD_FAMILY_HAP ? 2 : 3;
+  hipStream_t s = as_stream(stream);
+  PGLD_TRY_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s), "pgld_pack_planes memset");
+  if (n_var == 0) return PGLD_OK;
+  PGLD_TRY_HIP(hipMemsetAsync(marg, 0, (size_t)npl * n_var * sizeof(int32_t), s), "pgld_pack_planes memset");
+  if (k_pad == 0) return PGLD_OK;
+  dim3 g((unsigned)ceil_div(n_var, PK_V), (unsigned)(k_pad / PK_S));
+  const bool words = (n_var % 4 == 0) && (reinterpret_cast<uintptr_t>(data) % 4 == 0);
+#define PGLD_PACK(F, W) \
+  hipLaunchKernelGGL((k_pack<F, W>), g, dim3(256), 0, s, data, n_var, rows, n_sel, k_pad, planes, marg, flags)
+  if (family == PGLD_FAMILY_HAP) {
+    if (words) PGLD_PACK(PGLD_FAMILY_HAP, true); else PGLD_PACK(PGLD_FAMILY_HAP, false);
+  } else {
+    if (words) PGLD_PACK(PGLD_FAMILY_GENO, true); else PGLD_PACK(PGLD_FAMILY_GENO, false);
+  }
+#undef PGLD_PACK
+  return check_launch("k_pack");
+}
+
+int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int32_t key_mode,
+                          const double* key_coord, const double* edges, int32_t n_keys, double origin, double width,
+                          pgld_band_plan** out) {
+  PGLD_REQUIRE(out, "pgld_band_plan_create: null output");
+  *out = nullptr;
+  PGLD_REQUIRE(n_var >= 0 && n_var < 0x7fffff00, "pgld_band_plan_create: bad n_var");
+  PGLD_REQUIRE(n_var == 0 || pos, "pgld_band_plan_create: null pos");
+  PGLD_REQUIRE(max_dist >= 0.0, "pgld_band_plan_create: negative max_dist");
+  PGLD_REQUIRE(key_mode == PGLD_KEY_BIN_ABS || key_mode == PGLD_KEY_BIN || key_mode == PGLD_KEY_WINDOW,
+               "pgld_band_plan_create: unknown key mode %d", key_mode);
+  PGLD_REQUIRE(n_keys >= 1, "pgld_band_plan_create: n_keys must be >= 1");
+  PGLD_REQUIRE(key_mode == PGLD_KEY_WINDOW ? width > 0.0 : edges != nullptr,
+               "pgld_band_plan_create: windows need width > 0, bins need edges");
+  const double* kc = key_coord ? key_coord : pos;
+  for (int64_t v = 1; v < n_var; ++v) {
+    PGLD_REQUIRE(pos[v] >= pos[v - 1], "pgld_band_plan_create: positions not sorted at %lld", (long long)v);
+    PGLD_REQUIRE(kc[v] >= kc[v - 1], "pgld_band_plan_create: key coordinate decreases at %lld", (long long)v);
+  }
+  std::vector<double> e;
+  if (key_mode != PGLD_KEY_WINDOW) {
+    e.assign(edges, edges + n_keys + 1);
+    for (int k = 0; k < n_keys; ++k)
+      PGLD_REQUIRE(e[k] < e[k + 1], "pgld_band_plan_create: edges must increase");
+  }
+  auto* plan = new pgld_band_plan();
+  plan->n_var = n_var;
+  plan->max_dist = max_dist;
+  plan->key_mode = key_mode;
+  plan->n_keys = n_keys;
+  plan->origin = origin;
+  plan->width = width;
+
+  std::vector<int64_t> per(n_var), base(n_var + 1, 0);
+  for (int64_t v = 0; v < n_var; ++v) {
+    int64_t u = upper_index(pos, n_var, v, pos[v] + max_dist);
+    per[v] = std::max<int64_t>(0, u - v - 1);
+    base[v + 1] = base[v] + per[v];
+  }
+  plan->n_pairs = base[n_var];
+
+  auto key_i = [&](int64_t i) -> int64_t {  // window key of the left SNP
+    return (int64_t)std::floor((kc[i] - origin) / width);
+  };
+  std::vector<BandTile> tiles;
+  std::vector<std::pair<int32_t, int64_t>> key_slot;  // (key, slot) in tile order
+  int64_t n_slots = 0;
+  int max_ts = 0;
+  const int64_t n_rows = ceil_div(n_var, BT);
+  for (int64_t r = 0; r < n_rows; ++r) {
+    const int64_t i0 = r * BT, il = std::min(i0 + BT - 1, n_var - 1);
+    if (base[il + 1] - base[i0] == 0) continue;
+    const int64_t jhi = std::max(il, il + per[il]);
+    for (int64_t tc = r; tc * BT <= jhi; ++tc) {
+      const int64_t j0 = tc * BT, jl = std::min(j0 + BT - 1, n_var - 1);
+      int64_t klo, khi;
+      if (key_mode == PGLD_KEY_WINDOW) {
+        klo = std::max<int64_t>(0, key_i(i0));
+        khi = std::min<int64_t>(n_keys - 1, key_i(il));
+      } else {
+        double dmin = std::max(0.0, kc[j0] - kc[il]);
+        double dmax = kc[jl] - kc[i0];
+        if (dmax < e[0] || dmin >= e[n_keys]) {
+          klo = 1;
+          khi = 0;
+        } else {
+          klo = dmin < e[0] ? 0 : bin_host(dmin, e);
+          khi = dmax >= e[n_keys] ? n_keys - 1 : bin_host(dmax, e);
+        }
+      }
+      BandTile t;
+      t.i0 = (int32_t)i0;
+      t.j0 = (int32_t)j0;
+      t.key_lo = (int32_t)klo;
+      t.n_slots = khi >= klo ? (int32_t)(khi - klo + 1) : 0;
+      t.slot_base = n_slots;
+      for (int q = 0; q < t.n_slots; ++q) key_slot.push_back({(int32_t)(klo + q), n_slots + q});
+      n_slots += t.n_slots;
+      max_ts = std::max(max_ts, t.n_slots);
+      tiles.push_back(t);
+    }
+  }
+  plan->n_tiles = (int64_t)tiles.size();
+  plan->n_slots = n_slots;
+  plan->max_tile_slots = max_ts;
+  std::vector<int32_t> key_off(n_keys + 1, 0);
+  for (auto& ks : key_slot) key_off[ks.first + 1]++;
+  for (int k = 0; k < n_keys; ++k) key_off[k + 1] += key_off[k];
+  std::vector<int64_t> key_list(key_slot.size());
+  {
+    std::vector<int32_t> fill(key_off.begin(), key_off.end() - 1);
+    for (auto& ks : key_slot) key_list[fill[ks.first]++] = ks.second;
+  }
+
+  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) bytes = 8;
+    hipError_t err = hipMalloc(dst, bytes);
+    if (err == hipSuccess && src) err = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    return check_hip(err, "pgld_band_plan_create upload");
+  };
+  int rc = PGLD_OK;
+  std::vector<double> kcv(kc, kc + n_var);
+  if (!rc) rc = up((void**)&plan->d_pos, pos, n_var * sizeof(double));
+  if (!rc) rc = up((void**)&plan->d_kc, kcv.data(), n_var * sizeof(double));
+  if (!rc) rc = up((void**)&plan->d_edges, e.empty() ? nullptr : e.data(), e.size() * sizeof(double));
+  if (!rc) rc = up(&plan->d_tiles, tiles.data(), tiles.size() * sizeof(BandTile));
+  if (!rc) rc = up((void**)&plan->d_pair_base, base.data(), n_var * sizeof(int64_t));
+  if (!rc) rc = up((void**)&plan->d_key_off, key_off.data(), key_off.size() * sizeof(int32_t));
+  if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
+                   key_list.size() * sizeof(int64_t));
+  if (rc) {
+    pgld_band_plan_destroy(plan);
+    return rc;
+  }
+  *out = plan;
+  return PGLD_OK;
+}
+
+int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n_pairs, int64_t* n_slots) {
+  PGLD_REQUIRE(plan, "pgld_band_plan_info: null plan");
+  if (n_tiles) *n_tiles = plan->n_tiles;
+  if (n_pairs) *n_pairs = plan->n_pairs;
+  if (n_slots) *n_slots = plan->n_slots;
+  return PGLD_OK;
+}
+
+int pgld_band_plan_destroy(pgld_band_plan* plan) {
+  if (!plan) return PGLD_OK;
+  void* ptrs[] = {plan->d_pos, plan->d_kc, plan->d_edges, plan->d_tiles, plan->d_pair_base,
+                  plan->d_key_off, plan->d_key_list, plan->d_partials};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete plan;
+  return PGLD_OK;
+}
+
+static int fill_pop_args(int32_t n_pops, const int8_t* const* planes, const int32_t* const* marg,
+                         const int64_t* k_pad, const int32_t* n_samples, PopArgs& pa) {
+  pa = PopArgs{};
+  for (int p = 0; p < n_pops; ++p) {
+    PGLD_REQUIRE(planes[p] && marg[p], "band: null planes/marg for pop %d", p);
+    PGLD_REQUIRE(k_pad[p] % KSTEP == 0 && k_pad[p] >= n_samples[p] && n_samples[p] >= 0,
+                 "band: k_pad must be a multiple of 128 >= n_samples (pop %d)", p);
+    PGLD_REQUIRE(n_samples[p] <= 32767, "band: at most 32767 samples per population");
+    pa.planes[p] = planes[p];
+    pa.marg[p] = marg[p];
+    pa.k_pad[p] = k_pad[p];
+    pa.n_samples[p] = n_samples[p];
+  }
+  return PGLD_OK;
+}
+
+int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, const int8_t* const* planes,
+                   const int32_t* const* marg, const int64_t* k_pad, const int32_t* n_samples, int32_t with_missing,
+                   int32_t stat_set, double* sums, double* key_pairs, void* stream) {
+  PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples && sums, "pgld_band_sums: null pointer");
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_sums: bad family");
+  PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_band_sums: n_pops must be 1..4");
+  PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_band_sums: bad stat_set");
+  PopArgs pa;
+  int rc = fill_pop_args(n_pops, planes, marg, k_pad, n_samples, pa);
+  if (rc) return rc;
+  SumsLauncher L = nullptr;
+  const bool miss = with_missing != 0;
+  if (family == PGLD_FAMILY_HAP)
+    L = miss ? pick_launcher<PGLD_FAMILY_HAP, true>(n_pops, stat_set)
+             : pick_launcher<PGLD_FAMILY_HAP, false>(n_pops, stat_set);
+  else
+    L = miss ? pick_launcher<PGLD_FAMILY_GENO, true>(n_pops, stat_set)
+             : pick_launcher<PGLD_FAMILY_GENO, false>(n_pops, stat_set);
+  if (!L) {
+    set_error("pgld_band_sums: no fused kernel for family %d with %d populations and stat set %d", family, n_pops,
+              stat_set);
+    return PGLD_EUNSUPPORTED;
+  }
+  const int n_stats = n_stats_of(family, n_pops, stat_set);
+  const int ns_total = n_stats + 1;
+  const int nc = family == PGLD_FAMILY_HAP ? 4 : 9;
+  const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
+  const size_t stash_bytes = (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
+  if (red_bytes > (size_t)RED_LDS_BYTES || red_bytes + stash_bytes > 150 * 1024) {
+    set_error("pgld_band_sums: a tile touches %d keys; %d statistics x that many keys exceed LDS",
+              plan->max_tile_slots, n_stats);
+    return PGLD_EUNSUPPORTED;
+  }
+  hipStream_t s = as_stream(stream);
+  auto* pl = const_cast<pgld_band_plan*>(plan);
+  const size_t need = (size_t)std::max<int64_t>(1, plan->n_slots) * ns_total;
+  if (pl->partials_cap < need) {
+    if (pl->d_partials) PGLD_TRY_HIP(hipFree(pl->d_partials), "pgld_band_sums free");
+    pl->d_partials = nullptr;
+    pl->partials_cap = 0;
+    PGLD_TRY_HIP(hipMalloc((void**)&pl->d_partials, need * sizeof(double)), "pgld_band_sums workspace");
+    pl->partials_cap = need;
+  }
+  PGLD_TRY_HIP(hipMemsetAsync(sums, 0, (size_t)plan->n_keys * n_stats * sizeof(double), s), "pgld_band_sums memset");
+  if (key_pairs) PGLD_TRY_HIP(hipMemsetAsync(key_pairs, 0, plan->n_keys * sizeof(double), s), "memset");
+  if (plan->n_tiles == 0) return PGLD_OK;
+  BandArgs ba{};
+  ba.tiles = reinterpret_cast<const BandTile*>(plan->d_tiles);
+  ba.pos = plan->d_pos;
+  ba.kc = plan->d_kc;
+  ba.edges = plan->d_edges;
+  ba.n_var = plan->n_var;
+  ba.max_dist = plan->max_dist;
+  ba.origin = plan->origin;
+  ba.width = plan->width;
+  ba.key_mode = plan->key_mode;
+  ba.n_keys = plan->n_keys;
+  ba.partials = pl->d_partials;
+  L(plan, ba, pa, red_bytes + stash_bytes, s);
+  rc = check_launch("k_band_sums");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)plan->n_keys), dim3(256), 0, s, pl->d_partials, plan->d_key_off,
+                     plan->d_key_list, ns_total, n_stats, sums, key_pairs);
+  return check_launch("k_band_reduce");
+}
+
+int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes, const int32_t* marg,
+                     int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
+                     int32_t* counts, void* stream) {
+  PGLD_REQUIRE(plan && planes && marg && counts, "pgld_band_counts: null pointer");
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_counts: bad family");
+  PopArgs pa;
+  int rc = fill_pop_args(1, &planes, &marg, &k_pad, &n_samples, pa);
+  if (rc) return rc;
+  if (plan->n_tiles == 0) return PGLD_OK;
+  BandArgs ba{};
+  ba.tiles = reinterpret_cast<const BandTile*>(plan->d_tiles);
+  ba.pos = plan->d_pos;
+  ba.n_var = plan->n_var;
+  ba.max_dist = plan->max_dist;
+  ba.counts_out = counts;
+  ba.pair_base = pair_base ? pair_base : plan->d_pair_base;
+  dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
+  hipStream_t s = as_stream(stream);
+  const bool miss = with_missing != 0;
+  if (family == PGLD_FAMILY_HAP) {
+    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, true>), g, b, 0, s, ba, pa);
+    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, false>), g, b, 0, s, ba, pa);
+  } else {
+    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, true>), g, b, 0, s, ba, pa);
+    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, false>), g, b, 0, s, ba, pa);
+  }
+  return check_launch("k_band_counts");
+}
+
+}  // extern "C"

@@ -91,9 +91,61 @@ def pi2_case(q):
     return ("alldiff", (i, j, k, l))
 
 
+def hap_call(t, q, sym):
+    """Numerator and reciprocal-product expressions of one haplotype term."""
+    R = lambda p, k: "R[%d].i%d" % (p, k)
+    H = lambda *ps: ", ".join("H[%d]" % x for x in ps)
+    if t == "dd":
+        a, b = q
+        if a == b:
+            return "num_dd_single(%s)" % H(a), R(a, 4)
+        return "num_dd_between(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
+    if t == "dz":
+        a, b, c = q
+        if a == b == c:
+            return "num_dz_iii(%s)" % H(a), R(a, 4)
+        if a == b:
+            return "num_dz_iij(%s)" % H(a, c), "%s * %s" % (R(c, 1), R(a, 3))
+        if a == c:
+            return "num_dz_iji(%s)" % H(a, b), "%s * %s" % (R(b, 1), R(a, 3))
+        if b == c:
+            return "num_dz_ijj(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
+        return "num_dz_diff(%s)" % H(a, b, c), "%s * %s * %s" % (R(a, 2), R(b, 1), R(c, 1))
+    case, r = pi2_case(q)
+    if case == "same":
+        return "num_pi2_iiii(%s)" % H(*r), R(r[0], 4)
+    if case in ("triple_a", "triple_b"):
+        tp, sp = r
+        return "num_pi2_triple(%s)" % H(tp, sp), "%s * %s" % (R(sp, 1), R(tp, 3))
+    if case == "iikk":
+        a, k = r
+        if sym:  # symmetric in the two populations
+            a, k = min(a, k), max(a, k)
+            return "num_pi2_iikk_sym(%s)" % H(a, k), "%s * %s" % (R(min(a, k), 2), R(max(a, k), 2))
+        return "num_pi2_iikk(%s)" % H(a, k), "%s * %s" % (R(a, 2), R(k, 2))
+    if case == "iikl":
+        a, k, l = r
+        return "num_pi2_iikl(%s)" % H(a, k, l), "0.5 * %s * %s * %s" % (R(a, 2), R(k, 1), R(l, 1))
+    if case == "ijkk":
+        a, b, k = r
+        return "num_pi2_ijkk(%s)" % H(a, b, k), "0.5 * %s * %s * %s" % (R(k, 2), R(a, 1), R(b, 1))
+    if case == "ijij":
+        a, b = min(r), max(r)  # the numerator is exactly symmetric in (i, j)
+        return "num_pi2_ijij(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
+    if case == "shared":
+        s_, x, y = r
+        return "num_pi2_shared(%s)" % H(s_, x, y), "%s * %s * %s" % (R(s_, 2), R(x, 1), R(y, 1))
+    a, b, c, d = r
+    return "num_pi2_alldiff(%s)" % H(a, b, c, d), "0.25 * %s * %s * %s * %s" % (R(a, 1), R(b, 1), R(c, 1),
+                                                                              R(d, 1))
+
+
 def call_expr(fam, t, q, sym):
     V = "H" if fam == "hap" else "G"
     pre = "hap" if fam == "hap" else "geno"
+    if fam == "hap":
+        num, den = hap_call(t, q, sym)
+        return "%s * (%s)" % (num, den)
     if t == "dd":
         a, b = q
         if a == b:
@@ -115,11 +167,7 @@ def call_expr(fam, t, q, sym):
     if case == "same":
         return "%s_pi2_iiii(%s)" % (pre, args)
     if case in ("triple_a", "triple_b"):
-        if fam == "hap":
-            return "hap_pi2_triple(%s)" % args
         return "geno_pi2_%s(%s)" % ("iiij" if case == "triple_b" else "ijii", args)
-    if case == "iikk" and fam == "hap" and sym:
-        return "hap_pi2_iikk_sym(%s)" % args
     return "%s_pi2_%s(%s)" % (pre, case, args)
 
 
@@ -136,13 +184,20 @@ def emit_set(fam, P, names, sym, fname):
                 calls[e] = "t%d" % len(calls)
                 lines.append("  const double %s = %s;" % (calls[e], e))
             parts.append((w, calls[e]))
-        if len(parts) == 1 and parts[0][0] == 1.0:
-            body.append("  out[%d] = %s;" % (s, parts[0][1]))
+        merged = {}
+        for w, v in parts:
+            merged[v] = merged.get(v, 0.0) + w
+        items = sorted(merged.items(), key=lambda kv: int(kv[0][1:]))
+        if len(items) == 1 and items[0][1] == 1.0:
+            body.append("  out[%d] = %s;" % (s, items[0][0]))
         else:
-            expr = " + ".join("%r * %s" % (w, v) for w, v in parts)
+            expr = " + ".join("%r * %s" % (w, v) for v, w in items)
             body.append("  out[%d] = %s;" % (s, expr))
     head = "__device__ __forceinline__ void %s(const %s* %s, double* out) {" % (fname, T, V)
-    return "\n".join([head] + lines + body + ["}"]), len(names)
+    pre = []
+    if fam == "hap":
+        pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in range(P)]
+    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
 
 
 def self_names(P):

@@ -53,67 +53,57 @@ PGLD_HD double f4(double n) { return n * (n - 1.0) * (n - 2.0) * (n - 3.0); }
 PGLD_HD double f3(double n) { return n * (n - 1.0) * (n - 2.0); }
 PGLD_HD double f2(double n) { return n * (n - 1.0); }
 
-PGLD_HD double hap_dd_single(const HapP& a) {
-  double num = a.c1 * (a.c1 - 1.0) * a.c4 * (a.c4 - 1.0) + a.c2 * (a.c2 - 1.0) * a.c3 * (a.c3 - 1.0) -
-               2.0 * a.c1 * a.c2 * a.c3 * a.c4;
-  return safe_div(num, f4(a.n));
+// Numerators.  Each statistic below is num_x(...) / den_x(n...) with a
+// denominator that is a product of per-population falling factorials.
+PGLD_HD double num_dd_single(const HapP& a) {
+  return a.c1 * (a.c1 - 1.0) * a.c4 * (a.c4 - 1.0) + a.c2 * (a.c2 - 1.0) * a.c3 * (a.c3 - 1.0) -
+         2.0 * a.c1 * a.c2 * a.c3 * a.c4;
 }
-PGLD_HD double hap_dd_between(const HapP& a, const HapP& b) {
-  return safe_div(a.D * b.D, f2(a.n) * f2(b.n));
-}
-PGLD_HD double hap_dz_iii(const HapP& a) {
+PGLD_HD double num_dd_between(const HapP& a, const HapP& b) { return a.D * b.D; }
+PGLD_HD double num_dz_iii(const HapP& a) {
   double diff = a.c1 * a.c4 - a.c2 * a.c3;
-  double num = diff * (a.qA - a.pA) * (a.qB - a.pB) + diff * ((a.c2 + a.c3) - (a.c1 + a.c4)) +
-               2.0 * (a.c2 * a.c3 + a.c1 * a.c4);
-  return safe_div(num, f4(a.n));
+  return diff * (a.qA - a.pA) * (a.qB - a.pB) + diff * ((a.c2 + a.c3) - (a.c1 + a.c4)) +
+         2.0 * (a.c2 * a.c3 + a.c1 * a.c4);
 }
 // Dz(i,i,k): a = pop i, b = pop k
-PGLD_HD double hap_dz_iij(const HapP& a, const HapP& b) {
-  double num = (a.qA - a.pA) * (-a.D) * (-b.c1 + b.c2 - b.c3 + b.c4);
-  return safe_div(num, b.n * f3(a.n));
+PGLD_HD double num_dz_iij(const HapP& a, const HapP& b) {
+  return (a.qA - a.pA) * (-a.D) * (-b.c1 + b.c2 - b.c3 + b.c4);
 }
 // Dz(i,j,i): a = pop i, b = pop j
-PGLD_HD double hap_dz_iji(const HapP& a, const HapP& b) {
-  double num = (-a.c1 + a.c2 - a.c3 + a.c4) * (-a.D) * (b.qA - b.pA);
-  return safe_div(num, b.n * f3(a.n));
+PGLD_HD double num_dz_iji(const HapP& a, const HapP& b) {
+  return (-a.c1 + a.c2 - a.c3 + a.c4) * (-a.D) * (b.qA - b.pA);
 }
 // Dz(i,j,j): a = pop i, b = pop j
-PGLD_HD double hap_dz_ijj(const HapP& a, const HapP& b) {
-  double num = (-a.D) * ((-b.c1 + b.c2 + b.c3 - b.c4) + (-b.c1 + b.c2 - b.c3 + b.c4) * (b.qA - b.pA));
-  return safe_div(num, f2(a.n) * f2(b.n));
+PGLD_HD double num_dz_ijj(const HapP& a, const HapP& b) {
+  return (-a.D) * ((-b.c1 + b.c2 + b.c3 - b.c4) + (-b.c1 + b.c2 - b.c3 + b.c4) * (b.qA - b.pA));
 }
-PGLD_HD double hap_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
-  double num = -(a.D * (b.pA - b.qA) * (c.c1 - c.c2 + c.c3 - c.c4));
-  return safe_div(num, f2(a.n) * b.n * c.n);
+PGLD_HD double num_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
+  return -(a.D * (b.pA - b.qA) * (c.c1 - c.c2 + c.c3 - c.c4));
 }
-PGLD_HD double hap_pi2_iiii(const HapP& a) {
-  double num = a.pA * a.pB * a.qB * a.qA - a.c1 * a.c4 * (-1.0 + a.c1 + 3.0 * a.c2 + 3.0 * a.c3 + a.c4) -
-               a.c2 * a.c3 * (-1.0 + 3.0 * a.c1 + a.c2 + a.c3 + 3.0 * a.c4);
-  return safe_div(num, f4(a.n));
+PGLD_HD double num_pi2_iiii(const HapP& a) {
+  return a.pA * a.pB * a.qB * a.qA - a.c1 * a.c4 * (-1.0 + a.c1 + 3.0 * a.c2 + 3.0 * a.c3 + a.c4) -
+         a.c2 * a.c3 * (-1.0 + 3.0 * a.c1 + a.c2 + a.c3 + 3.0 * a.c4);
 }
 // three samples of pop t, one of pop s (haplotype_kernels.py:268-289)
-PGLD_HD double hap_pi2_triple(const HapP& t, const HapP& s) {
-  double num = (-(t.pA * t.c4 * s.pB) - (t.c2 * t.qA * s.pB) + (t.pA * (t.c2 + t.c4) * t.qA * s.pB) +
-                (t.pA * t.qA * (-2.0 * s.c2 - 2.0 * s.c4)) + (t.pA * t.c4 * s.qB) + (t.c2 * t.qA * s.qB) +
-                (t.pA * (t.c1 + t.c3) * t.qA * s.qB)) * 0.5;
-  return safe_div(num, s.n * f3(t.n));
+PGLD_HD double num_pi2_triple(const HapP& t, const HapP& s) {
+  return (-(t.pA * t.c4 * s.pB) - (t.c2 * t.qA * s.pB) + (t.pA * (t.c2 + t.c4) * t.qA * s.pB) +
+          (t.pA * t.qA * (-2.0 * s.c2 - 2.0 * s.c4)) + (t.pA * t.c4 * s.qB) + (t.c2 * t.qA * s.qB) +
+          (t.pA * (t.c1 + t.c3) * t.qA * s.qB)) * 0.5;
 }
-PGLD_HD double hap_pi2_iikk(const HapP& a, const HapP& b) {
-  return safe_div(a.pA * a.qA * (b.pB * b.qB), f2(a.n) * f2(b.n));
+PGLD_HD double num_pi2_iikk(const HapP& a, const HapP& b) { return a.pA * a.qA * (b.pB * b.qB); }
+PGLD_HD double num_pi2_iikk_sym(const HapP& a, const HapP& b) {
+  return 0.5 * (a.pA * a.qA * b.pB * b.qB + b.pA * b.qA * a.pB * a.qB);
 }
-PGLD_HD double hap_pi2_iikk_sym(const HapP& a, const HapP& b) {
-  double num = 0.5 * (a.pA * a.qA * b.pB * b.qB + b.pA * b.qA * a.pB * a.qB);
-  return safe_div(num, f2(a.n) * f2(b.n));
+// pi2(i,i,k,l), denominator 2 n_i(n_i-1) n_k n_l
+PGLD_HD double num_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
+  return a.pA * a.qA * (b.pB * c.qB + b.qB * c.pB);
 }
-// pi2(i,i,k,l)
-PGLD_HD double hap_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
-  return safe_div(a.pA * a.qA * (b.pB * c.qB + b.qB * c.pB), 2.0 * f2(a.n) * b.n * c.n);
+// pi2(i,j,k,k), denominator 2 n_k(n_k-1) n_i n_j
+PGLD_HD double num_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
+  return c.pB * c.qB * (a.pA * b.qA + a.qA * b.pA);
 }
-// pi2(i,j,k,k)
-PGLD_HD double hap_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
-  return safe_div(c.pB * c.qB * (a.pA * b.qA + a.qA * b.pA), 2.0 * f2(c.n) * a.n * b.n);
-}
-PGLD_HD double hap_pi2_ijij(const HapP& A, const HapP& B) {
+// pi2(i,j,i,j) (exactly symmetric in the two populations)
+PGLD_HD double num_pi2_ijij(const HapP& A, const HapP& B) {
   const double a1 = A.c1, a2 = A.c2, a3 = A.c3, a4 = A.c4;
   const double b1 = B.c1, b2 = B.c2, b3 = B.c3, b4 = B.c4;
   double quart = A.qB * A.qA * B.pA * B.pB + A.pB * A.qA * B.pA * B.qB + A.pA * A.qB * B.pB * B.qA +
@@ -125,10 +115,10 @@ PGLD_HD double hap_pi2_ijij(const HapP& A, const HapP& B) {
                      a2 * b3) +
                b4 * (a1 - a1 * a1 - a1 * a2 - a1 * a3 - a2 * a3 - a2 * b1 - a3 * b1 - a1 * b2 - a3 * b2 -
                      a1 * b3 - a2 * b3 - a1 * b4);
-  return safe_div((quart + cub) * 0.25, f2(A.n) * f2(B.n));
+  return (quart + cub) * 0.25;
 }
 // one population shared between the two locus pairs (haplotype_kernels.py:218-241)
-PGLD_HD double hap_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
+PGLD_HD double num_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
   const double s1 = S.c1, s2 = S.c2, s3 = S.c3, s4 = S.c4;
   const double xa = X.pA, xq = X.qA;
   const double b1 = Y.c1, b2 = Y.c2, b3 = Y.c3, b4 = Y.c4;
@@ -137,12 +127,72 @@ PGLD_HD double hap_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
                s4 * (s1 * xq * Y.pB + xa * inner) +
                s2 * (s4 * (xa + xq) * Y.pB + s3 * (xa * Y.pB + xq * Y.qB) +
                      xq * ((-1.0 + s1) * b1 - b3 + s1 * (b2 + b3 + b4)));
-  return safe_div(num * 0.25, f2(S.n) * X.n * Y.n);
+  return num * 0.25;
+}
+// denominator 4 n_a n_b n_c n_d
+PGLD_HD double num_pi2_alldiff(const HapP& a, const HapP& b, const HapP& c, const HapP& d) {
+  return (a.pA * b.qA + a.qA * b.pA) * (c.pB * d.qB + c.qB * d.pB);
+}
+
+// Division forms: one correctly rounded division per statistic, 0 where the
+// denominator is 0 (ld_statistics.py:1124-1127).
+PGLD_HD double hap_dd_single(const HapP& a) { return safe_div(num_dd_single(a), f4(a.n)); }
+PGLD_HD double hap_dd_between(const HapP& a, const HapP& b) {
+  return safe_div(num_dd_between(a, b), f2(a.n) * f2(b.n));
+}
+PGLD_HD double hap_dz_iii(const HapP& a) { return safe_div(num_dz_iii(a), f4(a.n)); }
+PGLD_HD double hap_dz_iij(const HapP& a, const HapP& b) { return safe_div(num_dz_iij(a, b), b.n * f3(a.n)); }
+PGLD_HD double hap_dz_iji(const HapP& a, const HapP& b) { return safe_div(num_dz_iji(a, b), b.n * f3(a.n)); }
+PGLD_HD double hap_dz_ijj(const HapP& a, const HapP& b) { return safe_div(num_dz_ijj(a, b), f2(a.n) * f2(b.n)); }
+PGLD_HD double hap_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
+  return safe_div(num_dz_diff(a, b, c), f2(a.n) * b.n * c.n);
+}
+PGLD_HD double hap_pi2_iiii(const HapP& a) { return safe_div(num_pi2_iiii(a), f4(a.n)); }
+PGLD_HD double hap_pi2_triple(const HapP& t, const HapP& s) {
+  return safe_div(num_pi2_triple(t, s), s.n * f3(t.n));
+}
+PGLD_HD double hap_pi2_iikk(const HapP& a, const HapP& b) { return safe_div(num_pi2_iikk(a, b), f2(a.n) * f2(b.n)); }
+PGLD_HD double hap_pi2_iikk_sym(const HapP& a, const HapP& b) {
+  return safe_div(num_pi2_iikk_sym(a, b), f2(a.n) * f2(b.n));
+}
+PGLD_HD double hap_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
+  return safe_div(num_pi2_iikl(a, b, c), 2.0 * f2(a.n) * b.n * c.n);
+}
+PGLD_HD double hap_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
+  return safe_div(num_pi2_ijkk(a, b, c), 2.0 * f2(c.n) * a.n * b.n);
+}
+PGLD_HD double hap_pi2_ijij(const HapP& A, const HapP& B) {
+  return safe_div(num_pi2_ijij(A, B), f2(A.n) * f2(B.n));
+}
+PGLD_HD double hap_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
+  return safe_div(num_pi2_shared(S, X, Y), f2(S.n) * X.n * Y.n);
 }
 PGLD_HD double hap_pi2_alldiff(const HapP& a, const HapP& b, const HapP& c, const HapP& d) {
-  double HA = a.pA * b.qA + a.qA * b.pA;
-  double HB = c.pB * d.qB + c.qB * d.pB;
-  return safe_div(HA * HB, 4.0 * a.n * b.n * c.n * d.n);
+  return safe_div(num_pi2_alldiff(a, b, c, d), 4.0 * a.n * b.n * c.n * d.n);
+}
+
+// Reciprocal form for the fused epilogue: the four falling-factorial
+// reciprocals of one population from a single division (a few ulp from the
+// division form); 0 where the factorial is 0, which reproduces the reference's
+// "invalid denominator -> 0".
+struct HapR {
+  double i1, i2, i3, i4;  // 1/n, 1/(n)_2, 1/(n)_3, 1/(n)_4
+};
+PGLD_HD HapR hap_recips(double n) {
+  HapR r;
+  const double g2 = n * (n - 1.0), g3 = g2 * (n - 2.0), g4 = g3 * (n - 3.0);
+  if (g4 > 0.0) {
+    r.i4 = 1.0 / g4;
+    r.i3 = r.i4 * (n - 3.0);
+    r.i2 = r.i3 * (n - 2.0);
+    r.i1 = r.i2 * (n - 1.0);
+  } else {
+    r.i4 = 0.0;
+    r.i3 = g3 > 0.0 ? 1.0 / g3 : 0.0;
+    r.i2 = g2 > 0.0 ? 1.0 / g2 : 0.0;
+    r.i1 = n > 0.0 ? 1.0 / n : 0.0;
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------------------

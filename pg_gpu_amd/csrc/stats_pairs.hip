@@ -10,6 +10,7 @@ namespace pgld {
 
 struct CountPtrs {
   const int32_t* c[4];
+  const int32_t* nv[4];
 };
 
 template <int NC, int P>
@@ -24,11 +25,16 @@ struct PairCounts {
   }
 };
 
-__device__ __forceinline__ HapP hp_of(const double* c) { return hap_pop(c[0], c[1], c[2], c[3]); }
+// c[4] carries the population's sample size n (n_valid or the sum of counts)
+__device__ __forceinline__ HapP hp_of(const double* c) {
+  HapP p = hap_pop(c[0], c[1], c[2], c[3]);
+  p.n = c[4];
+  return p;
+}
 
 // Haplotype term (haplotype_kernels.py:335-581 case dispatch)
 template <int P>
-__device__ double hap_term(const PairCounts<4, P>& pc, int type, int a, int b, int c, int d) {
+__device__ double hap_term(const PairCounts<5, P>& pc, int type, int a, int b, int c, int d) {
   if (type == ST_DD) {
     HapP A = hp_of(pc.pop(a));
     if (a == b) return hap_dd_single(A);
@@ -139,13 +145,17 @@ __global__ __launch_bounds__(256) void k_stats_pairs(CountPtrs cp, int64_t n, co
                                                      const int32_t* __restrict__ off, int32_t n_stats,
                                                      double* __restrict__ out) {
   constexpr int NC = FAMILY == PGLD_FAMILY_HAP ? 4 : 9;
+  constexpr int NV = FAMILY == PGLD_FAMILY_HAP ? 5 : 9;  // haplotypes keep n in slot 4
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  PairCounts<NC, P> pc;
+  PairCounts<NV, P> pc;
 #pragma unroll
-  for (int q = 0; q < P; ++q)
+  for (int q = 0; q < P; ++q) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) pc.v[q][k] = (double)cp.c[q][t * NC + k];
+    if constexpr (FAMILY == PGLD_FAMILY_HAP)
+      pc.v[q][4] = cp.nv[q] ? (double)cp.nv[q][t] : pc.v[q][0] + pc.v[q][1] + pc.v[q][2] + pc.v[q][3];
+  }
   for (int s = 0; s < n_stats; ++s) {
     double acc = 0.0;
     for (int e = off[s]; e < off[s + 1]; ++e) {
@@ -178,9 +188,9 @@ static int launch_stats(int32_t n_pops, CountPtrs cp, int64_t n, const pgld_term
 
 using namespace pgld;
 
-extern "C" int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* const* counts, int64_t n_pairs,
-                                const pgld_term* terms, const int32_t* stat_offsets, int32_t n_stats, double* out,
-                                void* stream) {
+extern "C" int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* const* counts,
+                                const int32_t* const* n_valid, int64_t n_pairs, const pgld_term* terms,
+                                const int32_t* stat_offsets, int32_t n_stats, double* out, void* stream) {
   PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_stats_pairs: bad family %d", family);
   PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_stats_pairs: n_pops must be 1..4");
   PGLD_REQUIRE(n_pairs >= 0 && n_stats >= 0, "pgld_stats_pairs: negative size");
@@ -190,6 +200,7 @@ extern "C" int pgld_stats_pairs(int32_t family, int32_t n_pops, const int32_t* c
   for (int q = 0; q < n_pops; ++q) {
     PGLD_REQUIRE(counts[q], "pgld_stats_pairs: null counts pointer for pop %d", q);
     cp.c[q] = counts[q];
+    cp.nv[q] = n_valid ? n_valid[q] : nullptr;
   }
   hipStream_t s = as_stream(stream);
   if (family == PGLD_FAMILY_HAP) return launch_stats<PGLD_FAMILY_HAP>(n_pops, cp, n_pairs, terms, stat_offsets,

@@ -1,0 +1,283 @@
+"""GPU parity: every libpgld path against the oracle on seeded inputs.
+
+Counts must be bit-exact.  Per-pair statistics are evaluated from exact
+integer numerators with one float64 division, so they must agree with the
+oracle to a few ulp (rtol 1e-12).  Keyed sums are float64 reductions in a
+different order than the oracle's: rtol 1e-12 against max(|sum|, sum of |v|).
+"""
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from pg_gpu_amd import band, ld_pipeline as LP, ld_statistics as LS  # noqa: E402
+from pg_gpu_amd import haplotype_kernels as HK, genotype_kernels as GK  # noqa: E402
+from pg_gpu_amd import synthetic as SY  # noqa: E402
+
+
+def np_(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
+def hap_data(seed, n_hap, n_var, missing):
+    rng = np.random.default_rng(seed)
+    h = SY.neutral_haplotypes(n_hap, n_var, rng, missing=missing)
+    pos = SY.uniform_positions(n_var, n_var * 100, rng)
+    return h, pos
+
+
+def geno_data(seed, n_ind, n_var, missing):
+    rng = np.random.default_rng(seed)
+    g = SY.hwe_genotypes(n_ind, n_var, rng, missing=missing)
+    pos = SY.uniform_positions(n_var, n_var * 100, rng)
+    return g, pos
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("chunk", [7, 1000, 10**7])
+def test_pairs_match_oracle(chunk):
+    rng = np.random.default_rng(5)
+    pos = np.unique(np.sort(rng.integers(0, 20000, size=600)))
+    md = 700
+    got = [(np_(a), np_(b)) for a, b in LP.iter_pairs_within_distance(pos, md, chunk)]
+    ii = np.concatenate([g[0] for g in got])
+    jj = np.concatenate([g[1] for g in got])
+    ei, ej = O.pairs_within_distance(pos, md)
+    np.testing.assert_array_equal(ii, ei)
+    np.testing.assert_array_equal(jj, ej)
+    # chunk boundaries follow the reference's anchor blocks
+    bounds = O.chunk_bounds(pos, md, chunk)
+    assert len(bounds) == len(got)
+
+
+def test_pairs_edge_cases():
+    assert list(LP.iter_pairs_within_distance(np.array([], dtype=np.int64), 10, 5)) == []
+    assert list(LP.iter_pairs_within_distance(np.array([3]), 10, 5)) == []
+    assert list(LP.iter_pairs_within_distance(np.arange(10), 0, 4)) == []
+    n = 25
+    got = list(LP.iter_pairs_within_distance(np.arange(n) * 10, 10**9, 7))
+    assert sum(int(a.shape[0]) for a, _ in got) == n * (n - 1) // 2
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("missing", [0.0, 0.05])
+@pytest.mark.parametrize("n_hap", [37, 200])
+def test_hap_counts_exact(missing, n_hap):
+    h, pos = hap_data(11, n_hap, 3000, missing)
+    ii, jj = O.pairs_within_distance(pos, 2500)
+    # plus pairs spanning more than the LDS staging width (direct path)
+    ii = np.concatenate([ii, [0, 5, 17]])
+    jj = np.concatenate([jj, [2999, 2900, 2500]])
+    rows = list(range(0, n_hap, 2))
+    for sel in (None, rows):
+        c, nv = LP.compute_counts_for_pairs(h, ii, jj, sel)
+        ec, env = O.hap_counts(h, ii, jj, sel)
+        np.testing.assert_array_equal(np_(c), ec)
+        assert (nv is None) == (env is None)
+        if env is not None:
+            np.testing.assert_array_equal(np_(nv), env)
+
+
+def test_hap_counts_kat():
+    # tests/test_haplotype_counting.py:56-112 known answers; rows are ind0..ind3
+    # first haplotypes then ind0..ind3 second haplotypes (the VCF (s, s+n) layout)
+    hap = np.array([[1, 0, 1, 0, 1, 1, 0, 0], [1, 1, 1, 0, 0, 0, 1, 0]], dtype=np.int8).T
+    c, _ = LP.compute_counts_for_pairs(hap, [0], [1], [0, 1, 4, 5])
+    np.testing.assert_array_equal(np_(c)[0], [1, 2, 1, 0])
+    c, _ = LP.compute_counts_for_pairs(hap, [0], [1], [2, 3, 6, 7])
+    np.testing.assert_array_equal(np_(c)[0], [1, 0, 1, 2])
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.05])
+def test_geno_counts_exact(missing):
+    g, pos = geno_data(12, 53, 2600, missing)
+    ii, jj = O.pairs_within_distance(pos, 3000)
+    ii = np.concatenate([ii, [0]])
+    jj = np.concatenate([jj, [2599]])
+    for sel in (None, list(range(1, 53, 3))):
+        c, nv = LP.compute_genotype_counts_for_pairs(g, ii, jj, sel)
+        ec, env = O.geno_counts(g, ii, jj, sel)
+        np.testing.assert_array_equal(np_(c), ec)
+        assert (nv is None) == (env is None)
+
+
+def test_counts_domain_error():
+    h = np.zeros((4, 5), dtype=np.int8)
+    h[1, 2] = 2
+    with pytest.raises(ValueError):
+        LP.compute_counts_for_pairs(h, [0], [2])
+
+
+# ---------------------------------------------------------------------------
+def _rand_counts(rng, N, P, n_max, kind="hap"):
+    out = []
+    for _ in range(P):
+        n = rng.integers(0, n_max, size=N)
+        k = 4 if kind == "hap" else 9
+        w = rng.dirichlet(np.ones(k), size=N)
+        c = np.floor(w * n[:, None]).astype(np.int64)
+        out.append(c)
+    return out
+
+
+def assert_close(got, exp, rtol=1e-12, atol=1e-15):
+    got, exp = np_(got), np.asarray(exp)
+    fin = np.isfinite(exp)
+    np.testing.assert_array_equal(np.isfinite(got), fin)
+    np.testing.assert_allclose(got[fin], exp[fin], rtol=rtol, atol=atol)
+
+
+def test_two_pop_fast_path():
+    rng = np.random.default_rng(3)
+    c0, c1 = _rand_counts(rng, 4000, 2, 60)
+    got = LP.compute_two_pop_statistics_batch(c0, c1, None, None)
+    assert_close(got, O.two_pop_stats(c0, c1))
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_multi_pop_hap(P):
+    rng = np.random.default_rng(10 + P)
+    cl = _rand_counts(rng, 1500, P, 40)
+    got = HK.compute_multi_pop_statistics_batch_hap(cl, [None] * P, None, LP.generate_stat_specs(P))
+    assert_close(got, O.multi_pop_stats_hap(cl, [None] * P))
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_multi_pop_geno(P):
+    rng = np.random.default_rng(20 + P)
+    cl = _rand_counts(rng, 1200, P, 40, kind="geno")
+    for c in cl:  # keep every population at n >= 4 (the reference divides unguarded)
+        c[:, 0] += 4
+    got = GK.compute_multi_pop_statistics_batch_geno(cl, [None] * P, None, LP.generate_stat_specs(P))
+    assert_close(got, O.multi_pop_stats_geno(cl, [None] * P), rtol=1e-11, atol=1e-13)
+
+
+def test_single_stat_api_and_n_valid():
+    rng = np.random.default_rng(4)
+    c0, c1 = _rand_counts(rng, 800, 2, 50)
+    both = np.concatenate([c0, c1], axis=1)
+    P = [O.HapPop(c0), O.HapPop(c1)]
+    assert_close(LS.dd(c0), O.hap_dd([O.HapPop(c0)], 0, 0))
+    assert_close(LS.dd(both, (0, 1)), O.hap_dd(P, 0, 1))
+    assert_close(LS.dz(both, (0, 1, 1)), O.hap_dz(P, 0, 1, 1))
+    assert_close(LS.pi2(both, (0, 0, 1, 1)), O.hap_pi2_python(P, 0, 0, 1, 1))
+    assert_close(LS.pi2(both, (0, 1, 0, 1)), O.hap_pi2_python(P, 0, 1, 0, 1))
+    nv = c0.sum(1)
+    assert_close(LS.dd(c0, n_valid=nv), O.hap_dd([O.HapPop(c0, nv)], 0, 0))
+
+
+def test_r_family():
+    rng = np.random.default_rng(8)
+    (c,) = _rand_counts(rng, 3000, 1, 80)
+    c = np.concatenate([c, [[5, 0, 0, 5], [25, 25, 25, 25], [0, 5, 5, 0], [5, 5, 0, 0], [28, 52, 2, 18]]])
+    assert_close(LS.r(c), O.r(c))
+    assert_close(LS.r_squared(c), O.r_squared(c))
+    assert_close(LS.d_prime(c), O.d_prime(c))
+    np.testing.assert_allclose(np_(LS.d_prime(c))[-1], 2.0 / 3.0, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("family,missing", [("hap", True), ("hap", False), ("geno", True), ("geno", False)])
+def test_band_counts_exact(family, missing):
+    if family == "hap":
+        d, pos = hap_data(30, 173, 900, 0.03 if missing else 0.0)
+    else:
+        d, pos = geno_data(31, 141, 900, 0.03 if missing else 0.0)
+    md = 4000
+    plan = band.BandPlan(pos, md, edges=[0.0, 1e9])
+    rows = list(range(0, d.shape[0], 1))
+    pk = band.pack_population(d, rows, family=family)
+    assert pk.has_missing == missing
+    got = np_(plan.counts(pk))
+    ii, jj = O.pairs_within_distance(pos, md)
+    assert plan.n_pairs == len(ii)
+    exp = (O.hap_counts if family == "hap" else O.geno_counts)(d, ii, jj, rows)[0]
+    np.testing.assert_array_equal(got, exp)
+
+
+def _sums_oracle(family, d, pos, pops, md, stat_set, key):
+    ii, jj = O.pairs_within_distance(pos, md)
+    cl = [(O.hap_counts if family == "hap" else O.geno_counts)(d, ii, jj, r)[0] for r in pops]
+    P = len(pops)
+    if family == "hap":
+        if stat_set == "self" and P > 1:
+            specs = [(f"{t}", [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(P)]
+            stats = O.multi_pop_stats_hap(cl, [None] * P, specs)
+        elif P == 2:
+            stats = O.two_pop_stats(cl[0], cl[1])
+        else:
+            stats = O.multi_pop_stats_hap(cl, [None] * P)
+    else:
+        stats = O.multi_pop_stats_geno(cl, [None] * P)
+    keys = key(ii, jj)
+    return stats, keys
+
+
+def _check_sums(got, stats, keys, n_keys):
+    exp = O.keyed_sums(keys, stats, n_keys)
+    scale = O.keyed_sums(keys, np.abs(stats), n_keys)
+    got = np_(got)
+    assert np.all(np.abs(got - exp) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300)
+
+
+@pytest.mark.parametrize("P,missing,stat_set", [(1, True, "moments"), (2, True, "moments"),
+                                                (2, False, "moments"), (3, True, "moments"),
+                                                (4, True, "self"), (2, True, "self")])
+def test_band_sums_windows_hap(P, missing, stat_set):
+    d, pos = hap_data(40 + P, 40 * P + 17, 1300, 0.02 if missing else 0.0)
+    n = d.shape[0]
+    pops = [list(range(p, n, P)) for p in range(P)]
+    md, width = 5000.0, 20000.0
+    plan = band.BandPlan(pos, md, window=width)
+    packed = [band.pack_population(d, r, "hap") for r in pops]
+    sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
+    stats, keys = _sums_oracle("hap", d, pos, pops, md, stat_set,
+                               lambda ii, jj: O.window_key(pos[ii], 0.0, width))
+    _check_sums(sums, stats, keys, plan.n_keys)
+    np.testing.assert_array_equal(np_(kp), np.bincount(keys, minlength=plan.n_keys))
+
+
+@pytest.mark.parametrize("P,missing", [(1, True), (2, True), (2, False)])
+def test_band_sums_bins(P, missing):
+    d, pos = hap_data(60 + P, 50 * P + 9, 1100, 0.03 if missing else 0.0)
+    n = d.shape[0]
+    pops = [list(range(p, n, P)) for p in range(P)]
+    edges = np.logspace(1, 4, 9)
+    md = float(edges[-1])
+    plan = band.BandPlan(pos, md, edges=edges)
+    packed = [band.pack_population(d, r, "hap") for r in pops]
+    sums = plan.sums(packed)
+    stats, keys = _sums_oracle("hap", d, pos, pops, md, "moments",
+                               lambda ii, jj: O.bin_index(pos[jj] - pos[ii], edges))
+    _check_sums(sums, stats, keys, len(edges) - 1)
+
+
+@pytest.mark.parametrize("P,missing", [(1, True), (1, False), (2, True)])
+def test_band_sums_geno(P, missing):
+    d, pos = geno_data(70 + P, 60 * P + 5, 800, 0.05 if missing else 0.0)
+    n = d.shape[0]
+    pops = [list(range(p, n, P)) for p in range(P)]
+    edges = np.logspace(0, 5, 17)
+    md = float(edges[-1])
+    plan = band.BandPlan(pos, md, edges=edges)
+    packed = [band.pack_population(d, r, "geno") for r in pops]
+    sums = plan.sums(packed)
+    stats, keys = _sums_oracle("geno", d, pos, pops, md, "moments",
+                               lambda ii, jj: O.bin_index(pos[jj] - pos[ii], edges))
+    _check_sums(sums, stats, keys, len(edges) - 1)
+
+
+def test_band_sums_deterministic():
+    d, pos = hap_data(80, 301, 1500, 0.02)
+    pops = [list(range(0, 150)), list(range(150, 301))]
+    plan = band.BandPlan(pos, 8000.0, window=10000.0)
+    packed = [band.pack_population(d, r, "hap") for r in pops]
+    a = np_(plan.sums(packed))
+    b = np_(plan.sums(packed))
+    np.testing.assert_array_equal(a, b)

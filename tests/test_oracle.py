@@ -1,0 +1,235 @@
+"""CPU pins of the oracle: brute force, closed forms, the reference's known
+answers and the frozen moments output.  No GPU."""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as O
+import ustat_bruteforce as U
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------------------
+# pair enumeration (tests/test_ld_pair_iteration.py:21-115)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [2, 10, 50])
+@pytest.mark.parametrize("factor", [0.1, 0.5, 1.0, 10.0])
+def test_pairs_bruteforce(n, factor):
+    rng = np.random.default_rng(n * 997 + int(factor * 1000))
+    pos = np.unique(np.sort(rng.integers(0, 10 * n, size=n)))
+    if len(pos) < 2:
+        pytest.skip("collapsed")
+    md = max(1, int((pos[-1] - pos[0]) * factor))
+    ii, jj = O.pairs_within_distance(pos, md)
+    exp = {(i, j) for i in range(len(pos)) for j in range(i + 1, len(pos)) if pos[j] - pos[i] <= md}
+    assert set(zip(ii.tolist(), jj.tolist())) == exp
+    assert len(ii) == len(exp)
+    assert np.all(np.diff(ii) >= 0)
+
+
+def test_pairs_edges_and_chunks():
+    assert O.pairs_within_distance(np.array([]), 5)[0].size == 0
+    assert O.pairs_within_distance(np.array([1]), 5)[0].size == 0
+    assert O.pairs_within_distance(np.arange(10), 0)[0].size == 0
+    ii, _ = O.pairs_within_distance(np.arange(25) * 10, 10**9)
+    assert ii.size == 25 * 24 // 2
+    rng = np.random.default_rng(12345)
+    pos = np.unique(np.sort(rng.integers(0, 100_000, size=500)))
+    total = O.pairs_within_distance(pos, 5000)[0].size
+    for cs in (1, 100, 10_000, 1_000_000):
+        b = O.chunk_bounds(pos, 5000, cs)
+        per = np.searchsorted(pos, pos + 5000, side="right") - np.arange(len(pos)) - 1
+        assert sum(int(np.maximum(per, 0)[lo:hi].sum()) for lo, hi in b) == total
+
+
+# ---------------------------------------------------------------------------
+# counts
+# ---------------------------------------------------------------------------
+def test_hap_counts_kat():
+    # tests/test_haplotype_counting.py:56-112
+    hap = np.array([[1, 0, 1, 0, 1, 1, 0, 0], [1, 1, 1, 0, 0, 0, 1, 0]], dtype=np.int8).T
+    c, nv = O.hap_counts(hap, [0], [1], [0, 1, 4, 5])
+    np.testing.assert_array_equal(c[0], [1, 2, 1, 0])
+    assert nv is None
+    c, _ = O.hap_counts(hap, [0], [1], [2, 3, 6, 7])
+    np.testing.assert_array_equal(c[0], [1, 0, 1, 2])
+
+
+def test_counts_definition_bruteforce():
+    rng = np.random.default_rng(2)
+    h = rng.integers(-1, 2, size=(23, 12)).astype(np.int8)
+    g = rng.integers(-1, 3, size=(19, 12)).astype(np.int8)
+    ii, jj = np.triu_indices(12, 1)
+    ch, nvh = O.hap_counts(h, ii, jj)
+    cg, nvg = O.geno_counts(g, ii, jj)
+    for p, (i, j) in enumerate(zip(ii, jj)):
+        exp = [0, 0, 0, 0]
+        for s in range(23):
+            a, b = h[s, i], h[s, j]
+            if a >= 0 and b >= 0:
+                exp[{(1, 1): 0, (1, 0): 1, (0, 1): 2, (0, 0): 3}[(a, b)]] += 1
+        assert list(ch[p]) == exp and nvh[p] == sum(exp)
+        e9 = [0] * 9
+        for s in range(19):
+            a, b = g[s, i], g[s, j]
+            if a >= 0 and b >= 0:
+                e9[3 * a + b] += 1
+        assert list(cg[p]) == e9 and nvg[p] == sum(e9)
+    # symmetry i <-> j swaps n10 and n01
+    cs, _ = O.hap_counts(h, jj, ii)
+    np.testing.assert_array_equal(cs[:, [0, 2, 1, 3]], ch)
+
+
+# ---------------------------------------------------------------------------
+# r family KATs (tests/test_ld_r_squared.py:15-57, :171-230)
+# ---------------------------------------------------------------------------
+def test_r_family_kats():
+    c = np.array([[5, 0, 0, 5], [25, 25, 25, 25], [0, 5, 5, 0], [5, 5, 0, 0], [28, 52, 2, 18]])
+    r2 = O.r_squared(c)
+    np.testing.assert_allclose(r2[:3], [1.0, 0.0, 1.0], atol=1e-12)
+    assert np.isnan(r2[3])
+    np.testing.assert_allclose(O.r(c)[2], -1.0, rtol=1e-12)
+    dp = O.d_prime(c)
+    np.testing.assert_allclose(dp[[0, 1, 2, 4]], [1.0, 0.0, -1.0, 2.0 / 3.0], atol=1e-12)
+    assert np.isnan(dp[3])
+
+
+# ---------------------------------------------------------------------------
+# DD / Dz / pi2 against brute-force U-statistics (every population case)
+# ---------------------------------------------------------------------------
+def _calls(P):
+    out = set()
+    for _, terms in O.stat_specs(P):
+        for _, t, p in terms:
+            out.add((t, p))
+    return sorted(out)
+
+
+def _hap_triple_normalised(p):
+    """The reference evaluates every pi2 with one population three times in
+    its (t, t | t, s) form (haplotype_kernels.py:268-289, ld_statistics.py:
+    1323-1328): the U-statistic of that form is the pin."""
+    cnt = {x: p.count(x) for x in set(p)}
+    if len(cnt) == 2 and max(cnt.values()) == 3:
+        t = [x for x, c in cnt.items() if c == 3][0]
+        s = [x for x, c in cnt.items() if c == 1][0]
+        return (t, t, t, s)
+    return p
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_hap_stats_bruteforce(seed):
+    P = 3
+    rng = np.random.default_rng(seed)
+    pops = [rng.integers(0, 2, size=(rng.integers(4, 6), 2)) for _ in range(P)]
+    H = [O.HapPop(U.hap_counts_of(p)[None, :]) for p in pops]
+    for t, p in _calls(P):
+        if t == "dd":
+            got = O.hap_dd(H, *p)
+        elif t == "dz":
+            got = O.hap_dz(H, *p)
+        else:
+            got = O.hap_pi2_kernel(H, *p)
+            p = _hap_triple_normalised(p)
+        exp = U.hap_stat(pops, t, p)
+        assert abs(float(got[0]) - exp) <= 1e-12 * max(1.0, abs(exp)), (t, p)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_geno_stats_bruteforce(seed):
+    P = 3
+    rng = np.random.default_rng(100 + seed)
+    pops = [rng.integers(0, 3, size=(rng.integers(4, 6), 2)) for _ in range(P)]
+    G = [O.GenoPop(U.geno_counts_of(p)[None, :]) for p in pops]
+    fn = {"dd": O.geno_dd, "dz": O.geno_dz, "pi2": O.geno_pi2}
+    for t, p in _calls(P):
+        got = float(fn[t](G, *p)[0])
+        exp = U.geno_stat(pops, t, p)
+        assert abs(got - exp) <= 1e-12 * max(1.0, abs(exp)), (t, p)
+
+
+def test_hap_pi2_forms_agree_after_averaging():
+    rng = np.random.default_rng(9)
+    cl = [rng.integers(0, 30, size=(200, 4)) for _ in range(2)]
+    a = O.two_pop_stats(cl[0], cl[1])
+    b = O.multi_pop_stats_hap(cl, [None, None])
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+def test_sigma_d2_geno_matches_ratio():
+    rng = np.random.default_rng(5)
+    c = rng.integers(0, 20, size=(300, 9))
+    G = [O.GenoPop(c)]
+    ratio = O.geno_dd(G, 0, 0) / O.geno_pi2(G, 0, 0, 0, 0)
+    got = O.sigma_d2_geno_from_counts(c)
+    ok = np.isfinite(got)
+    np.testing.assert_allclose(got[ok], ratio[ok], rtol=1e-12)
+
+
+def test_sigma_d2_hap_tile_matches_dd_over_pi2():
+    rng = np.random.default_rng(6)
+    c = rng.integers(0, 20, size=(300, 4))
+    P = [O.HapPop(c)]
+    s, valid = O.sigma_d2_hap_from_counts(c)
+    ratio = O.hap_dd(P, 0, 0) / O.hap_pi2_kernel(P, 0, 0, 0, 0)
+    ok = valid & np.isfinite(ratio) & (O.hap_pi2_kernel(P, 0, 0, 0, 0) != 0)
+    np.testing.assert_allclose(s[ok], ratio[ok], rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# names / specs (ld_pipeline.py:357-457)
+# ---------------------------------------------------------------------------
+def test_names_counts_and_spec_weights():
+    assert [len(O.ld_names(p)) for p in (1, 2, 3, 4)] == [3, 15, 45, 105]
+    assert O.ld_names(2) == ["DD_0_0", "DD_0_1", "DD_1_1", "Dz_0_0_0", "Dz_0_0_1", "Dz_0_1_1", "Dz_1_0_0",
+                             "Dz_1_0_1", "Dz_1_1_1", "pi2_0_0_0_0", "pi2_0_0_0_1", "pi2_0_0_1_1",
+                             "pi2_0_1_0_1", "pi2_0_1_1_1", "pi2_1_1_1_1"]
+    for P in (1, 2, 3, 4):
+        for _, terms in O.stat_specs(P):
+            assert abs(sum(w for w, _, _ in terms) - 1.0) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# the frozen moments output on the IM example (Appendix-style golden vector)
+# ---------------------------------------------------------------------------
+def _im_example():
+    from pg_gpu_amd import vcf
+    d = vcf.read_vcf(os.path.join(GOLD, "im-parsing-example.vcf"))
+    gt, pos, samples = d["gt"], d["positions"], d["samples"]
+    ns = len(samples)
+    hap = np.concatenate([gt[:, :, 0], gt[:, :, 1]], axis=1).T
+    pm = vcf.read_pop_file(os.path.join(GOLD, "im_pop.txt"))
+    rows = {"deme0": [], "deme1": []}
+    for i, s in enumerate(samples):
+        if pm.get(s) in rows:
+            rows[pm[s]] += [i, i + ns]
+    return hap, pos, [rows["deme0"], rows["deme1"]]
+
+
+def test_moments_golden_im_example():
+    G = json.load(open(os.path.join(GOLD, "moments_im_4bins.json")))
+    hap, pos, pops = _im_example()
+    keep = O.biallelic_filter_hap(hap)
+    hap, pos = hap[:, keep], pos[keep]
+    bins = np.array([G["bins"][0][0]] + [b[1] for b in G["bins"]])
+    sums = O.moments_ld_sums(hap, pos, pops, bins)
+    np.testing.assert_allclose(sums, np.array(G["sums"]), rtol=1e-10)
+    H = O.heterozygosity([hap[r] for r in pops])
+    np.testing.assert_allclose([H[k] for k in G["het_names"]], G["het"], rtol=1e-13)
+    assert O.ld_names(2) == G["ld_names"]
+
+
+def test_keyed_sums_bruteforce():
+    rng = np.random.default_rng(1)
+    keys = rng.integers(-2, 7, size=500)
+    vals = rng.normal(size=(500, 3))
+    got = O.keyed_sums(keys, vals, 5)
+    for k in range(5):
+        np.testing.assert_allclose(got[k], vals[keys == k].sum(axis=0), rtol=1e-13)
+    # digitize semantics: [b_k, b_{k+1}); the last edge itself is outside
+    e = np.array([1.0, 10.0, 100.0])
+    np.testing.assert_array_equal(O.bin_index([0.5, 1.0, 9.99, 10.0, 99.9, 100.0], e), [-1, 0, 0, 1, 1, 2])
