@@ -120,22 +120,102 @@ __device__ __forceinline__ void wave_products(const int8_t* __restrict__ planes,
     for (int b = 0; b < N; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+  // One operand buffer used as a rolling pipeline: chunk t of step k+1 is
+  // requested right after the MFMAs that consume chunk t of step k, so a
+  // whole step of loads is in flight behind the MFMAs.
+  v4i A[N][4], B[N][4];
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + 16 * t);
+      B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + 16 * t);
+    }
   for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
-    v4i A[N][4], B[N][4];
+    const bool more = k0 + KSTEP < k_pad;
+    const int64_t kn = more ? k0 + KSTEP : k0;  // re-read (harmless) on the last step
 #pragma unroll
-    for (int a = 0; a < N; ++a)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + k0 + 16 * t);
-        B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + k0 + 16 * t);
-      }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t) {
 #pragma unroll
       for (int a = 0; a < N; ++a)
 #pragma unroll
         for (int b = 0; b < N; ++b)
           acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a][t], B[b][t], acc.c[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + kn + 16 * t);
+        B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + kn + 16 * t);
+      }
+    }
+  }
+}
+
+// LDS-staged products for the whole workgroup (all four waves must call it:
+// it contains barriers).  Per 128-byte K chunk the 64 A rows and 64 B rows of
+// every plane are copied with full-line loads (8 lanes per 128-byte line) into
+// LDS rows padded to 144 bytes, which makes the MFMA-layout ds_read_b128 reads
+// (lane l -> row l % 32) bank-conflict free.  The next chunk's global loads
+// are issued before the MFMAs of the current chunk.
+constexpr int OP_ROW = 144;  // padded LDS row stride (bytes)
+
+template <int N>
+__host__ __device__ constexpr int opbuf_bytes() { return 2 * N * BT * OP_ROW; }
+
+template <int FAMILY, bool MISSING>
+__device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
+                                               int i0, int j0, int wr, int wc, bool wave_live,
+                                               unsigned char* __restrict__ opbuf,
+                                               WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
+  using PL = Planes<FAMILY, MISSING>;
+  constexpr int N = PL::N;
+  constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int8_t* src[ITEMS];
+  int dst[ITEMS];
+#pragma unroll
+  for (int m = 0; m < ITEMS; ++m) {
+    const int e = tid + BAND_THREADS * m;
+    const int q = e & 7, row = (e >> 3) & (BT - 1), rest = e >> 9;
+    const int a = rest % N, side = rest / N;
+    const int64_t snp = min((int64_t)(side ? j0 : i0) + row, n_var - 1);
+    src[m] = planes + ((int64_t)(PL::FIRST + a) * n_var + snp) * k_pad + 16 * q;
+    dst[m] = ((side * N + a) * BT + row) * OP_ROW + 16 * q;
+  }
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+  v4i pre[ITEMS];
+#pragma unroll
+  for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m]);
+  const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
+  for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
+    __syncthreads();  // the previous chunk has been read by every wave
+#pragma unroll
+    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dst[m]) = pre[m];
+    __syncthreads();
+    if (k0 + KSTEP < k_pad) {
+#pragma unroll
+      for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + KSTEP);
+    }
+    if (wave_live) {
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        v4i A[N], B[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+          A[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
+          B[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
+        }
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc.c[a][b], 0, 0, 0);
+      }
+    }
   }
 }
 
@@ -209,6 +289,17 @@ __device__ __forceinline__ int32_t key_bin(double d, const double* edges, int32_
   return lo;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD
+// b % 8).  Tile t of the band schedule is given to workgroup b such that each
+// XCD walks one contiguous range of tiles: consecutive tiles share their A rows
+// and most B columns, so they hit the same L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t x = b % N_XCD, slot = b / N_XCD;
+  const int64_t per = n / N_XCD, rem = n % N_XCD;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
 struct BandArgs {
   const BandTile* tiles;
   const double* pos;
@@ -278,17 +369,26 @@ struct PopVal<PGLD_FAMILY_GENO> {
 // ---------------------------------------------------------------------------
 // Fused band kernel: counts for all populations, statistics, keyed sums.
 // ---------------------------------------------------------------------------
+// Two workgroups per CU where the register budget allows it without spills, so
+// the MFMA phase of one tile overlaps the FP64 epilogue of another.
+constexpr int band_min_waves(int family, int P, int set) {
+  return ((family == PGLD_FAMILY_HAP && (P <= 2 || set == PGLD_SET_SELF)) || (family == PGLD_FAMILY_GENO && P == 1))
+             ? 2
+             : 1;
+}
+
 template <int FAMILY, bool MISSING, int P, int SET>
-__global__ __launch_bounds__(BAND_THREADS) void k_band_sums(BandArgs ba, PopArgs pa) {
+__global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
+                                                                                            PopArgs pa) {
   using PL = Planes<FAMILY, MISSING>;
   using SS = StatSet<FAMILY, P, SET>;
   constexpr int NC = PL::NC;
   constexpr int NS = SS::N + 1;  // statistics + pair count
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  // layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16]
+  // layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
 
-  const BandTile tile = ba.tiles[blockIdx.x];
+  const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (tile.n_slots <= 0) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   double* red = reinterpret_cast<double*>(smem);
@@ -313,23 +413,24 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band_sums(BandArgs ba, PopArgs
   bool wave_live = (int64_t)rj < ba.n_var && rj + 31 > ri && (int64_t)ri < ba.n_var &&
                    s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ba.max_dist;
 
+  unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
   WaveAcc<PL::N> acc;
-  if (wave_live) {
 #pragma unroll 1
-    for (int p = 0; p < P - 1; ++p) {
-      wave_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], ri, rj, acc);
-      const int64_t j = rj + (lane & 31);
+  for (int p = 0; p < P; ++p) {
+    block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live, opbuf,
+                                    acc);
+    if (p == P - 1 || !wave_live) continue;
+    // all populations but the last park their counts in LDS as int16
+    const int64_t j = rj + (lane & 31);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int64_t i = ri + acc_row(r, lane);
-        int c[NC];
-        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(j, ba.n_var - 1),
-                                     pa.n_samples[p], c);
+    for (int r = 0; r < 16; ++r) {
+      const int64_t i = ri + acc_row(r, lane);
+      int c[NC];
+      elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(j, ba.n_var - 1),
+                                   pa.n_samples[p], c);
 #pragma unroll
-        for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
-      }
+      for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
     }
-    wave_products<FAMILY, MISSING>(pa.planes[P - 1], ba.n_var, pa.k_pad[P - 1], ri, rj, acc);
   }
 
   // ---- epilogue: statistics and keyed running sums ----
@@ -419,13 +520,16 @@ template <int FAMILY, bool MISSING>
 __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopArgs pa) {
   using PL = Planes<FAMILY, MISSING>;
   constexpr int NC = PL::NC;
-  const BandTile tile = ba.tiles[blockIdx.x];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int ri = tile.i0 + wr, rj = tile.j0 + wc;
-  if ((int64_t)rj >= ba.n_var || (int64_t)ri >= ba.n_var || rj + 31 <= ri) return;
+  const bool wave_live = (int64_t)rj < ba.n_var && (int64_t)ri < ba.n_var && rj + 31 > ri;
   WaveAcc<PL::N> acc;
-  wave_products<FAMILY, MISSING>(pa.planes[0], ba.n_var, pa.k_pad[0], ri, rj, acc);
+  block_products<FAMILY, MISSING>(pa.planes[0], ba.n_var, pa.k_pad[0], tile.i0, tile.j0, wr, wc, wave_live, smem,
+                                  acc);
+  if (!wave_live) return;
   const int64_t j = rj + (lane & 31);
   if (j >= ba.n_var) return;
   const double pj = ba.pos[j];
@@ -471,10 +575,11 @@ __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // Plane packing: (rows x variants) int8 -> [plane][variant][k_pad] 0/1 bytes
 // ---------------------------------------------------------------------------
-// Tile of 128 samples x 64 variants per block: rows are read as dwords
-// (4 variants) when the row stride allows, transposed through LDS, and each
-// (plane, variant) gets one full 128-byte line of sample bytes.
-constexpr int PK_S = 128, PK_V = 64;
+// Tile of 128 samples x 128 variants per block: every row segment is read as
+// one full 128-byte line (32 lanes x dword) when the row stride allows,
+// transposed through LDS, and each (plane, variant) gets one full 128-byte
+// line of sample bytes.
+constexpr int PK_S = 128, PK_V = 128;
 
 template <int FAMILY, bool WORDS>
 __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
@@ -486,12 +591,13 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   __shared__ uint32_t tile[PK_S][PK_V / 4 + 1];  // [sample][variant quad], padded
   __shared__ int msum[NPL][PK_V];
   const int tid = threadIdx.x;
+  static_assert(PK_V == 128 && PK_S == 128, "index math below assumes 128 x 128 tiles");
   const int64_t v0 = (int64_t)blockIdx.x * PK_V, k0 = (int64_t)blockIdx.y * PK_S;
-  if (tid < NPL * PK_V) msum[tid / PK_V][tid % PK_V] = 0;
+  for (int t = tid; t < NPL * PK_V; t += 256) msum[t / PK_V][t % PK_V] = 0;
   int any_neg = 0, any_bad = 0;
-  // 16 threads per row (one dword = 4 variants each), 16 rows per pass
-  const int q = tid & 15, rr = tid >> 4;
-  for (int s = rr; s < PK_S; s += 16) {
+  // 32 threads per row (one dword = 4 variants each), 8 rows per pass
+  const int q = tid & 31, rr = tid >> 5;
+  for (int s = rr; s < PK_S; s += 8) {
     uint32_t word = 0xffffffffu;  // padding samples read as missing
     if (k0 + s < n_sel) {
       const int64_t base = (int64_t)(rows ? rows[k0 + s] : k0 + s) * n_var + v0 + 4 * q;
@@ -559,8 +665,8 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
     }
   }
   __syncthreads();
-  if (tid < NPL * PK_V) {
-    const int p = tid / PK_V, vv = tid % PK_V;
+  for (int t = tid; t < NPL * PK_V; t += 256) {
+    const int p = t / PK_V, vv = t % PK_V;
     if (v0 + vv < n_var && msum[p][vv]) atomicAdd(&marg[(int64_t)p * n_var + v0 + vv], msum[p][vv]);
   }
   unsigned long long mneg = __ballot(any_neg), mbad = __ballot(any_bad);
@@ -832,7 +938,11 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
   const int nc = family == PGLD_FAMILY_HAP ? 4 : 9;
   const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
   const size_t stash_bytes = (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
-  if (red_bytes > (size_t)RED_LDS_BYTES || red_bytes + stash_bytes > 150 * 1024) {
+  const int n_planes_used = family == PGLD_FAMILY_HAP ? (with_missing ? 2 : 1) : (with_missing ? 3 : 2);
+  const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
+  // 160 KiB of LDS per workgroup on gfx950, minus the kernel's static arrays
+  if (red_bytes > (size_t)RED_LDS_BYTES ||
+      red_bytes + stash_bytes + op_bytes + 4 * BT * sizeof(double) > 160 * 1024) {
     set_error("pgld_band_sums: a tile touches %d keys; %d statistics x that many keys exceed LDS",
               plan->max_tile_slots, n_stats);
     return PGLD_EUNSUPPORTED;
@@ -862,7 +972,7 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
   ba.key_mode = plan->key_mode;
   ba.n_keys = plan->n_keys;
   ba.partials = pl->d_partials;
-  L(plan, ba, pa, red_bytes + stash_bytes, s);
+  L(plan, ba, pa, red_bytes + stash_bytes + op_bytes, s);
   rc = check_launch("k_band_sums");
   if (rc) return rc;
   hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)plan->n_keys), dim3(256), 0, s, pl->d_partials, plan->d_key_off,
@@ -889,12 +999,14 @@ int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* p
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   hipStream_t s = as_stream(stream);
   const bool miss = with_missing != 0;
+  const int npl = family == PGLD_FAMILY_HAP ? (miss ? 2 : 1) : (miss ? 3 : 2);
+  const size_t lds = (size_t)2 * npl * BT * OP_ROW;
   if (family == PGLD_FAMILY_HAP) {
-    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, true>), g, b, 0, s, ba, pa);
-    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, false>), g, b, 0, s, ba, pa);
+    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, true>), g, b, lds, s, ba, pa);
+    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, false>), g, b, lds, s, ba, pa);
   } else {
-    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, true>), g, b, 0, s, ba, pa);
-    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, false>), g, b, 0, s, ba, pa);
+    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, true>), g, b, lds, s, ba, pa);
+    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, false>), g, b, lds, s, ba, pa);
   }
   return check_launch("k_band_counts");
 }

@@ -102,20 +102,18 @@ PGLD_HD double num_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
 PGLD_HD double num_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
   return c.pB * c.qB * (a.pA * b.qA + a.qA * b.pA);
 }
-// pi2(i,j,i,j) (exactly symmetric in the two populations)
+// pi2(i,j,i,j) (exactly symmetric in the two populations).  The reference's
+// 4 quartic products and 36-monomial cubic correction (haplotype_kernels.py:
+// 243-266) regrouped by marginals: the quartic part factors as
+// (qA_a pA_b + pA_a qA_b)(qB_a pB_b + pB_a qB_b), and the cubic part pairs each
+// haplotype class of one population with the opposite class of the other.
+// Every term is an integer, so the value is exactly the reference's.
 PGLD_HD double num_pi2_ijij(const HapP& A, const HapP& B) {
-  const double a1 = A.c1, a2 = A.c2, a3 = A.c3, a4 = A.c4;
-  const double b1 = B.c1, b2 = B.c2, b3 = B.c3, b4 = B.c4;
-  double quart = A.qB * A.qA * B.pA * B.pB + A.pB * A.qA * B.pA * B.qB + A.pA * A.qB * B.pB * B.qA +
-                 A.pA * A.pB * B.qB * B.qA;
-  // cubic correction, grouped by the b-count it multiplies
-  double cub = b1 * (-(a2 * a3) + a4 - a2 * a4 - a3 * a4 - a4 * a4 - a4 * b1) +
-               b2 * (a3 - a1 * a3 - a3 * a3 - a1 * a4 - a3 * a4 - a3 * b1 - a4 * b1 - a3 * b2) +
-               b3 * (a2 - a1 * a2 - a2 * a2 - a1 * a4 - a2 * a4 - a2 * b1 - a4 * b1 - a1 * b2 - a4 * b2 -
-                     a2 * b3) +
-               b4 * (a1 - a1 * a1 - a1 * a2 - a1 * a3 - a2 * a3 - a2 * b1 - a3 * b1 - a1 * b2 - a3 * b2 -
-                     a1 * b3 - a2 * b3 - a1 * b4);
-  return (quart + cub) * 0.25;
+  const double quart = (A.qA * B.pA + A.pA * B.qA) * (A.qB * B.pB + A.pB * B.qB);
+  const double lin = A.c4 * B.c1 + A.c3 * B.c2 + A.c2 * B.c3 + A.c1 * B.c4;
+  const double qa = B.c1 * (A.qB * A.qA) + B.c2 * (A.pB * A.qA) + B.c3 * (A.pA * A.qB) + B.c4 * (A.pA * A.pB);
+  const double qb = A.c4 * (B.pA * B.pB) + A.c3 * (B.pA * B.qB) + A.c2 * (B.pB * B.qA) + A.c1 * (B.qB * B.qA);
+  return (quart + lin - qa - qb) * 0.25;
 }
 // one population shared between the two locus pairs (haplotype_kernels.py:218-241)
 PGLD_HD double num_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
