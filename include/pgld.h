@@ -63,6 +63,14 @@ const char* pgld_last_error(void);
 int pgld_scan_rows(const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
                    int64_t n_sel, int32_t max_value, int32_t* flags, void* stream);
 
+/* Per-variant allele counts over the selected rows: alt[v] = sum of the
+ * values > 0 (haplotypes: number of 1s; genotypes: dosage sum) and valid[v]
+ * = number of values >= 0 (int32, n_var each).  Feeds the biallelic filters
+ * (haplotype_matrix.py:724-789, genotype_matrix.py:506-539, moments_ld.py:
+ * 256-285) and the heterozygosity statistics (moments_ld.py:346-388). */
+int pgld_allele_counts(const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows, int64_t n_sel,
+                       int32_t* alt, int32_t* valid, void* stream);
+
 /* Pair enumeration of iter_pairs_within_distance (ld_pipeline.py:57-122).
  * per[v] = #{j > v : pos[j] - pos[v] <= max_dist}; positions sorted. */
 int pgld_pairs_per_variant(const double* pos, int64_t n, double max_dist, int64_t* per, void* stream);

@@ -893,12 +893,14 @@ def union_biallelic_keep_geno(geno, pop_rows):
     return np.nonzero(keep)[0]
 
 
-def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distances=None):
+def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distances=None, max_dist=None):
     """Per-bin FP64 sums of the ld_names(P) statistics (moments_ld.py:248-343).
 
     data: (n_rows, m) int8 matrix (already biallelic-filtered for haplotypes),
     pop_rows: list of row-index lists.  Haplotype mode with two populations
-    uses the fast path; otherwise the spec-weighted fused-kernel formulas."""
+    uses the fast path; otherwise the spec-weighted fused-kernel formulas.
+    ``distances`` (genetic map coordinates) key the bins by |d_j - d_i| over
+    pairs within the physical span ``max_dist`` (moments_ld.py:152-160)."""
     data = np.asarray(data)
     pos = np.asarray(positions)
     if use_genotypes:
@@ -910,7 +912,9 @@ def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distan
     bins = np.asarray(bins, dtype=np.float64)
     n_bins = len(bins) - 1
     P = len(pop_rows)
-    ii, jj = pairs_within_distance(pos, float(bins[-1]) if distances is None else np.inf)
+    if max_dist is None:
+        max_dist = float(bins[-1]) if distances is None else np.inf
+    ii, jj = pairs_within_distance(pos, max_dist)
     if distances is None:
         dist = pos[jj] - pos[ii]
     else:

@@ -29,6 +29,7 @@ _SIGNATURES = {
     "pgld_version": ([], C.c_int),
     "pgld_last_error": ([], C.c_char_p),
     "pgld_scan_rows": ([_vp, _i64, _i64, _vp, _i64, _i32, _vp, _vp], C.c_int),
+    "pgld_allele_counts": ([_vp, _i64, _i64, _vp, _i64, _vp, _vp, _vp], C.c_int),
     "pgld_pairs_per_variant": ([_vp, _i64, _f64, _vp, _vp], C.c_int),
     "pgld_pairs_fill": ([_vp, _i64, _i64, _i64, _vp, _vp, _vp], C.c_int),
     "pgld_counts_pairs_hap": ([_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp], C.c_int),

@@ -31,34 +31,49 @@ class PackedPop:
     marg: object         # int32 tensor (n_planes, n_var)
     k_pad: int
     n_samples: int
-    has_missing: bool
+    has_missing: object  # bool, or None when the flags were not read back
+    flags: object = None  # int32 tensor [any missing, any out of domain]
 
 
-def pack_population(data, rows=None, family="hap", k_pad=None):
-    """Pack one population's rows of a (rows, n_var) int8 matrix."""
+def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, out=None):
+    """Pack one population's rows of a (rows, n_var) int8 matrix.
+
+    validate=True reads the kernel's value flags back (one small sync) to
+    reject out-of-domain values and report missing data; validate=False skips
+    that read for inputs already checked (has_missing is then None).  ``out``
+    may pass a previous PackedPop of the same shape whose buffers are reused."""
     fam = L.FAMILY_HAP if family == "hap" else L.FAMILY_GENO
     g = L.to_device(data, np.int8)
     n_rows, n_var = g.shape
     rows_t = None
     n_sel = n_rows
     if rows is not None:
-        r = np.asarray(L.to_numpy(rows), dtype=np.int64)
-        if r.size and (r.min() < 0 or r.max() >= n_rows):
-            raise IndexError("population row index out of range")
-        rows_t = L.to_device(r, np.int32)
-        n_sel = int(r.size)
+        T = L.torch()
+        if not validate and L.is_tensor(rows) and rows.is_cuda and rows.dtype == T.int32 and rows.dim() == 1:
+            rows_t, n_sel = rows, int(rows.shape[0])  # trusted device row list: no host round trip
+        else:
+            r = np.asarray(L.to_numpy(rows), dtype=np.int64)
+            if r.size and (r.min() < 0 or r.max() >= n_rows):
+                raise IndexError("population row index out of range")
+            rows_t = L.to_device(r, np.int32)
+            n_sel = int(r.size)
     kp = k_pad_for(n_sel) if k_pad is None else int(k_pad)
     npl = 2 if fam == L.FAMILY_HAP else 3
-    planes = L.empty((npl, n_var, kp), np.int8)
-    marg = L.empty((npl, n_var), np.int32)
-    flags = L.zeros((2,), np.int32)
+    if out is not None and tuple(out.planes.shape) == (npl, n_var, kp):
+        planes, marg, flags = out.planes, out.marg, out.flags
+    else:
+        planes = L.empty((npl, n_var, kp), np.int8)
+        marg = L.empty((npl, n_var), np.int32)
+        flags = L.empty((2,), np.int32)
     L.call("pgld_pack_planes", fam, L.ptr(g), n_rows, n_var, L.ptr(rows_t), n_sel, kp, L.ptr(planes),
            L.ptr(marg), L.ptr(flags), L.stream_handle())
+    if not validate:
+        return PackedPop(fam, planes, marg, kp, n_sel, None, flags)
     f = flags.cpu().numpy()
     if f[1]:
         raise ValueError("input values outside {-1, 0, 1}" if fam == L.FAMILY_HAP
                          else "genotype values outside {-1, 0, 1, 2}")
-    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]))
+    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]), flags)
 
 
 class BandPlan:
@@ -120,6 +135,8 @@ class BandPlan:
         if any(p.family != fam for p in pops):
             raise ValueError("all populations must be of one family")
         if with_missing is None:
+            if any(p.has_missing is None for p in pops):
+                raise ValueError("with_missing must be given for populations packed with validate=False")
             with_missing = any(p.has_missing for p in pops)
         sset = L.SET_MOMENTS if stat_set == "moments" else L.SET_SELF
         if sset == L.SET_SELF and P > 1:

@@ -43,6 +43,23 @@ __global__ void k_scan_rows(const int8_t* __restrict__ data, int64_t n_var, cons
   }
 }
 
+// Per-variant allele counts over selected rows: alt[v] = sum of values > 0
+// (haplotypes: number of 1s; genotypes: dosage sum), valid[v] = number of
+// values >= 0.  One thread per variant, rows walked in order.
+__global__ void k_allele_counts(const int8_t* __restrict__ data, int64_t n_var, const int32_t* __restrict__ rows,
+                                int64_t n_sel, int32_t* __restrict__ alt, int32_t* __restrict__ valid) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_var) return;
+  int a = 0, n = 0;
+  for (int64_t k = 0; k < n_sel; ++k) {
+    const int x = data[(int64_t)(rows ? rows[k] : k) * n_var + v];
+    a += x > 0 ? x : 0;
+    n += x >= 0;
+  }
+  alt[v] = a;
+  valid[v] = n;
+}
+
 // ---------------------------------------------------------------------------
 // Pair enumeration
 // ---------------------------------------------------------------------------
@@ -173,6 +190,17 @@ int pgld_scan_rows(const int8_t* data, int64_t n_rows, int64_t n_var, const int3
   hipLaunchKernelGGL(k_scan_rows, dim3((unsigned)blocks), dim3(256), 0, s, data, n_var, rows, n_sel, max_value,
                      flags);
   return check_launch("k_scan_rows");
+}
+
+int pgld_allele_counts(const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows, int64_t n_sel,
+                       int32_t* alt, int32_t* valid, void* stream) {
+  PGLD_REQUIRE(n_rows >= 0 && n_var >= 0, "pgld_allele_counts: negative size");
+  if (!rows) n_sel = n_rows;
+  if (n_var == 0) return PGLD_OK;
+  PGLD_REQUIRE(data && alt && valid, "pgld_allele_counts: null pointer");
+  hipLaunchKernelGGL(k_allele_counts, dim3((unsigned)ceil_div(n_var, 256)), dim3(256), 0, as_stream(stream), data,
+                     n_var, rows, n_sel, alt, valid);
+  return check_launch("k_allele_counts");
 }
 
 int pgld_pairs_per_variant(const double* pos, int64_t n, double max_dist, int64_t* per, void* stream) {

@@ -54,6 +54,9 @@ constexpr int BT = 64;       // tile edge (SNPs)
 constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
 constexpr int BAND_THREADS = 256;
 constexpr int RED_LDS_BYTES = 64 * 1024;
+constexpr int MAX_LDS_EDGES = 130;  // bin edges kept in LDS when n_keys < 130
+// static LDS of k_band_sums: positions, key coordinates, window keys, edges
+constexpr int STATIC_LDS_BYTES = 4 * 64 * 8 + 64 * 4 + MAX_LDS_EDGES * 8;
 
 struct BandTile {
   int32_t i0, j0, key_lo, n_slots;
@@ -93,62 +96,6 @@ template <int N>
 struct WaveAcc {
   v16i c[N][N];
 };
-
-// One wave: products of its 32 A-rows and 32 B-cols over all K of one
-// population.  Lane l reads 64 contiguous bytes of row (l % 32) at K offset
-// 64 * (l / 32) per 128-byte step; A and B use the same lane -> K assignment,
-// so every K byte is paired with itself exactly once.
-template <int FAMILY, bool MISSING>
-__device__ __forceinline__ void wave_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
-                                              int ri, int rj, WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
-  using PL = Planes<FAMILY, MISSING>;
-  constexpr int N = PL::N;
-  const int lane = threadIdx.x & 63;
-  const int64_t si = min((int64_t)(ri + (lane & 31)), n_var - 1);
-  const int64_t sj = min((int64_t)(rj + (lane & 31)), n_var - 1);
-  const int64_t kq = (lane >> 5) * 64;
-  const int8_t* pa[N];
-  const int8_t* pb[N];
-#pragma unroll
-  for (int a = 0; a < N; ++a) {
-    pa[a] = planes + ((int64_t)(PL::FIRST + a) * n_var + si) * k_pad + kq;
-    pb[a] = planes + ((int64_t)(PL::FIRST + a) * n_var + sj) * k_pad + kq;
-  }
-#pragma unroll
-  for (int a = 0; a < N; ++a)
-#pragma unroll
-    for (int b = 0; b < N; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
-  // One operand buffer used as a rolling pipeline: chunk t of step k+1 is
-  // requested right after the MFMAs that consume chunk t of step k, so a
-  // whole step of loads is in flight behind the MFMAs.
-  v4i A[N][4], B[N][4];
-#pragma unroll
-  for (int a = 0; a < N; ++a)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + 16 * t);
-      B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + 16 * t);
-    }
-  for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
-    const bool more = k0 + KSTEP < k_pad;
-    const int64_t kn = more ? k0 + KSTEP : k0;  // re-read (harmless) on the last step
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-      for (int a = 0; a < N; ++a)
-#pragma unroll
-        for (int b = 0; b < N; ++b)
-          acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a][t], B[b][t], acc.c[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < N; ++a) {
-        A[a][t] = *reinterpret_cast<const v4i*>(pa[a] + kn + 16 * t);
-        B[a][t] = *reinterpret_cast<const v4i*>(pb[a] + kn + 16 * t);
-      }
-    }
-  }
-}
 
 // LDS-staged products for the whole workgroup (all four waves must call it:
 // it contains barriers).  Per 128-byte K chunk the 64 A rows and 64 B rows of
@@ -387,6 +334,8 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
+  __shared__ int s_wkey[BT];
+  __shared__ double s_edges[MAX_LDS_EDGES];
 
   const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   if (tile.n_slots <= 0) return;
@@ -402,8 +351,16 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
     s_pos_j[tid] = ba.pos[gj];
     s_kc_i[tid] = ba.kc[gi];
     s_kc_j[tid] = ba.kc[gj];
+    if (ba.key_mode == PGLD_KEY_WINDOW) {
+      const double wk = floor((ba.kc[gi] - ba.origin) / ba.width);
+      s_wkey[tid] = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
+    }
   }
+  const bool edges_in_lds = ba.key_mode != PGLD_KEY_WINDOW && ba.n_keys < MAX_LDS_EDGES;
+  if (edges_in_lds)
+    for (int t = tid; t <= ba.n_keys; t += BAND_THREADS) s_edges[t] = ba.edges[t];
   __syncthreads();
+  const double* edges = edges_in_lds ? s_edges : ba.edges;
 
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int ri = tile.i0 + wr, rj = tile.j0 + wc;
@@ -472,12 +429,11 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
       int key = -1;
       if (valid) {
         if (ba.key_mode == PGLD_KEY_WINDOW) {
-          double wk = floor((s_kc_i[il] - ba.origin) / ba.width);
-          key = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
+          key = s_wkey[il];
         } else {
           double d = s_kc_j[jl] - s_kc_i[il];
           if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
-          key = key_bin(d, ba.edges, ba.n_keys);
+          key = key_bin(d, edges, ba.n_keys);
         }
       }
       const bool has = valid && key >= 0;
@@ -590,27 +546,40 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   constexpr int maxv = FAMILY == PGLD_FAMILY_HAP ? 1 : 2;
   __shared__ uint32_t tile[PK_S][PK_V / 4 + 1];  // [sample][variant quad], padded
   __shared__ int msum[NPL][PK_V];
+  __shared__ int64_t srow[PK_S];
   const int tid = threadIdx.x;
   static_assert(PK_V == 128 && PK_S == 128, "index math below assumes 128 x 128 tiles");
   const int64_t v0 = (int64_t)blockIdx.x * PK_V, k0 = (int64_t)blockIdx.y * PK_S;
   for (int t = tid; t < NPL * PK_V; t += 256) msum[t / PK_V][t % PK_V] = 0;
+  if (tid < PK_S) srow[tid] = k0 + tid < n_sel ? (int64_t)(rows ? rows[k0 + tid] : k0 + tid) * n_var : -1;
+  __syncthreads();
   int any_neg = 0, any_bad = 0;
-  // 32 threads per row (one dword = 4 variants each), 8 rows per pass
+  // 32 threads per row (one dword = 4 variants each), 8 rows per pass; all 16
+  // loads of a thread are independent and issued back to back
   const int q = tid & 31, rr = tid >> 5;
-  for (int s = rr; s < PK_S; s += 8) {
+  uint32_t words_in[PK_S / 8];
+#pragma unroll
+  for (int u = 0; u < PK_S / 8; ++u) {
+    const int64_t base = srow[rr + 8 * u];
     uint32_t word = 0xffffffffu;  // padding samples read as missing
-    if (k0 + s < n_sel) {
-      const int64_t base = (int64_t)(rows ? rows[k0 + s] : k0 + s) * n_var + v0 + 4 * q;
+    if (base >= 0) {
+      const int64_t at = base + v0 + 4 * q;
       if (WORDS && v0 + 4 * q + 3 < n_var) {
-        word = *reinterpret_cast<const uint32_t*>(data + base);
+        word = *reinterpret_cast<const uint32_t*>(data + at);
       } else {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          uint32_t byte = 0xffu;
-          if (v0 + 4 * q + b < n_var) byte = (uint8_t)data[base + b];
+          const uint32_t byte = v0 + 4 * q + b < n_var ? (uint8_t)data[at + b] : 0xffu;
           word = (word & ~(0xffu << (8 * b))) | (byte << (8 * b));
         }
       }
+    }
+    words_in[u] = word;
+  }
+#pragma unroll
+  for (int u = 0; u < PK_S / 8; ++u) {
+    const uint32_t word = words_in[u];
+    if (srow[rr + 8 * u] >= 0) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         if (v0 + 4 * q + b < n_var) {
@@ -620,7 +589,7 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
         }
       }
     }
-    tile[s][q] = word;
+    tile[rr + 8 * u][q] = word;
   }
   __syncthreads();
   // 8 lanes per (variant): each builds 16 sample bytes of every plane
@@ -942,7 +911,7 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
   const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
   // 160 KiB of LDS per workgroup on gfx950, minus the kernel's static arrays
   if (red_bytes > (size_t)RED_LDS_BYTES ||
-      red_bytes + stash_bytes + op_bytes + 4 * BT * sizeof(double) > 160 * 1024) {
+      red_bytes + stash_bytes + op_bytes + STATIC_LDS_BYTES > 160 * 1024) {
     set_error("pgld_band_sums: a tile touches %d keys; %d statistics x that many keys exceed LDS",
               plan->max_tile_slots, n_stats);
     return PGLD_EUNSUPPORTED;

@@ -176,11 +176,14 @@ PGLD_HD double hap_pi2_alldiff(const HapP& a, const HapP& b, const HapP& c, cons
 struct HapR {
   double i1, i2, i3, i4;  // 1/n, 1/(n)_2, 1/(n)_3, 1/(n)_4
 };
-PGLD_HD HapR hap_recips(double n) {
+__device__ __forceinline__ HapR hap_recips(double n) {
   HapR r;
   const double g2 = n * (n - 1.0), g3 = g2 * (n - 2.0), g4 = g3 * (n - 3.0);
   if (g4 > 0.0) {
-    r.i4 = 1.0 / g4;
+    // v_rcp_f64 refined by one Newton step: within an ulp of 1 / g4
+    double q = __builtin_amdgcn_rcp(g4);
+    q = fma(fma(-g4, q, 1.0), q, q);
+    r.i4 = q;
     r.i3 = r.i4 * (n - 3.0);
     r.i2 = r.i3 * (n - 2.0);
     r.i1 = r.i2 * (n - 1.0);

@@ -1,0 +1,221 @@
+"""GPU parity of the drop-in layer: HaplotypeMatrix, GenotypeMatrix and
+moments_ld.compute_ld_statistics against the frozen moments output of the
+reference's IM example and against the oracle.
+
+Bin sums are float64 reductions in a different order than the oracle's, so
+they are compared with rtol 1e-10 on the golden vector (the precision it was
+frozen at) and rtol 1e-12 against max(|sum|, sum of |terms|) elsewhere.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from pg_gpu_amd import GenotypeMatrix, HaplotypeMatrix, moments_ld  # noqa: E402
+from pg_gpu_amd import ld_statistics_genotype as LSG  # noqa: E402
+from pg_gpu_amd import synthetic as SY  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+VCF = os.path.join(GOLD, "im-parsing-example.vcf")
+POPS = os.path.join(GOLD, "im_pop.txt")
+
+
+def np_(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
+def close_sums(got, want, rtol=1e-12, scale=None):
+    got, want = np.asarray(got), np.asarray(want)
+    s = np.maximum(np.abs(want), 1.0) if scale is None else scale
+    err = np.max(np.abs(got - want) / s) if want.size else 0.0
+    assert err <= rtol, f"max scaled error {err:.3e}"
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("method", ["auto", "plain"])
+def test_moments_golden_im_example(method):
+    G = json.load(open(os.path.join(GOLD, "moments_im_4bins.json")))
+    bins = np.array([G["bins"][0][0]] + [b[1] for b in G["bins"]])
+    res = moments_ld.compute_ld_statistics(VCF, pop_file=POPS, pops=G["pops"], bp_bins=bins, report=False,
+                                           method=method)
+    assert res["stats"][0] == G["ld_names"] and list(res["stats"][1]) == G["het_names"]
+    assert [list(b) for b in res["bins"]] == G["bins"]
+    np.testing.assert_allclose(np.array(res["sums"][:-1]), np.array(G["sums"]), rtol=1e-10)
+    np.testing.assert_allclose(res["sums"][-1], G["het"], rtol=1e-13)
+
+
+def _im_geno():
+    gm = GenotypeMatrix.from_vcf(VCF)
+    gm.load_pop_file(POPS, pops=["deme0", "deme1"])
+    return gm.apply_biallelic_filter()
+
+
+@pytest.mark.parametrize("method", ["auto", "plain"])
+def test_moments_genotypes_im_example(method):
+    bins = np.logspace(2, 5, 4)
+    gm = _im_geno()
+    g = np_(gm.genotypes)
+    pos = np_(gm.positions)
+    rows = [gm.sample_sets["deme0"], gm.sample_sets["deme1"]]
+    want = O.moments_ld_sums(g, pos, rows, bins, use_genotypes=True)
+    res = moments_ld.compute_ld_statistics(genotype_matrix=gm, pops=["deme0", "deme1"], bp_bins=bins,
+                                           use_genotypes=True, report=False, method=method)
+    close_sums(np.array(res["sums"][:-1]), want)
+    H = O.heterozygosity([g[r] for r in rows], use_genotypes=True)
+    np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
+
+
+@pytest.mark.parametrize("P,method", [(1, "auto"), (3, "auto"), (3, "plain"), (4, "auto")])
+def test_moments_hap_pop_counts(P, method):
+    """1, 3 and 4 populations on a synthetic matrix with missing data (4
+    populations take the per-pair path)."""
+    rng = np.random.default_rng(40 + P)
+    h = SY.neutral_haplotypes(24 * P, 300, rng, missing=0.03)
+    pos = SY.uniform_positions(300, 60000, rng)
+    names = [f"p{k}" for k in range(P)]
+    sets = {n: list(range(24 * k, 24 * (k + 1))) for k, n in enumerate(names)}
+    hm = HaplotypeMatrix(h, pos, sample_sets=sets)
+    bins = np.array([0.0, 500.0, 2000.0, 8000.0])
+    res = moments_ld.compute_ld_statistics(haplotype_matrix=hm, pops=names, bp_bins=bins, report=False,
+                                           method=method)
+    want = O.moments_ld_sums(h, pos, [sets[n] for n in names], bins)
+    close_sums(np.array(res["sums"][:-1]), want)
+    H = O.heterozygosity([h[sets[n]] for n in names])
+    np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
+
+
+@pytest.mark.parametrize("use_genotypes", [False, True])
+def test_moments_r_bins(tmp_path, use_genotypes):
+    """r_bins: pairs within the physical span derived from the map, keyed by
+    the genetic distance |g_j - g_i| in Morgans."""
+    rng = np.random.default_rng(77)
+    n_var = 400
+    pos = SY.uniform_positions(n_var, 200000, rng)
+    mp = np.linspace(0, 210000, 12)
+    cm = np.cumsum(rng.uniform(0.05, 0.4, size=12))
+    rec = tmp_path / "rec.map"
+    rec.write_text("pos\tcM\n" + "".join(f"{a:.1f}\t{b:.6f}\n" for a, b in zip(mp, cm)))
+    r_bins = np.array([0.0, 1e-4, 5e-4, 2e-3])
+    if use_genotypes:
+        g = SY.hwe_genotypes(60, n_var, rng, missing=0.02)
+        sets = {"a": list(range(30)), "b": list(range(30, 60))}
+        mat = GenotypeMatrix(g, pos, sample_sets=sets)
+        kw = dict(genotype_matrix=mat)
+    else:
+        g = SY.neutral_haplotypes(80, n_var, rng, missing=0.02)
+        sets = {"a": list(range(40)), "b": list(range(40, 80))}
+        mat = HaplotypeMatrix(g, pos, sample_sets=sets)
+        kw = dict(haplotype_matrix=mat)
+    gen = np.interp(pos.astype(np.float64), mp, cm / 100.0)
+    max_bp = moments_ld._max_bp_for_r_dist(pos.astype(np.float64), gen, float(r_bins[-1]))
+    res = moments_ld.compute_ld_statistics(rec_map_file=str(rec), pops=["a", "b"], r_bins=r_bins,
+                                           use_genotypes=use_genotypes, report=False, **kw)
+    want = O.moments_ld_sums(g, pos, [sets["a"], sets["b"]], r_bins, use_genotypes=use_genotypes,
+                             distances=gen, max_dist=max_bp)
+    close_sums(np.array(res["sums"][:-1]), want)
+
+
+# ---------------------------------------------------------------------------
+def test_haplotype_matrix_vcf_layout_and_filters():
+    hm = HaplotypeMatrix.from_vcf(VCF)
+    hm.load_pop_file(POPS, pops=["deme0", "deme1"])
+    n = len(hm.samples)
+    for rows in hm.sample_sets.values():
+        assert all((r + n in rows) for r in rows if r < n)
+    h = np_(hm.haplotypes)
+    hm.transfer_to_gpu()
+    alt, valid = (np_(a) for a in hm.allele_counts(hm.sample_sets["deme0"]))
+    X = h[hm.sample_sets["deme0"]]
+    np.testing.assert_array_equal(alt, (X == 1).sum(axis=0))
+    np.testing.assert_array_equal(valid, (X >= 0).sum(axis=0))
+    f = hm.apply_biallelic_filter()
+    np.testing.assert_array_equal(np_(f.positions), np_(hm.positions)[O.biallelic_filter_hap(h)])
+
+
+def test_haplotype_matrix_tally_and_bin_stats():
+    rng = np.random.default_rng(3)
+    h = SY.neutral_haplotypes(64, 120, rng, missing=0.0)
+    pos = SY.uniform_positions(120, 30000, rng)
+    hm = HaplotypeMatrix(h, pos, sample_sets={"x": list(range(32)), "y": list(range(32, 64))})
+    c, nv = hm.tally_gpu_haplotypes()
+    iu = np.triu_indices(120, k=1)
+    want, _ = O.hap_counts(h, iu[0], iu[1], None)
+    np.testing.assert_array_equal(np_(c), want)
+    c2, n1, n2 = hm.tally_gpu_haplotypes_two_pops("x", "y")
+    np.testing.assert_array_equal(np_(c2)[:, :4], O.hap_counts(h, iu[0], iu[1], list(range(32)))[0])
+    np.testing.assert_array_equal(np_(c2)[:, 4:], O.hap_counts(h, iu[0], iu[1], list(range(32, 64)))[0])
+    bins = [0.0, 1000.0, 5000.0, 20000.0]
+    one = hm.compute_ld_statistics_gpu_single_pop(bins, raw=True)
+    want1 = O.moments_ld_sums(h[:, O.biallelic_filter_hap(h)], pos[O.biallelic_filter_hap(h)],
+                              [list(range(64))], np.array(bins))
+    close_sums(np.array([one[k] for k in sorted(one)]), want1)
+    two = hm.compute_ld_statistics_gpu_two_pops(bins, "x", "y", raw=True)
+    keep = O.biallelic_filter_hap(h)
+    want2 = O.moments_ld_sums(h[:, keep], pos[keep], [list(range(32)), list(range(32, 64))], np.array(bins))
+    close_sums(np.array([list(two[k].values()) for k in sorted(two)]), want2)
+
+
+def test_haplotype_matrix_windowed_sums():
+    rng = np.random.default_rng(9)
+    h = SY.neutral_haplotypes(50, 500, rng, missing=0.02)
+    pos = SY.uniform_positions(500, 100000, rng)
+    hm = HaplotypeMatrix(h, pos, sample_sets={"a": list(range(25)), "b": list(range(25, 50))})
+    sums, kp, names = hm.windowed_ld_sums(10000.0, pops=["a", "b"])
+    ii, jj = O.pairs_within_distance(pos, 10000.0)
+    key = O.window_key(pos[ii], 0.0, 10000.0)
+    cl = [O.hap_counts(h, ii, jj, r) for r in (list(range(25)), list(range(25, 50)))]
+    st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+    nk = sums.shape[0]
+    close_sums(np_(sums), O.keyed_sums(key, st, nk))
+    np.testing.assert_array_equal(np_(kp), np.bincount(key, minlength=nk)[:nk])
+    assert names == O.ld_names(2)
+
+
+def test_genotype_matrix_roundtrip_and_bins():
+    rng = np.random.default_rng(12)
+    h = SY.neutral_haplotypes(60, 200, rng, missing=0.0)
+    pos = SY.uniform_positions(200, 50000, rng)
+    hm = HaplotypeMatrix(h, pos, sample_sets={"a": list(range(30)), "b": list(range(30, 60))})
+    gm = GenotypeMatrix.from_haplotype_matrix(hm)
+    g = np_(gm.genotypes)
+    np.testing.assert_array_equal(g, h[0::2] + h[1::2])
+    assert gm.sample_sets["a"] == list(range(15))
+    back = gm.to_haplotype_matrix()
+    np.testing.assert_array_equal(np_(back.haplotypes)[0::2] + np_(back.haplotypes)[1::2], g)
+    bins = [0.0, 2000.0, 10000.0]
+    sums, kp, names = gm.ld_bin_sums(bins, pops=["a", "b"])
+    ii, jj = O.pairs_within_distance(pos, 10000.0)
+    key = O.bin_index(pos[jj] - pos[ii], np.array(bins))
+    cl = [O.geno_counts(g, ii, jj, r) for r in (list(range(15)), list(range(15, 30)))]
+    st = O.multi_pop_stats_geno([c for c, _ in cl], [nv for _, nv in cl])
+    close_sums(np_(sums), O.keyed_sums(key, st, 2))
+    np.testing.assert_array_equal(np_(kp), np.bincount(key[key >= 0], minlength=2)[:2])
+    assert names == O.ld_names(2)
+
+
+def test_sigma_d2_geno():
+    rng = np.random.default_rng(21)
+    g = SY.hwe_genotypes(40, 60, rng, missing=0.0)
+    gm = GenotypeMatrix(g, np.arange(60) * 10)
+    got = np_(LSG.sigma_d2_geno(gm))
+    iu = np.triu_indices(60, k=1)
+    c, _ = O.geno_counts(g, iu[0], iu[1], None)
+    want = O.sigma_d2_geno_from_counts(c)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_allclose(got[ok], want[ok], rtol=1e-12)
+    gmiss = g.copy()
+    gmiss[0, 0] = -1
+    with pytest.raises(ValueError):
+        LSG.sigma_d2_geno(GenotypeMatrix(gmiss, np.arange(60) * 10))
+    with pytest.raises(TypeError):
+        LSG.sigma_d2_geno(g)
