@@ -893,14 +893,17 @@ def union_biallelic_keep_geno(geno, pop_rows):
     return np.nonzero(keep)[0]
 
 
-def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distances=None, max_dist=None):
+def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distances=None, max_dist=None,
+                    absolute=False):
     """Per-bin FP64 sums of the ld_names(P) statistics (moments_ld.py:248-343).
 
     data: (n_rows, m) int8 matrix (already biallelic-filtered for haplotypes),
     pop_rows: list of row-index lists.  Haplotype mode with two populations
     uses the fast path; otherwise the spec-weighted fused-kernel formulas.
     ``distances`` (genetic map coordinates) key the bins by |d_j - d_i| over
-    pairs within the physical span ``max_dist`` (moments_ld.py:152-160)."""
+    pairs within the physical span ``max_dist`` (moments_ld.py:152-160).
+    ``absolute`` sums |statistic| instead: the scale against which a sum
+    taken in another order is compared."""
     data = np.asarray(data)
     pos = np.asarray(positions)
     if use_genotypes:
@@ -934,7 +937,7 @@ def moments_ld_sums(data, positions, pop_rows, bins, use_genotypes=False, distan
         stats = two_pop_stats(cl[0], cl[1], nl[0], nl[1])
     else:
         stats = multi_pop_stats_hap(cl, nl)
-    return keyed_sums(key, stats, n_bins)
+    return keyed_sums(key, np.abs(stats) if absolute else stats, n_bins)
 
 
 # ---------------------------------------------------------------------------

@@ -4,7 +4,9 @@ reference's IM example and against the oracle.
 
 Bin sums are float64 reductions in a different order than the oracle's, so
 they are compared with rtol 1e-10 on the golden vector (the precision it was
-frozen at) and rtol 1e-12 against max(|sum|, sum of |terms|) elsewhere.
+frozen at) and otherwise against max(|sum|, sum of |terms|): rtol 1e-12 for
+haplotype statistics and 1e-11 for genotype statistics, whose per-pair values
+already agree only to ~1e-11 (their polynomials cancel within a pair).
 """
 import json
 import os
@@ -34,8 +36,10 @@ def np_(t):
 
 
 def close_sums(got, want, rtol=1e-12, scale=None):
+    """|got - want| <= rtol * scale, scale = max(|want|, 1) or the sum of
+    |terms| where the sum cancels."""
     got, want = np.asarray(got), np.asarray(want)
-    s = np.maximum(np.abs(want), 1.0) if scale is None else scale
+    s = np.maximum(np.abs(want), 1.0) if scale is None else np.maximum(np.asarray(scale), 1.0)
     err = np.max(np.abs(got - want) / s) if want.size else 0.0
     assert err <= rtol, f"max scaled error {err:.3e}"
 
@@ -67,9 +71,10 @@ def test_moments_genotypes_im_example(method):
     pos = np_(gm.positions)
     rows = [gm.sample_sets["deme0"], gm.sample_sets["deme1"]]
     want = O.moments_ld_sums(g, pos, rows, bins, use_genotypes=True)
+    scale = O.moments_ld_sums(g, pos, rows, bins, use_genotypes=True, absolute=True)
     res = moments_ld.compute_ld_statistics(genotype_matrix=gm, pops=["deme0", "deme1"], bp_bins=bins,
                                            use_genotypes=True, report=False, method=method)
-    close_sums(np.array(res["sums"][:-1]), want)
+    close_sums(np.array(res["sums"][:-1]), want, rtol=1e-11, scale=scale)
     H = O.heterozygosity([g[r] for r in rows], use_genotypes=True)
     np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
 
@@ -88,7 +93,8 @@ def test_moments_hap_pop_counts(P, method):
     res = moments_ld.compute_ld_statistics(haplotype_matrix=hm, pops=names, bp_bins=bins, report=False,
                                            method=method)
     want = O.moments_ld_sums(h, pos, [sets[n] for n in names], bins)
-    close_sums(np.array(res["sums"][:-1]), want)
+    scale = O.moments_ld_sums(h, pos, [sets[n] for n in names], bins, absolute=True)
+    close_sums(np.array(res["sums"][:-1]), want, scale=scale)
     H = O.heterozygosity([h[sets[n]] for n in names])
     np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
 
@@ -119,9 +125,10 @@ def test_moments_r_bins(tmp_path, use_genotypes):
     max_bp = moments_ld._max_bp_for_r_dist(pos.astype(np.float64), gen, float(r_bins[-1]))
     res = moments_ld.compute_ld_statistics(rec_map_file=str(rec), pops=["a", "b"], r_bins=r_bins,
                                            use_genotypes=use_genotypes, report=False, **kw)
-    want = O.moments_ld_sums(g, pos, [sets["a"], sets["b"]], r_bins, use_genotypes=use_genotypes,
-                             distances=gen, max_dist=max_bp)
-    close_sums(np.array(res["sums"][:-1]), want)
+    kw = dict(use_genotypes=use_genotypes, distances=gen, max_dist=max_bp)
+    want = O.moments_ld_sums(g, pos, [sets["a"], sets["b"]], r_bins, **kw)
+    scale = O.moments_ld_sums(g, pos, [sets["a"], sets["b"]], r_bins, absolute=True, **kw)
+    close_sums(np.array(res["sums"][:-1]), want, rtol=1e-11 if use_genotypes else 1e-12, scale=scale)
 
 
 # ---------------------------------------------------------------------------
@@ -135,7 +142,7 @@ def test_haplotype_matrix_vcf_layout_and_filters():
     hm.transfer_to_gpu()
     alt, valid = (np_(a) for a in hm.allele_counts(hm.sample_sets["deme0"]))
     X = h[hm.sample_sets["deme0"]]
-    np.testing.assert_array_equal(alt, (X == 1).sum(axis=0))
+    np.testing.assert_array_equal(alt, np.maximum(X, 0).astype(np.int64).sum(axis=0))
     np.testing.assert_array_equal(valid, (X >= 0).sum(axis=0))
     f = hm.apply_biallelic_filter()
     np.testing.assert_array_equal(np_(f.positions), np_(hm.positions)[O.biallelic_filter_hap(h)])
@@ -155,13 +162,14 @@ def test_haplotype_matrix_tally_and_bin_stats():
     np.testing.assert_array_equal(np_(c2)[:, 4:], O.hap_counts(h, iu[0], iu[1], list(range(32, 64)))[0])
     bins = [0.0, 1000.0, 5000.0, 20000.0]
     one = hm.compute_ld_statistics_gpu_single_pop(bins, raw=True)
-    want1 = O.moments_ld_sums(h[:, O.biallelic_filter_hap(h)], pos[O.biallelic_filter_hap(h)],
-                              [list(range(64))], np.array(bins))
-    close_sums(np.array([one[k] for k in sorted(one)]), want1)
-    two = hm.compute_ld_statistics_gpu_two_pops(bins, "x", "y", raw=True)
     keep = O.biallelic_filter_hap(h)
-    want2 = O.moments_ld_sums(h[:, keep], pos[keep], [list(range(32)), list(range(32, 64))], np.array(bins))
-    close_sums(np.array([list(two[k].values()) for k in sorted(two)]), want2)
+    args1 = (h[:, keep], pos[keep], [list(range(64))], np.array(bins))
+    close_sums(np.array([one[k] for k in sorted(one)]), O.moments_ld_sums(*args1),
+               scale=O.moments_ld_sums(*args1, absolute=True))
+    two = hm.compute_ld_statistics_gpu_two_pops(bins, "x", "y", raw=True)
+    args2 = (h[:, keep], pos[keep], [list(range(32)), list(range(32, 64))], np.array(bins))
+    close_sums(np.array([list(two[k].values()) for k in sorted(two)]), O.moments_ld_sums(*args2),
+               scale=O.moments_ld_sums(*args2, absolute=True))
 
 
 def test_haplotype_matrix_windowed_sums():
@@ -175,7 +183,7 @@ def test_haplotype_matrix_windowed_sums():
     cl = [O.hap_counts(h, ii, jj, r) for r in (list(range(25)), list(range(25, 50)))]
     st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
     nk = sums.shape[0]
-    close_sums(np_(sums), O.keyed_sums(key, st, nk))
+    close_sums(np_(sums), O.keyed_sums(key, st, nk), scale=O.keyed_sums(key, np.abs(st), nk))
     np.testing.assert_array_equal(np_(kp), np.bincount(key, minlength=nk)[:nk])
     assert names == O.ld_names(2)
 
@@ -197,7 +205,7 @@ def test_genotype_matrix_roundtrip_and_bins():
     key = O.bin_index(pos[jj] - pos[ii], np.array(bins))
     cl = [O.geno_counts(g, ii, jj, r) for r in (list(range(15)), list(range(15, 30)))]
     st = O.multi_pop_stats_geno([c for c, _ in cl], [nv for _, nv in cl])
-    close_sums(np_(sums), O.keyed_sums(key, st, 2))
+    close_sums(np_(sums), O.keyed_sums(key, st, 2), rtol=1e-11, scale=O.keyed_sums(key, np.abs(st), 2))
     np.testing.assert_array_equal(np_(kp), np.bincount(key[key >= 0], minlength=2)[:2])
     assert names == O.ld_names(2)
 
