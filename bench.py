@@ -28,6 +28,12 @@ sys.path.insert(0, ROOT)
 PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
 MACS_PER_PAIR_SAMPLE = 4  # haplotypes with missing data: hh, hv, vh, vv
+# HBM bytes per k_band_sums launch on the default workload, from rocprofv3
+# FETCH_SIZE (2,187,585 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
+# by the same counters' reading of a 1 GiB device copy (--copy-calibration:
+# FETCH x 2.000, WRITE x 1.000).  Summary and commands:
+# profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
+TRAFFIC_BYTES_PER_LAUNCH = 4.484e9
 
 
 def parse():
@@ -38,7 +44,9 @@ def parse():
     ap.add_argument("--n-var", type=int, default=100_000)
     ap.add_argument("--n-hap", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-pairs", type=int, default=200_000)
+    ap.add_argument("--cpu-pairs", type=int, default=1_500_000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 device copies of 1 GiB (the HBM-counter reference of the traffic figure)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -56,6 +64,15 @@ def main():
               "SNPs plus a max_dist halo); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        n = 1 << 30
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {n} bytes 4 times")
+        return
     from pg_gpu_amd import band, synthetic, ld_pipeline as LP
     from pg_gpu_amd import _lib as L
 
@@ -96,19 +113,22 @@ def main():
     value = pair_samples / (ms_step * 1e-3) / 1e9
     log(f"[bench] fused step {ms_step:.3f} ms  ({wall:.3f} s wall for {args.steps})")
 
-    # dominant kernel: the band pass alone (k_band_sums + k_band_reduce), HIP events
+    # dominant kernel: k_band_sums alone, timed by the library with HIP events
+    # recorded on the stream it is launched on, averaged over launches
     reps = max(3, args.steps)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    plan.sums(packed, with_missing=with_missing)
-    e0.record()
+    plan.set_timing(True)
+    kms = []
     for _ in range(reps):
         plan.sums(packed, with_missing=with_missing)
-    e1.record()
-    torch.cuda.synchronize()
-    band_ms = e0.elapsed_time(e1) / reps
+        kms.append(plan.kernel_ms())
+    plan.set_timing(False)
+    band_ms = float(np.mean(kms))
     ops = 2.0 * MACS_PER_PAIR_SAMPLE * pair_samples
     achieved = ops / (band_ms * 1e-3)
-    log(f"[bench] band pass {band_ms:.3f} ms -> {achieved / 1e12:.1f} TOP/s int8")
+    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {min(kms):.3f}) -> {achieved / 1e12:.1f} int8 TOP/s")
+    traffic = None
+    if (args.n_var, args.n_hap) == (100_000, 2000):
+        traffic = TRAFFIC_BYTES_PER_LAUNCH
 
     # plain per-pair path on identical inputs and outputs (speedup vs plain)
     plain_ms = None
@@ -145,15 +165,19 @@ def main():
         ii, jj = O.pairs_within_distance(pos, max_dist)
         sel = np.sort(rng.choice(len(ii), size=min(args.cpu_pairs, len(ii)), replace=False))
         ii, jj = ii[sel], jj[sel]
+        chunk = 250_000
         t2 = time.perf_counter()
-        cl = [O.hap_counts(hap, ii, jj, r) for r in pops]
-        st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
-        O.keyed_sums(O.window_key(pos[ii], 0.0, window), st, n_keys)
+        acc = np.zeros((n_keys, 15))
+        for c0 in range(0, len(ii), chunk):
+            a, b = ii[c0:c0 + chunk], jj[c0:c0 + chunk]
+            cl = [O.hap_counts(hap, a, b, r) for r in pops]
+            st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+            acc += O.keyed_sums(O.window_key(pos[a], 0.0, window), st, n_keys)
         cpu_s = time.perf_counter() - t2
         cpu = {"value": len(ii) * sum(S) / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
                "sample": f"{len(ii)} random pairs of the same band (of {plan.n_pairs}), both populations, "
-                         f"counts + 15 statistics + window sums, NumPy oracle, {cpu_s:.1f} s"}
-        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s")
+                         f"counts + 15 statistics + window sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
 
     line = {
         "metric": "pair-sample counts/sec (Gpair*samples/s)",
@@ -172,9 +196,10 @@ def main():
                                "2% MCAR missing, gamma(0.5) heterogeneous positions over 10 Mb, pairs within "
                                "100 kb, 15 two-pop stats summed per 100 kb window (FP64)",
                    "n_var": args.n_var, "n_hap": args.n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_INT8_OPS, "traffic": None, "kernel": "k_band_sums",
-                     "kernel_ms": band_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": traffic,
+                     "kernel": "k_band_sums", "kernel_ms": band_ms,
+                     "ops": "int8 MFMA ops (1 MAC = 2 ops) of 4 MACs per pair*sample"},
         "plain_ms_per_step": plain_ms,
         "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
         "fused_vs_plain_max_rel": check,

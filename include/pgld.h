@@ -159,6 +159,13 @@ int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int
                           double width, pgld_band_plan** plan);
 int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n_pairs, int64_t* n_slots);
 int pgld_band_plan_destroy(pgld_band_plan* plan);
+/* Kernel timing for measurement (no counterpart in the reference, whose
+ * kernels are timed from the host): with timing enabled, pgld_band_sums
+ * records a pair of HIP events on its stream around the fused kernel, and
+ * pgld_band_plan_kernel_ms waits for the last pair and returns the elapsed
+ * milliseconds of that launch. */
+int pgld_band_plan_set_timing(pgld_band_plan* plan, int32_t enable);
+int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
 
 #define PGLD_SET_MOMENTS 0 /* ld_names(n_pops): 3 / 15 / 45 statistics for 1 / 2 / 3 pops */
 #define PGLD_SET_SELF 1    /* DD_i_i, Dz_i_i_i, pi2_i_i_i_i per population (3 * n_pops) */

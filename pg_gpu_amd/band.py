@@ -115,6 +115,16 @@ class BandPlan:
         L.call("pgld_band_plan_info", h, C.byref(nt), C.byref(npairs), C.byref(ns))
         self.n_tiles, self.n_pairs, self.n_slots = nt.value, npairs.value, ns.value
 
+    def set_timing(self, enable=True):
+        """Record HIP events around each fused-kernel launch of sums()."""
+        L.call("pgld_band_plan_set_timing", self._h, int(bool(enable)))
+
+    def kernel_ms(self):
+        """Duration of the last timed fused-kernel launch (waits for it)."""
+        ms = C.c_float()
+        L.call("pgld_band_plan_kernel_ms", self._h, C.byref(ms))
+        return float(ms.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             L.lib().pgld_band_plan_destroy(self._h)

@@ -43,6 +43,8 @@ struct pgld_band_plan {
   int64_t* d_key_list = nullptr;   // slots grouped by key, ascending tile order
   double* d_partials = nullptr;    // workspace
   size_t partials_cap = 0;         // doubles
+  bool timing = false;             // record events around each k_band_sums launch
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 namespace pgld {
@@ -822,9 +824,8 @@ int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int
   }
 
   auto up = [&](void** dst, const void* src, size_t bytes) -> int {
-    if (bytes == 0) bytes = 8;
-    hipError_t err = hipMalloc(dst, bytes);
-    if (err == hipSuccess && src) err = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    hipError_t err = hipMalloc(dst, bytes ? bytes : 8);
+    if (err == hipSuccess && src && bytes) err = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
     return check_hip(err, "pgld_band_plan_create upload");
   };
   int rc = PGLD_OK;
@@ -859,7 +860,27 @@ int pgld_band_plan_destroy(pgld_band_plan* plan) {
                   plan->d_key_off, plan->d_key_list, plan->d_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : plan->ev)
+    if (e) (void)hipEventDestroy(e);
   delete plan;
+  return PGLD_OK;
+}
+
+int pgld_band_plan_set_timing(pgld_band_plan* plan, int32_t enable) {
+  PGLD_REQUIRE(plan, "pgld_band_plan_set_timing: null plan");
+  if (enable && !plan->ev[0]) {
+    PGLD_TRY_HIP(hipEventCreate(&plan->ev[0]), "pgld_band_plan_set_timing event");
+    PGLD_TRY_HIP(hipEventCreate(&plan->ev[1]), "pgld_band_plan_set_timing event");
+  }
+  plan->timing = enable != 0;
+  return PGLD_OK;
+}
+
+int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms) {
+  PGLD_REQUIRE(plan && ms, "pgld_band_plan_kernel_ms: null pointer");
+  PGLD_REQUIRE(plan->ev[0], "pgld_band_plan_kernel_ms: timing was never enabled on this plan");
+  PGLD_TRY_HIP(hipEventSynchronize(plan->ev[1]), "pgld_band_plan_kernel_ms sync");
+  PGLD_TRY_HIP(hipEventElapsedTime(ms, plan->ev[0], plan->ev[1]), "pgld_band_plan_kernel_ms elapsed");
   return PGLD_OK;
 }
 
@@ -926,9 +947,9 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
     PGLD_TRY_HIP(hipMalloc((void**)&pl->d_partials, need * sizeof(double)), "pgld_band_sums workspace");
     pl->partials_cap = need;
   }
-  PGLD_TRY_HIP(hipMemsetAsync(sums, 0, (size_t)plan->n_keys * n_stats * sizeof(double), s), "pgld_band_sums memset");
-  if (key_pairs) PGLD_TRY_HIP(hipMemsetAsync(key_pairs, 0, plan->n_keys * sizeof(double), s), "memset");
-  if (plan->n_tiles == 0) return PGLD_OK;
+  // k_band_reduce writes every key (zero where no tile touches it), so the
+  // outputs need no clearing and the call records no memset node when
+  // captured into a graph.
   BandArgs ba{};
   ba.tiles = reinterpret_cast<const BandTile*>(plan->d_tiles);
   ba.pos = plan->d_pos;
@@ -941,9 +962,13 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
   ba.key_mode = plan->key_mode;
   ba.n_keys = plan->n_keys;
   ba.partials = pl->d_partials;
-  L(plan, ba, pa, red_bytes + stash_bytes + op_bytes, s);
-  rc = check_launch("k_band_sums");
-  if (rc) return rc;
+  if (plan->n_tiles > 0) {
+    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "pgld_band_sums event");
+    L(plan, ba, pa, red_bytes + stash_bytes + op_bytes, s);
+    rc = check_launch("k_band_sums");
+    if (rc) return rc;
+    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[1], s), "pgld_band_sums event");
+  }
   hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)plan->n_keys), dim3(256), 0, s, pl->d_partials, plan->d_key_off,
                      plan->d_key_list, ns_total, n_stats, sums, key_pairs);
   return check_launch("k_band_reduce");
