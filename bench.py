@@ -1,16 +1,20 @@
-"""Benchmark: batched windowed haplotype LD on one MI355X.
+"""Benchmark of the LD pair-statistics path on one MI355X.
 
-Workload: 100,000 SNPs x 2,000 haplotypes (two populations of 1,000), 2% MCAR
-missing, heterogeneous positions over 10 Mb; all pairs within 100 kb; the 15
-two-population moments statistics summed in FP64 per 100 kb window (keyed by
-the left SNP).  One step = pack both populations' indicator planes from the
-resident int8 matrix + one fused tiled int8-MFMA band pass + the per-window
-reduction.
+Default workload (the headline): batched windowed haplotype LD -- 100,000 SNPs
+x 2,000 haplotypes (two populations of 1,000), 2% MCAR missing, heterogeneous
+positions over 10 Mb; all pairs within 100 kb; the 15 two-population moments
+statistics summed in FP64 per 100 kb window (keyed by the left SNP).  One step
+= pack both populations' indicator planes from the resident int8 matrix + one
+fused tiled int8-MFMA band pass + the per-window reduction.
 
-Prints one JSON line (see README of the harness contract): value is
-Gpair*samples/s for the whole job; roofline is for the dominant kernel
-(k_band_sums, int8 MFMA bound); cpu_baseline is the NumPy oracle on a bounded
-random sample of the same pairs.
+Other workloads (--workload): genotype_bins (50,000 SNPs x 1,000 diploids, 5%
+missing, all pairs in 1 Mb, 16 log distance bins), plain_hap (2,000 SNPs x
+200 haplotypes, per-pair r^2/DD/Dz/pi2 within 20 kb through the plain
+per-pair kernels).
+
+Prints one JSON line: value is Gpair*samples/s for the whole job; roofline is
+for the dominant kernel; cpu_baseline is the NumPy oracle on a bounded sample
+of the same pairs.
 """
 from __future__ import annotations
 
@@ -27,7 +31,6 @@ sys.path.insert(0, ROOT)
 
 PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
-MACS_PER_PAIR_SAMPLE = 4  # haplotypes with missing data: hh, hv, vh, vv
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
 # FETCH_SIZE (2,187,585 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
 # by the same counters' reading of a 1 GiB device copy (--copy-calibration:
@@ -41,19 +44,334 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n-var", type=int, default=100_000)
-    ap.add_argument("--n-hap", type=int, default=2000)
+    ap.add_argument("--workload", default="windowed_hap", choices=["windowed_hap", "genotype_bins", "plain_hap"])
+    ap.add_argument("--n-var", type=int, default=None)
+    ap.add_argument("--n-samples", "--n-hap", dest="n_samples", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-pairs", type=int, default=1_500_000)
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run 4 device copies of 1 GiB (the HBM-counter reference of the traffic figure)")
+    ap.add_argument("--cpu-pairs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 device copies of 1 GiB (the HBM-counter reference of the traffic figure)")
     return ap.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def timed_steps(torch, step, steps, warmup):
+    """Average ms of `steps` calls after `warmup`, HIP events on the current
+    stream, synchronized on both sides."""
+    out = None
+    for _ in range(warmup):
+        out = step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        out = step()
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps, out
+
+
+def band_kernel_ms(plan, packed, with_missing, reps):
+    """k_band_sums alone: HIP events recorded by the library around each launch
+    on the stream it is launched on, averaged over launches."""
+    plan.set_timing(True)
+    kms = []
+    for _ in range(reps):
+        plan.sums(packed, with_missing=with_missing)
+        kms.append(plan.kernel_ms())
+    plan.set_timing(False)
+    return float(np.mean(kms)), float(np.min(kms))
+
+
+def band_pair_sample(pos, max_dist, want, seed=7):
+    """`want` pairs drawn uniformly (with replacement; every pair once when
+    want covers the band) from {(i, j): i < j, pos[j] - pos[i] <= max_dist},
+    without listing the band: the
+    left SNP with probability proportional to its pair count, then the right
+    SNP uniformly among its partners.  Sorted by (i, j)."""
+    rng = np.random.default_rng(seed)
+    pos = np.asarray(pos)
+    per = np.searchsorted(pos, pos + max_dist, side="right") - np.arange(len(pos)) - 1
+    per = np.maximum(per, 0)
+    cum = np.cumsum(per)
+    total = int(cum[-1])
+    r = np.arange(total) if want >= total else rng.integers(0, total, size=want)
+    ii = np.searchsorted(cum, r, side="right")
+    jj = ii + 1 + (r - (cum[ii] - per[ii]))
+    order = np.lexsort((jj, ii))
+    return ii[order], jj[order]
+
+
+def base_line(args, value, ms_step, workload, config, dtype="int8"):
+    return {
+        "metric": "pair-sample counts/sec (Gpair*samples/s)",
+        "value": value,
+        "unit": "Gpair*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": dict({"workload": workload}, **config),
+    }
+
+
+def mfma_roofline(ops, kernel_ms, kernel, macs, traffic=None):
+    achieved = ops / (kernel_ms * 1e-3)
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_INT8_OPS, "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_ms,
+            "ops": f"int8 MFMA ops (1 MAC = 2 ops) of {macs} MACs per pair*sample"}
+
+
+def _plain_compare(torch, plain_step, fused_out, steps):
+    ref = plain_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        ref = plain_step()
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t1) * 1e3 / steps
+    a, b = fused_out.cpu().numpy(), ref.cpu().numpy()
+    check = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+    log(f"[bench] plain path {plain_ms:.1f} ms, max rel diff fused vs plain {check:.2e}")
+    return plain_ms, check
+
+
+# ---------------------------------------------------------------------------
+def run_windowed_hap(args, torch):
+    from pg_gpu_amd import band, synthetic, ld_pipeline as LP
+    from pg_gpu_amd import _lib as L
+
+    n_var = args.n_var or 100_000
+    n_hap = args.n_samples or 2000
+    macs = 4  # haplotypes with missing data: hh, hv, vh, vv
+    log("[bench] generating workload")
+    hap, pos, pops = synthetic.windowed_haplotype_panel(n_hap=n_hap, n_var=n_var)
+    max_dist, window = 100_000.0, 100_000.0
+    n_keys = 100
+    dev = L.device()
+    hap_d = torch.as_tensor(hap, device=dev)
+    rows_d = [torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in pops]
+    plan = band.BandPlan(pos, max_dist, window=window, n_keys=n_keys, origin=0.0)
+    S = [len(r) for r in pops]
+    pair_samples = plan.n_pairs * sum(S)
+    log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
+
+    # the input is validated once (domain, missing data); a timed step then
+    # repacks both populations into the same buffers and runs the fused pass
+    packed = [band.pack_population(hap_d, r, "hap") for r in rows_d]
+    with_missing = any(p.has_missing for p in packed)
+
+    def step():
+        for k, r in enumerate(rows_d):
+            band.pack_population(hap_d, r, "hap", validate=False, out=packed[k])
+        return plan.sums(packed, with_missing=with_missing)
+
+    ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    log(f"[bench] fused step {ms_step:.3f} ms")
+    band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
+    roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs,
+                         TRAFFIC_BYTES_PER_LAUNCH if (n_var, n_hap) == (100_000, 2000) else None)
+    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
+
+    # plain per-pair path on identical inputs and outputs (speedup vs plain)
+    plain_ms = check = None
+    if not args.no_plain:
+        pos_d = torch.as_tensor(pos.astype(np.float64), device=dev)
+
+        def plain_step():
+            sums = torch.zeros((n_keys, 15), dtype=torch.float64, device=dev)
+            chunk = LP.estimate_ld_chunk_size(max(S), num_pops=2)
+            for ii, jj in LP.iter_pairs_within_distance(pos, max_dist, chunk):
+                cl = [LP.compute_counts_for_pairs(hap_d, ii, jj, r) for r in rows_d]
+                st = LP.compute_two_pop_statistics_batch(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+                keys = torch.empty(ii.shape[0], dtype=torch.int32, device=dev)
+                L.call("pgld_pair_keys", L.ptr(pos_d), L.ptr(ii), L.ptr(jj), ii.shape[0], L.KEY_WINDOW, None,
+                       n_keys, 0.0, window, L.ptr(keys), L.stream_handle())
+                L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], 15, n_keys, L.ptr(sums),
+                       L.stream_handle())
+            return sums
+        plain_ms, check = _plain_compare(torch, plain_step, out, args.plain_steps)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ld_oracle as O
+        ii, jj = band_pair_sample(pos, max_dist, args.cpu_pairs or 1_500_000)
+        t2 = time.perf_counter()
+        acc = np.zeros((n_keys, 15))
+        for c0 in range(0, len(ii), 250_000):
+            a, b = ii[c0:c0 + 250_000], jj[c0:c0 + 250_000]
+            cl = [O.hap_counts(hap, a, b, r) for r in pops]
+            st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+            acc += O.keyed_sums(O.window_key(pos[a], 0.0, window), st, n_keys)
+        cpu_s = time.perf_counter() - t2
+        cpu = {"value": len(ii) * sum(S) / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
+               "sample": f"{len(ii)} random pairs of the same band (of {plan.n_pairs}), both populations, "
+                         f"counts + 15 statistics + window sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
+
+    line = base_line(args, value, ms_step,
+                     f"batched windowed haplotype LD: {n_var:,} SNPs x {n_hap:,} haplotypes (2 pops x "
+                     f"{n_hap // 2:,}), 2% MCAR missing, gamma(0.5) heterogeneous positions over 10 Mb, pairs "
+                     "within 100 kb, 15 two-pop stats summed per 100 kb window (FP64)",
+                     {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples})
+    line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
+                 "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
+                 "fused_vs_plain_max_rel": check, "cpu_baseline": cpu})
+    return line
+
+
+# ---------------------------------------------------------------------------
+def run_genotype_bins(args, torch):
+    from pg_gpu_amd import band, synthetic, ld_pipeline as LP
+    from pg_gpu_amd import _lib as L
+
+    n_var = args.n_var or 50_000
+    n_ind = args.n_samples or 1000
+    macs = 9  # genotypes with missing data: {v, e1, e2} x {v, e1, e2}
+    log("[bench] generating workload")
+    g, pos = synthetic.genotype_panel(n_ind=n_ind, n_var=n_var)
+    edges = np.logspace(0, 6, 17)
+    max_dist = 1e6
+    dev = L.device()
+    g_d = torch.as_tensor(g, device=dev)
+    plan = band.BandPlan(pos, max_dist, edges=edges)
+    pair_samples = plan.n_pairs * n_ind
+    log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
+    packed = [band.pack_population(g_d, None, "geno")]
+    with_missing = bool(packed[0].has_missing)
+
+    def step():
+        band.pack_population(g_d, None, "geno", validate=False, out=packed[0])
+        return plan.sums(packed, with_missing=with_missing)
+
+    ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    log(f"[bench] fused step {ms_step:.3f} ms")
+    band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
+    roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs)
+    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
+
+    plain_ms = check = None
+    if not args.no_plain:
+        pos_d = torch.as_tensor(pos.astype(np.float64), device=dev)
+        edges_d = torch.as_tensor(edges, device=dev)
+        specs = LP.generate_stat_specs(1)
+
+        def plain_step():
+            sums = torch.zeros((16, 3), dtype=torch.float64, device=dev)
+            chunk = LP.estimate_ld_chunk_size(n_ind, num_pops=1)
+            for ii, jj in LP.iter_pairs_within_distance(pos, max_dist, chunk):
+                c, nv = LP.compute_genotype_counts_for_pairs(g_d, ii, jj)
+                st = LP.stats_from_counts("geno", [c], [nv], specs)
+                keys = torch.empty(ii.shape[0], dtype=torch.int32, device=dev)
+                L.call("pgld_pair_keys", L.ptr(pos_d), L.ptr(ii), L.ptr(jj), ii.shape[0], L.KEY_BIN,
+                       L.ptr(edges_d), 16, 0.0, 1.0, L.ptr(keys), L.stream_handle())
+                L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], 3, 16, L.ptr(sums),
+                       L.stream_handle())
+            return sums
+        plain_ms, check = _plain_compare(torch, plain_step, out, args.plain_steps)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ld_oracle as O
+        ii, jj = band_pair_sample(pos, max_dist, args.cpu_pairs or 1_000_000)
+        t2 = time.perf_counter()
+        acc = np.zeros((16, 3))
+        for c0 in range(0, len(ii), 200_000):
+            a, b = ii[c0:c0 + 200_000], jj[c0:c0 + 200_000]
+            c, nv = O.geno_counts(g, a, b, None)
+            st = O.multi_pop_stats_geno([c], [nv])
+            acc += O.keyed_sums(O.bin_index(pos[b] - pos[a], edges), st, 16)
+        cpu_s = time.perf_counter() - t2
+        cpu = {"value": len(ii) * n_ind / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
+               "sample": f"{len(ii)} random pairs of the {plan.n_pairs} in the span, 9-way counts + DD/Dz/pi2 + "
+                         f"bin sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
+
+    line = base_line(args, value, ms_step,
+                     f"genotype LD: {n_var:,} SNPs x {n_ind:,} diploids, 5% missing, HWE Beta(0.5,0.5) dosages, "
+                     "all pairs in a 1 Mb span, 16 log-spaced distance bins (1 bp - 1 Mb), single-population "
+                     "DD/Dz/pi2 FP64 bin sums (no biallelic filter)",
+                     {"n_var": n_var, "n_ind": n_ind, "pairs": plan.n_pairs, "pair_samples": pair_samples})
+    line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
+                 "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
+                 "fused_vs_plain_max_rel": check, "cpu_baseline": cpu})
+    return line
+
+
+# ---------------------------------------------------------------------------
+def run_plain_hap(args, torch):
+    """Per-pair outputs through the plain drop-in path: pair enumeration,
+    plain counts, DD/Dz/pi2 and r^2 per pair."""
+    from pg_gpu_amd import synthetic, ld_pipeline as LP, ld_statistics as LS
+    from pg_gpu_amd import _lib as L
+
+    n_var = args.n_var or 2000
+    n_hap = args.n_samples or 200
+    hap, pos = synthetic.plain_haplotype_panel(n_hap=n_hap, n_var=n_var)
+    max_dist = 20_000
+    dev = L.device()
+    hap_d = torch.as_tensor(hap, device=dev)
+    ii, jj = next(iter(LP.iter_pairs_within_distance(pos, max_dist, 10**7)))
+    n_pairs = int(ii.shape[0])
+    pair_samples = n_pairs * n_hap
+
+    def step():
+        c, nv = LP.compute_counts_for_pairs(hap_d, ii, jj)
+        st = LP.compute_single_pop_statistics_batch(c, nv)
+        r2 = LS.r_squared(c, nv)
+        return st, r2
+
+    ms_step, _ = timed_steps(torch, step, args.steps, args.warmup)
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps)
+    e0.record()
+    for _ in range(reps):
+        LP.compute_counts_for_pairs(hap_d, ii, jj)
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    alg_bytes = 2.0 * pair_samples  # two sample columns per pair, no reuse
+    roof = {"bound": "hbm", "achieved": alg_bytes / (k_ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": alg_bytes / (k_ms * 1e-3) / PEAK_HBM, "traffic": None, "kernel": "k_counts_plain",
+            "kernel_ms": k_ms, "bytes": "2 B per pair*sample (both SNP columns, no reuse)"}
+    log(f"[bench] plain step {ms_step:.3f} ms, counts kernel {k_ms:.3f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ld_oracle as O
+        ii_h, jj_h = L.to_numpy(ii).astype(np.int64), L.to_numpy(jj).astype(np.int64)
+        t2 = time.perf_counter()
+        reps_cpu = 0
+        while True:
+            c, nv = O.hap_counts(hap, ii_h, jj_h, None)
+            O.multi_pop_stats_hap([c], [nv])
+            O.r_squared(c, nv)
+            reps_cpu += 1
+            if time.perf_counter() - t2 > 10.0:
+                break
+        cpu_s = (time.perf_counter() - t2) / reps_cpu
+        cpu = {"value": pair_samples / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
+               "sample": f"all {n_pairs} pairs, counts + DD/Dz/pi2 + r^2, NumPy oracle (single thread), "
+                         f"{reps_cpu} passes, {cpu_s:.2f} s each"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s")
+    line = base_line(args, value, ms_step,
+                     f"plain per-pair haplotype LD: {n_var:,} SNPs x {n_hap} haplotypes, neutral SFS, no missing "
+                     "data, positions uniform over 200 kb; r^2, DD, Dz, pi2 per pair within 20 kb, one population",
+                     {"n_var": n_var, "n_hap": n_hap, "pairs": n_pairs, "pair_samples": pair_samples})
+    line.update({"roofline": roof, "cpu_baseline": cpu})
+    return line
 
 
 def main():
@@ -73,138 +391,8 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {n} bytes 4 times")
         return
-    from pg_gpu_amd import band, synthetic, ld_pipeline as LP
-    from pg_gpu_amd import _lib as L
-
-    log("[bench] generating workload")
-    hap, pos, pops = synthetic.config_c2(n_hap=args.n_hap, n_var=args.n_var)
-    max_dist, window = 100_000.0, 100_000.0
-    n_keys = 100
-    dev = L.device()
-    hap_d = torch.as_tensor(hap, device=dev)
-    rows_d = [torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in pops]
-    plan = band.BandPlan(pos, max_dist, window=window, n_keys=n_keys, origin=0.0)
-    S = [len(r) for r in pops]
-    pair_samples = plan.n_pairs * sum(S)
-    log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
-
-    # the input is validated once (domain, missing data); a timed step then
-    # repacks both populations into the same buffers and runs the fused pass
-    packed = [band.pack_population(hap_d, r, "hap") for r in rows_d]
-    with_missing = any(p.has_missing for p in packed)
-
-    def step():
-        for k, r in enumerate(rows_d):
-            band.pack_population(hap_d, r, "hap", validate=False, out=packed[k])
-        return plan.sums(packed, with_missing=with_missing), packed
-
-    for _ in range(args.warmup):
-        out, packed = step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(args.steps):
-        out, packed = step()
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / args.steps
-    value = pair_samples / (ms_step * 1e-3) / 1e9
-    log(f"[bench] fused step {ms_step:.3f} ms  ({wall:.3f} s wall for {args.steps})")
-
-    # dominant kernel: k_band_sums alone, timed by the library with HIP events
-    # recorded on the stream it is launched on, averaged over launches
-    reps = max(3, args.steps)
-    plan.set_timing(True)
-    kms = []
-    for _ in range(reps):
-        plan.sums(packed, with_missing=with_missing)
-        kms.append(plan.kernel_ms())
-    plan.set_timing(False)
-    band_ms = float(np.mean(kms))
-    ops = 2.0 * MACS_PER_PAIR_SAMPLE * pair_samples
-    achieved = ops / (band_ms * 1e-3)
-    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {min(kms):.3f}) -> {achieved / 1e12:.1f} int8 TOP/s")
-    traffic = None
-    if (args.n_var, args.n_hap) == (100_000, 2000):
-        traffic = TRAFFIC_BYTES_PER_LAUNCH
-
-    # plain per-pair path on identical inputs and outputs (speedup vs plain)
-    plain_ms = None
-    check = None
-    if not args.no_plain:
-        def plain_step():
-            sums = torch.zeros((n_keys, 15), dtype=torch.float64, device=dev)
-            pos_d = torch.as_tensor(pos.astype(np.float64), device=dev)
-            chunk = LP.estimate_ld_chunk_size(max(S), num_pops=2)
-            for ii, jj in LP.iter_pairs_within_distance(pos, max_dist, chunk):
-                cl = [LP.compute_counts_for_pairs(hap_d, ii, jj, r) for r in rows_d]
-                st = LP.compute_two_pop_statistics_batch(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
-                keys = torch.empty(ii.shape[0], dtype=torch.int32, device=dev)
-                L.call("pgld_pair_keys", L.ptr(pos_d), L.ptr(ii), L.ptr(jj), ii.shape[0], L.KEY_WINDOW, None,
-                       n_keys, 0.0, window, L.ptr(keys), L.stream_handle())
-                L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], 15, n_keys, L.ptr(sums),
-                       L.stream_handle())
-            return sums
-        ref = plain_step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.plain_steps):
-            ref = plain_step()
-        torch.cuda.synchronize()
-        plain_ms = (time.perf_counter() - t1) * 1e3 / args.plain_steps
-        a, b = out.cpu().numpy(), ref.cpu().numpy()
-        check = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
-        log(f"[bench] plain path {plain_ms:.1f} ms, max rel diff fused vs plain {check:.2e}")
-
-    cpu = None
-    if not args.no_cpu:
-        from oracle import ld_oracle as O
-        rng = np.random.default_rng(7)
-        ii, jj = O.pairs_within_distance(pos, max_dist)
-        sel = np.sort(rng.choice(len(ii), size=min(args.cpu_pairs, len(ii)), replace=False))
-        ii, jj = ii[sel], jj[sel]
-        chunk = 250_000
-        t2 = time.perf_counter()
-        acc = np.zeros((n_keys, 15))
-        for c0 in range(0, len(ii), chunk):
-            a, b = ii[c0:c0 + chunk], jj[c0:c0 + chunk]
-            cl = [O.hap_counts(hap, a, b, r) for r in pops]
-            st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
-            acc += O.keyed_sums(O.window_key(pos[a], 0.0, window), st, n_keys)
-        cpu_s = time.perf_counter() - t2
-        cpu = {"value": len(ii) * sum(S) / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
-               "sample": f"{len(ii)} random pairs of the same band (of {plan.n_pairs}), both populations, "
-                         f"counts + 15 statistics + window sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
-        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
-
-    line = {
-        "metric": "pair-sample counts/sec (Gpair*samples/s)",
-        "value": value,
-        "unit": "Gpair*samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int8",
-        "data": "synthetic",
-        "config": {"workload": "batched windowed haplotype LD: 100,000 SNPs x 2,000 haplotypes (2 pops x 1,000), "
-                               "2% MCAR missing, gamma(0.5) heterogeneous positions over 10 Mb, pairs within "
-                               "100 kb, 15 two-pop stats summed per 100 kb window (FP64)",
-                   "n_var": args.n_var, "n_hap": args.n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": traffic,
-                     "kernel": "k_band_sums", "kernel_ms": band_ms,
-                     "ops": "int8 MFMA ops (1 MAC = 2 ops) of 4 MACs per pair*sample"},
-        "plain_ms_per_step": plain_ms,
-        "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
-        "fused_vs_plain_max_rel": check,
-        "cpu_baseline": cpu,
-    }
+    run = {"windowed_hap": run_windowed_hap, "genotype_bins": run_genotype_bins, "plain_hap": run_plain_hap}
+    line = run[args.workload](args, torch)
     print(json.dumps(line), flush=True)
 
 

@@ -81,7 +81,7 @@ def hwe_genotypes(n_ind, n_var, rng, beta=0.5, missing=0.0):
     return g
 
 
-def config_c1(seed=1001, n_hap=200, n_var=2000, span=200_000):
+def plain_haplotype_panel(seed=1001, n_hap=200, n_var=2000, span=200_000):
     """Plain haplotype case: 2,000 SNPs x 200 haplotypes, neutral SFS, no
     missing data, positions uniform over 200 kb; pairs within 20 kb."""
     rng = np.random.default_rng(seed)
@@ -90,7 +90,7 @@ def config_c1(seed=1001, n_hap=200, n_var=2000, span=200_000):
     return hap, pos
 
 
-def config_c2(seed=1002, n_hap=2000, n_var=100_000, span=10_000_000, missing=0.02, segment=10_000):
+def windowed_haplotype_panel(seed=1002, n_hap=2000, n_var=100_000, span=10_000_000, missing=0.02, segment=10_000):
     """Batched windowed haplotype LD: 100,000 SNPs x 2,000 haplotypes (two
     populations of 1,000), 2% MCAR missing, heterogeneous positions over
     10 Mb; pairs within 100 kb, 100 kb windows keyed by the left SNP."""
@@ -102,7 +102,7 @@ def config_c2(seed=1002, n_hap=2000, n_var=100_000, span=10_000_000, missing=0.0
     return hap, pos, pops
 
 
-def config_c3(seed=1003, n_ind=1000, n_var=50_000, span=1_000_000, missing=0.05):
+def genotype_panel(seed=1003, n_ind=1000, n_var=50_000, span=1_000_000, missing=0.05):
     """Genotype LD: 50,000 SNPs x 1,000 diploids, 5% missing, HWE dosages
     from Beta(0.5, 0.5) frequencies; all pairs in a 1 Mb span, 16 log bins."""
     rng = np.random.default_rng(seed)

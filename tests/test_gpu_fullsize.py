@@ -1,0 +1,178 @@
+"""GPU checks at the benchmark workloads' full sizes.
+
+The oracle cannot run these sizes in seconds, so the full-size checks are
+properties that do not depend on size: pair counts per key equal the closed
+form from the positions (integers, bit-exact), repeated runs are bit-identical,
+the fused band pass agrees with the plain per-pair path (two different GPU
+algorithms) on a contiguous sub-panel, and the per-pair plain workload
+(2,000 SNPs x 200 haplotypes, all pairs within 20 kb) is checked against the
+oracle in full.
+"""
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from pg_gpu_amd import band, ld_pipeline as LP, ld_statistics as LS  # noqa: E402
+from pg_gpu_amd import synthetic as SY  # noqa: E402
+from pg_gpu_amd import _lib as L  # noqa: E402
+
+
+def np_(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
+def pairs_per_key_window(pos, max_dist, width, n_keys):
+    per = np.searchsorted(pos, pos + max_dist, side="right") - np.arange(len(pos)) - 1
+    key = np.floor(pos / width).astype(np.int64)
+    ok = key < n_keys
+    return np.bincount(key[ok], weights=per[ok], minlength=n_keys)
+
+
+def pairs_per_key_bins(pos, edges):
+    """Pairs with distance in [e_k, e_{k+1}) for every bin, by searchsorted."""
+    out = np.zeros(len(edges) - 1)
+    for k in range(len(edges) - 1):
+        lo = np.searchsorted(pos, pos + edges[k], side="left")
+        hi = np.searchsorted(pos, pos + edges[k + 1], side="left")
+        lo = np.maximum(lo, np.arange(len(pos)) + 1)
+        out[k] = np.maximum(hi - lo, 0).sum()
+    return out
+
+
+def plain_sums(data, family, pos, rows, max_dist, n_keys, mode, edges=None, width=1.0, two_pop=False):
+    dev = L.device()
+    sums = torch.zeros((n_keys, 15 if two_pop else 3), dtype=torch.float64, device=dev)
+    pos_d = torch.as_tensor(np.asarray(pos, dtype=np.float64), device=dev)
+    e_d = None if edges is None else torch.as_tensor(edges, device=dev)
+    specs = LP.generate_stat_specs(1)
+    for ii, jj in LP.iter_pairs_within_distance(pos, max_dist, 5_000_000):
+        if family == "geno":
+            c, nv = LP.compute_genotype_counts_for_pairs(data, ii, jj, rows[0])
+            st = LP.stats_from_counts("geno", [c], [nv], specs)
+        else:
+            cl = [LP.compute_counts_for_pairs(data, ii, jj, r) for r in rows]
+            st = LP.compute_two_pop_statistics_batch(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+        keys = torch.empty(ii.shape[0], dtype=torch.int32, device=dev)
+        L.call("pgld_pair_keys", L.ptr(pos_d), L.ptr(ii), L.ptr(jj), ii.shape[0], mode, L.ptr(e_d), n_keys, 0.0,
+               width, L.ptr(keys), L.stream_handle())
+        L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], st.shape[1], n_keys, L.ptr(sums),
+               L.stream_handle())
+    return np_(sums)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def windowed_panel():
+    print("[fullsize] windowed haplotype panel 100,000 SNPs x 2,000 haplotypes", flush=True)
+    hap, pos, pops = SY.windowed_haplotype_panel()
+    return hap, pos, pops
+
+
+def test_windowed_panel_full_size(windowed_panel):
+    hap, pos, pops = windowed_panel
+    hap_d = torch.as_tensor(hap, device=L.device())
+    plan = band.BandPlan(pos, 100_000.0, window=100_000.0, n_keys=100)
+    packed = [band.pack_population(hap_d, r, "hap") for r in pops]
+    assert all(p.has_missing for p in packed)
+    s1, kp = plan.sums(packed, want_pairs=True)
+    s2 = plan.sums(packed)
+    np.testing.assert_array_equal(np_(s1), np_(s2))  # fixed-order reduction
+    want = pairs_per_key_window(pos.astype(np.float64), 100_000.0, 100_000.0, 100)
+    np.testing.assert_array_equal(np_(kp), want)
+    assert int(np_(kp).sum()) == plan.n_pairs == 111_173_550
+    # the pair counts of the MFMA path equal the plain kernel's on every pair
+    # of the first 3,000 SNPs' band
+    sub = slice(0, 3000)
+    sp = band.BandPlan(pos[sub], 100_000.0, window=100_000.0, n_keys=100)
+    pk = band.pack_population(hap_d[:, sub].contiguous(), pops[0], "hap")
+    got = np_(sp.counts(pk))
+    ii, jj = next(iter(LP.iter_pairs_within_distance(pos[sub], 100_000.0, 10**9)))
+    c, _ = LP.compute_counts_for_pairs(hap_d[:, sub].contiguous(), ii, jj, pops[0])
+    np.testing.assert_array_equal(got, np_(c))
+
+
+def test_windowed_panel_fused_vs_plain_subpanel(windowed_panel):
+    hap, pos, pops = windowed_panel
+    sub = slice(0, 20_000)
+    h = torch.as_tensor(np.ascontiguousarray(hap[:, sub]), device=L.device())
+    p = pos[sub]
+    plan = band.BandPlan(p, 100_000.0, window=100_000.0, n_keys=100)
+    fused = np_(plan.sums([band.pack_population(h, r, "hap") for r in pops]))
+    ref = plain_sums(h, "hap", p, pops, 100_000.0, 100, L.KEY_WINDOW, width=100_000.0, two_pop=True)
+    nz = np.abs(ref) > 0
+    assert rel_err(fused[nz], ref[nz]) < 1e-9
+    np.testing.assert_array_equal(fused[~nz], 0.0)
+
+
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def genotype_panel():
+    print("[fullsize] genotype panel 50,000 SNPs x 1,000 diploids", flush=True)
+    return SY.genotype_panel()
+
+
+def test_genotype_panel_full_size(genotype_panel):
+    g, pos = genotype_panel
+    edges = np.logspace(0, 6, 17)
+    g_d = torch.as_tensor(g, device=L.device())
+    plan = band.BandPlan(pos, 1e6, edges=edges)
+    assert plan.n_pairs == 50_000 * 49_999 // 2
+    packed = [band.pack_population(g_d, None, "geno")]
+    s1, kp = plan.sums(packed, want_pairs=True)
+    s2 = plan.sums(packed)
+    np.testing.assert_array_equal(np_(s1), np_(s2))
+    np.testing.assert_array_equal(np_(kp), pairs_per_key_bins(pos.astype(np.float64), edges))
+    assert np.all(np.isfinite(np_(s1)))
+
+
+def test_genotype_panel_fused_vs_plain_subpanel(genotype_panel):
+    g, pos = genotype_panel
+    sub = slice(0, 6000)
+    edges = np.logspace(0, 6, 17)
+    gd = torch.as_tensor(np.ascontiguousarray(g[:, sub]), device=L.device())
+    p = pos[sub]
+    plan = band.BandPlan(p, 1e6, edges=edges)
+    fused = np_(plan.sums([band.pack_population(gd, None, "geno")]))
+    ref = plain_sums(gd, "geno", p, [None], 1e6, 16, L.KEY_BIN, edges=edges)
+    nz = np.abs(ref) > 0
+    assert rel_err(fused[nz], ref[nz]) < 1e-9
+    np.testing.assert_array_equal(fused[~nz], 0.0)
+
+
+# ---------------------------------------------------------------------------
+def test_plain_haplotype_panel_full_size_vs_oracle():
+    """2,000 SNPs x 200 haplotypes, all pairs within 20 kb: counts bit-exact,
+    DD/Dz/pi2 and r^2 per pair against the oracle, through the plain path and
+    through the MFMA band counts."""
+    hap, pos = SY.plain_haplotype_panel()
+    hap_d = torch.as_tensor(hap, device=L.device())
+    ii, jj = next(iter(LP.iter_pairs_within_distance(pos, 20_000, 10**9)))
+    ei, ej = O.pairs_within_distance(pos, 20_000)
+    np.testing.assert_array_equal(np_(ii), ei)
+    np.testing.assert_array_equal(np_(jj), ej)
+    c, nv = LP.compute_counts_for_pairs(hap_d, ii, jj)
+    want_c, want_nv = O.hap_counts(hap, ei, ej, None)
+    assert nv is None and want_nv is None
+    np.testing.assert_array_equal(np_(c), want_c)
+    st = np_(LP.compute_single_pop_statistics_batch(c, nv))
+    want = O.multi_pop_stats_hap([want_c], [None])
+    np.testing.assert_allclose(st, want, rtol=1e-12, atol=1e-15)
+    r2 = np_(LS.r_squared(c))
+    w2 = O.r_squared(want_c)
+    np.testing.assert_array_equal(np.isnan(r2), np.isnan(w2))
+    ok = ~np.isnan(w2)
+    np.testing.assert_allclose(r2[ok], w2[ok], rtol=1e-12, atol=1e-15)
+    plan = band.BandPlan(pos, 20_000.0, window=1e9, n_keys=1)
+    mc = np_(plan.counts(band.pack_population(hap_d, None, "hap")))
+    np.testing.assert_array_equal(mc, want_c)
