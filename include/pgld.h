@@ -196,6 +196,30 @@ int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* p
                      const int32_t* marg, int64_t k_pad, int32_t n_samples, int32_t with_missing,
                      const int64_t* pair_base, int32_t* counts, void* stream);
 
+/* r^2 of every pair of the band from one haplotype population's planes,
+ * counts through the int8-MFMA tiles: r^2 = diff^2 / (nA (n - nA) nB (n - nB))
+ * with diff = n n11 - nA nB computed exactly (int64) and one FP64 division; NaN
+ * where the denominator is 0 (ld_statistics.r_squared, ld_statistics.py:
+ * 194-217; pairs with missing data use their pair-complete counts).  Outputs,
+ * each optional (NULL to skip):
+ *   hist      uint64 (n_keys, n_hist + 1): per plan key, pairs with r^2 in
+ *             [b / n_hist, (b + 1) / n_hist) (r^2 = 1 in the last bin); the
+ *             last column counts NaN pairs (the histogram analogue of
+ *             windowed_r_squared, haplotype_matrix.py:1233-1317);
+ *   sparse_i / sparse_j / sparse_r2 (FP32), capacity sparse_cap, with
+ *             *n_sparse (device uint64) = number of pairs with r^2 > thresh
+ *             (may exceed the capacity: only the first sparse_cap are
+ *             written; order unspecified);
+ *   r2_out    FP64 per band pair in the plan's (i, j) order (the condensed
+ *             upper triangle for an all-pairs plan, cf. pairwise_r2,
+ *             haplotype_matrix.py:1092-1170).
+ * Replaces tally_gpu_haplotypes + r_squared + the host-side binning of
+ * windowed_r_squared for dense all-pairs r^2. */
+int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
+                 int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
+                 int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
+                 double* r2_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

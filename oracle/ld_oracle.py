@@ -161,6 +161,38 @@ def r_squared(counts, n_valid=None):
         return np.where(den > 0.0, D * D / np.where(den > 0, den, 1.0), np.nan)
 
 
+def r_squared_exact(counts):
+    """r^2 of the dense/banded r^2 path: diff = n n11 - nA nB in int64 (n^2 D
+    exactly), r^2 = (diff * diff) / ((nA (n - nA)) * (nB (n - nB))) with each
+    factor converted to float64 and one division; NaN where the denominator is
+    0.  The same value as r_squared (haplotype_kernels.py:32-42) up to
+    rounding: that form rounds pA, pB and D before combining them."""
+    c = np.asarray(counts, dtype=np.int64)
+    n11 = c[:, 0]
+    nA = c[:, 0] + c[:, 1]
+    nB = c[:, 0] + c[:, 2]
+    n = c.sum(axis=1)
+    dd = (n * n11 - nA * nB).astype(np.float64)
+    num = dd * dd
+    den = (nA * (n - nA)).astype(np.float64) * (nB * (n - nB)).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(den > 0.0, num / np.where(den > 0.0, den, 1.0), np.nan)
+
+
+def r2_histogram(r2, keys, n_keys, n_hist):
+    """(hist (n_keys, n_hist), nan (n_keys,)) of r^2 values per key: bin
+    min(floor(r2 * n_hist), n_hist - 1); keys outside [0, n_keys) dropped."""
+    r2 = np.asarray(r2)
+    keys = np.asarray(keys)
+    ok = (keys >= 0) & (keys < n_keys)
+    fin = ok & ~np.isnan(r2)
+    b = np.minimum((r2[fin] * float(n_hist)).astype(np.int64), n_hist - 1)
+    hist = np.zeros((n_keys, n_hist), dtype=np.int64)
+    np.add.at(hist, (keys[fin], b), 1)
+    nan = np.bincount(keys[ok & np.isnan(r2)], minlength=n_keys)[:n_keys]
+    return hist, nan
+
+
 def d_prime(counts, n_valid=None):
     """Lewontin D' (haplotype_kernels.py:44-62)."""
     a, b, c, d, n = _rnd_inputs(counts, n_valid)

@@ -163,6 +163,47 @@ class BandPlan:
                L.ptr(sums), L.ptr(kp), L.stream_handle())
         return (sums, kp) if want_pairs else sums
 
+    def r2(self, pop, n_hist=100, thresh=0.2, sparse_cap=1 << 20, per_pair=False, histogram=True,
+           with_missing=None):
+        """r^2 of every band pair of one haplotype population through the MFMA
+        tiles (pgld_band_r2).  Returns a dict with
+          hist      int64 (n_keys, n_hist) pairs per r^2 bin [b/n, (b+1)/n),
+          nan_pairs int64 (n_keys,) pairs whose r^2 is NaN,
+          sparse_i, sparse_j, sparse_r2  pairs with r^2 > thresh, sorted by
+                    (i, j) (at most sparse_cap are kept),
+          n_sparse  number of such pairs (may exceed sparse_cap),
+          r2        FP64 per band pair in (i, j) order if per_pair."""
+        if pop.family != L.FAMILY_HAP:
+            raise ValueError("r^2 is defined here for haplotype populations")
+        if with_missing is None:
+            with_missing = pop.has_missing
+            if with_missing is None:
+                raise ValueError("with_missing must be given for populations packed with validate=False")
+        T = L.torch()
+        hist = L.empty((self.n_keys, n_hist + 1), np.int64) if histogram else None
+        cap = int(sparse_cap) if sparse_cap else 0
+        sp_i = L.empty((max(cap, 1),), np.int32)
+        sp_j = L.empty((max(cap, 1),), np.int32)
+        sp_r = L.empty((max(cap, 1),), np.float32)
+        n_sp = L.empty((1,), np.int64)
+        r2 = L.empty((self.n_pairs,), np.float64) if per_pair else None
+        L.call("pgld_band_r2", self._h, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad, pop.n_samples,
+               int(bool(with_missing)), int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),
+               L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.stream_handle())
+        n_total = int(n_sp.cpu().item())
+        k = min(n_total, cap)
+        si, sj, sr = sp_i[:k], sp_j[:k], sp_r[:k]
+        if k:
+            order = T.argsort(si.to(T.int64) * self.n_var + sj.to(T.int64))
+            si, sj, sr = si[order], sj[order], sr[order]
+        out = {"sparse_i": si, "sparse_j": sj, "sparse_r2": sr, "n_sparse": n_total}
+        if histogram:
+            out["hist"] = hist[:, :n_hist]
+            out["nan_pairs"] = hist[:, n_hist]
+        if per_pair:
+            out["r2"] = r2
+        return out
+
     def counts(self, pop, with_missing=None):
         """Per-pair counts of one population through the MFMA path, in the
         band's (i, j) pair order: int32 (n_pairs, 4) or (n_pairs, 9)."""

@@ -22,245 +22,14 @@
 #include <algorithm>
 #include <vector>
 
-#include "common.h"
+#include "band_core.h"
 #include "ld_math.h"
-#include "../../include/pgld.h"
 
 namespace pgld {
 #include "epilogue.inc"
 }
 
-struct pgld_band_plan {
-  int64_t n_var = 0, n_pairs = 0, n_tiles = 0, n_slots = 0;
-  double max_dist = 0.0, origin = 0.0, width = 1.0;
-  int32_t key_mode = 0, n_keys = 0, max_tile_slots = 0;
-  double* d_pos = nullptr;       // band coordinate
-  double* d_kc = nullptr;        // key coordinate
-  double* d_edges = nullptr;     // n_keys + 1 (bin modes)
-  void* d_tiles = nullptr;       // BandTile[n_tiles]
-  int64_t* d_pair_base = nullptr;  // exclusive prefix of per-variant pair counts
-  int32_t* d_key_off = nullptr;    // n_keys + 1
-  int64_t* d_key_list = nullptr;   // slots grouped by key, ascending tile order
-  double* d_partials = nullptr;    // workspace
-  size_t partials_cap = 0;         // doubles
-  bool timing = false;             // record events around each k_band_sums launch
-  hipEvent_t ev[2] = {nullptr, nullptr};
-};
-
 namespace pgld {
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
-
-constexpr int BT = 64;       // tile edge (SNPs)
-constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
-constexpr int BAND_THREADS = 256;
-constexpr int RED_LDS_BYTES = 64 * 1024;
-constexpr int MAX_LDS_EDGES = 130;  // bin edges kept in LDS when n_keys < 130
-// static LDS of k_band_sums: positions, key coordinates, window keys, edges
-constexpr int STATIC_LDS_BYTES = 4 * 64 * 8 + 64 * 4 + MAX_LDS_EDGES * 8;
-
-struct BandTile {
-  int32_t i0, j0, key_lo, n_slots;
-  int64_t slot_base;
-};
-
-struct PopArgs {
-  const int8_t* planes[4];
-  const int32_t* marg[4];
-  int64_t k_pad[4];
-  int32_t n_samples[4];
-};
-
-// ---------------------------------------------------------------------------
-// Plane selection per family / missing-data mode
-// ---------------------------------------------------------------------------
-template <int FAMILY, bool MISSING>
-struct Planes;
-template <>
-struct Planes<PGLD_FAMILY_HAP, true> {  // planes [h, v]; products {h, v} x {h, v}
-  static constexpr int N = 2, FIRST = 0, NPL = 2, NC = 4;
-};
-template <>
-struct Planes<PGLD_FAMILY_HAP, false> {  // h only
-  static constexpr int N = 1, FIRST = 0, NPL = 2, NC = 4;
-};
-template <>
-struct Planes<PGLD_FAMILY_GENO, true> {  // planes [v, e1, e2]; all 9 products
-  static constexpr int N = 3, FIRST = 0, NPL = 3, NC = 9;
-};
-template <>
-struct Planes<PGLD_FAMILY_GENO, false> {  // e1, e2 only
-  static constexpr int N = 2, FIRST = 1, NPL = 3, NC = 9;
-};
-
-template <int N>
-struct WaveAcc {
-  v16i c[N][N];
-};
-
-// LDS-staged products for the whole workgroup (all four waves must call it:
-// it contains barriers).  Per 128-byte K chunk the 64 A rows and 64 B rows of
-// every plane are copied with full-line loads (8 lanes per 128-byte line) into
-// LDS rows padded to 144 bytes, which makes the MFMA-layout ds_read_b128 reads
-// (lane l -> row l % 32) bank-conflict free.  The next chunk's global loads
-// are issued before the MFMAs of the current chunk.
-constexpr int OP_ROW = 144;  // padded LDS row stride (bytes)
-
-template <int N>
-__host__ __device__ constexpr int opbuf_bytes() { return 2 * N * BT * OP_ROW; }
-
-template <int FAMILY, bool MISSING>
-__device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
-                                               int i0, int j0, int wr, int wc, bool wave_live,
-                                               unsigned char* __restrict__ opbuf,
-                                               WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
-  using PL = Planes<FAMILY, MISSING>;
-  constexpr int N = PL::N;
-  constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int8_t* src[ITEMS];
-  int dst[ITEMS];
-#pragma unroll
-  for (int m = 0; m < ITEMS; ++m) {
-    const int e = tid + BAND_THREADS * m;
-    const int q = e & 7, row = (e >> 3) & (BT - 1), rest = e >> 9;
-    const int a = rest % N, side = rest / N;
-    const int64_t snp = min((int64_t)(side ? j0 : i0) + row, n_var - 1);
-    src[m] = planes + ((int64_t)(PL::FIRST + a) * n_var + snp) * k_pad + 16 * q;
-    dst[m] = ((side * N + a) * BT + row) * OP_ROW + 16 * q;
-  }
-#pragma unroll
-  for (int a = 0; a < N; ++a)
-#pragma unroll
-    for (int b = 0; b < N; ++b)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
-  v4i pre[ITEMS];
-#pragma unroll
-  for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m]);
-  const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
-  for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
-    __syncthreads();  // the previous chunk has been read by every wave
-#pragma unroll
-    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dst[m]) = pre[m];
-    __syncthreads();
-    if (k0 + KSTEP < k_pad) {
-#pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + KSTEP);
-    }
-    if (wave_live) {
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        v4i A[N], B[N];
-#pragma unroll
-        for (int a = 0; a < N; ++a) {
-          A[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
-          B[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
-        }
-#pragma unroll
-        for (int a = 0; a < N; ++a)
-#pragma unroll
-          for (int b = 0; b < N; ++b)
-            acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc.c[a][b], 0, 0, 0);
-      }
-    }
-  }
-}
-
-// Joint counts of accumulator element r (row i, col j) in the reference's
-// column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
-template <int FAMILY, bool MISSING>
-__device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING>::N>& acc, int r,
-                                            const int32_t* __restrict__ marg, int64_t n_var, int64_t i, int64_t j,
-                                            int32_t S, int* out) {
-  if constexpr (FAMILY == PGLD_FAMILY_HAP) {
-    int hh, hv, vh, vv;
-    hh = acc.c[0][0][r];
-    if constexpr (MISSING) {
-      hv = acc.c[0][1][r];
-      vh = acc.c[1][0][r];
-      vv = acc.c[1][1][r];
-    } else {
-      hv = marg[i];  // plane 0 (h) sums
-      vh = marg[j];
-      vv = S;
-    }
-    out[0] = hh;
-    out[1] = hv - hh;
-    out[2] = vh - hh;
-    out[3] = vv - hv - vh + hh;
-  } else {
-    int N[9];  // N[3a+b], a/b: 0 = valid, 1 = e1, 2 = e2
-    if constexpr (MISSING) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) N[3 * a + b] = acc.c[a][b][r];
-    } else {
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) N[3 * (a + 1) + (b + 1)] = acc.c[a][b][r];
-      N[0] = S;
-      N[1] = marg[n_var + j];      // e1 sums
-      N[2] = marg[2 * n_var + j];  // e2 sums
-      N[3] = marg[n_var + i];
-      N[6] = marg[2 * n_var + i];
-    }
-    int n11 = N[4], n12 = N[5], n21 = N[7], n22 = N[8];
-    int n10 = N[3] - n11 - n12;
-    int n20 = N[6] - n21 - n22;
-    int n01 = N[1] - n11 - n21;
-    int n02 = N[2] - n12 - n22;
-    out[0] = N[0] - n10 - n20 - n01 - n02 - n11 - n12 - n21 - n22;
-    out[1] = n01; out[2] = n02;
-    out[3] = n10; out[4] = n11; out[5] = n12;
-    out[6] = n20; out[7] = n21; out[8] = n22;
-  }
-}
-
-__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
-
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
-
-__device__ __forceinline__ int32_t key_bin(double d, const double* edges, int32_t n_keys) {
-  if (!(d >= edges[0]) || !(d < edges[n_keys])) return -1;
-  int lo = 0, hi = n_keys;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (edges[mid] <= d) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD
-// b % 8).  Tile t of the band schedule is given to workgroup b such that each
-// XCD walks one contiguous range of tiles: consecutive tiles share their A rows
-// and most B columns, so they hit the same L2.
-constexpr int N_XCD = 8;
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
-  const int64_t x = b % N_XCD, slot = b / N_XCD;
-  const int64_t per = n / N_XCD, rem = n % N_XCD;
-  return x * per + (x < rem ? x : rem) + slot;
-}
-
-struct BandArgs {
-  const BandTile* tiles;
-  const double* pos;
-  const double* kc;
-  const double* edges;
-  int64_t n_var;
-  double max_dist, origin, width;
-  int32_t key_mode, n_keys;
-  double* partials;
-  int32_t* counts_out;       // counts mode
-  const int64_t* pair_base;  // counts mode
-};
 
 template <int FAMILY, int P, int SET>
 struct StatSet;

@@ -281,3 +281,58 @@ def test_band_sums_deterministic():
     a = np_(plan.sums(packed))
     b = np_(plan.sums(packed))
     np.testing.assert_array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# r^2 through the MFMA tiles: identical FP64 operations as the oracle, so the
+# per-pair values, histograms and sparse lists must match exactly
+# ---------------------------------------------------------------------------
+def _r2_oracle(d, pos, md):
+    ii, jj = O.pairs_within_distance(pos, md)
+    c, _ = O.hap_counts(d, ii, jj)
+    return ii, jj, O.r_squared_exact(c)
+
+
+@pytest.mark.parametrize("missing", [False, True])
+def test_band_r2_windows_exact(missing):
+    d, pos = hap_data(91, 150, 700, 0.03 if missing else 0.0)
+    d[:, 5] = 0  # a monomorphic site -> NaN pairs
+    md, width = 6000.0, 15000.0
+    plan = band.BandPlan(pos, md, window=width)
+    pk = band.pack_population(d, None, "hap")
+    assert pk.has_missing == missing
+    out = plan.r2(pk, n_hist=50, thresh=0.1, per_pair=True)
+    ii, jj, r2 = _r2_oracle(d, pos, md)
+    got = np_(out["r2"])
+    nan = np.isnan(r2)
+    assert nan.any()
+    np.testing.assert_array_equal(np.isnan(got), nan)
+    np.testing.assert_array_equal(got[~nan], r2[~nan])
+    h, hn = O.r2_histogram(r2, O.window_key(pos[ii], 0.0, width), plan.n_keys, 50)
+    np.testing.assert_array_equal(np_(out["hist"]), h)
+    np.testing.assert_array_equal(np_(out["nan_pairs"]), hn)
+    sel = ~nan & (r2 > 0.1)
+    assert out["n_sparse"] == int(sel.sum()) > 0
+    np.testing.assert_array_equal(np_(out["sparse_i"]), ii[sel])
+    np.testing.assert_array_equal(np_(out["sparse_j"]), jj[sel])
+    np.testing.assert_array_equal(np_(out["sparse_r2"]), r2[sel].astype(np.float32))
+
+
+def test_band_r2_all_pairs_bins_and_capacity():
+    d, pos = hap_data(92, 260, 500, 0.0)
+    edges = np.array([0.0, 1000.0, 5000.0, 20000.0, 1e9])
+    plan = band.BandPlan(pos, np.inf, edges=edges)
+    assert plan.n_pairs == 500 * 499 // 2
+    pk = band.pack_population(d, None, "hap")
+    full = plan.r2(pk, n_hist=100, thresh=0.05)
+    ii, jj, r2 = _r2_oracle(d, pos, np.inf)
+    h, hn = O.r2_histogram(r2, O.bin_index(pos[jj] - pos[ii], edges), 4, 100)
+    np.testing.assert_array_equal(np_(full["hist"]), h)
+    np.testing.assert_array_equal(np_(full["nan_pairs"]), hn)
+    n_hit = int((r2 > 0.05).sum())
+    assert full["n_sparse"] == n_hit > 20
+    # a capacity below the hit count keeps a subset of true hits and reports the total
+    cut = plan.r2(pk, n_hist=100, thresh=0.05, sparse_cap=7, histogram=False)
+    assert cut["n_sparse"] == n_hit and len(cut["sparse_i"]) == 7
+    hits = set(zip(ii[r2 > 0.05].tolist(), jj[r2 > 0.05].tolist()))
+    assert set(zip(np_(cut["sparse_i"]).tolist(), np_(cut["sparse_j"]).tolist())) <= hits

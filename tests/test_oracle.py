@@ -233,3 +233,26 @@ def test_keyed_sums_bruteforce():
     # digitize semantics: [b_k, b_{k+1}); the last edge itself is outside
     e = np.array([1.0, 10.0, 100.0])
     np.testing.assert_array_equal(O.bin_index([0.5, 1.0, 9.99, 10.0, 99.9, 100.0], e), [-1, 0, 0, 1, 1, 2])
+
+
+def test_r_squared_exact_matches_reference_form():
+    """The exact-integer r^2 of the dense path equals the reference's float
+    formula up to rounding, on random counts with and without monomorphic
+    margins, and is 1 for identical and 0 for independent-by-construction
+    columns."""
+    rng = np.random.default_rng(11)
+    c = rng.integers(0, 60, size=(4000, 4))
+    c[:50, 1:3] = 0          # perfect association -> r^2 = 1
+    c[50:80, [0, 1]] = 0     # site A monomorphic -> NaN
+    got = O.r_squared_exact(c)
+    want = O.r_squared(c)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_allclose(got[ok], want[ok], rtol=1e-13, atol=1e-300)
+    assert np.all(got[:50][~np.isnan(got[:50])] == 1.0)
+    # n11 * n00 == n10 * n01 -> D = 0 -> r^2 = 0 exactly
+    z = np.array([[6, 3, 4, 2], [10, 5, 2, 1]])
+    np.testing.assert_array_equal(O.r_squared_exact(z), [0.0, 0.0])
+    h, nan = O.r2_histogram(np.array([0.0, 0.0099, 0.01, 0.5, 1.0, np.nan, 0.3]),
+                            np.array([0, 0, 0, 1, 1, 1, -1]), 2, 100)
+    assert h[0, 0] == 2 and h[0, 1] == 1 and h[1, 50] == 1 and h[1, 99] == 1 and list(nan) == [0, 1]
