@@ -10,7 +10,8 @@ fused tiled int8-MFMA band pass + the per-window reduction.
 Other workloads (--workload): genotype_bins (50,000 SNPs x 1,000 diploids, 5%
 missing, all pairs in 1 Mb, 16 log distance bins), plain_hap (2,000 SNPs x
 200 haplotypes, per-pair r^2/DD/Dz/pi2 within 20 kb through the plain
-per-pair kernels).
+per-pair kernels), dense_r2 (40,000 SNPs x 10,000 haplotypes, r^2 of all
+7.9998e8 pairs, 100-bin histogram + sparse list of pairs with r^2 > 0.2).
 
 Prints one JSON line: value is Gpair*samples/s for the whole job; roofline is
 for the dominant kernel; cpu_baseline is the NumPy oracle on a bounded sample
@@ -44,7 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="windowed_hap", choices=["windowed_hap", "genotype_bins", "plain_hap"])
+    ap.add_argument("--workload", default="windowed_hap",
+                    choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2"])
     ap.add_argument("--n-var", type=int, default=None)
     ap.add_argument("--n-samples", "--n-hap", dest="n_samples", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=1)
@@ -374,6 +376,67 @@ def run_plain_hap(args, torch):
     return line
 
 
+def run_dense_r2(args, torch):
+    from pg_gpu_amd import band, synthetic
+    from pg_gpu_amd import _lib as L
+
+    n_var = args.n_var or 40_000
+    n_hap = args.n_samples or 10_000
+    macs = 1  # haplotypes without missing data: h_i.h_j only
+    log("[bench] generating workload")
+    hap, pos = synthetic.dense_r2_panel(n_hap=n_hap, n_var=n_var)
+    dev = L.device()
+    hap_d = torch.as_tensor(hap, device=dev)
+    plan = band.BandPlan(pos, np.inf, window=1e15, n_keys=1)
+    pair_samples = plan.n_pairs * n_hap
+    log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
+    pk = band.pack_population(hap_d, None, "hap")
+    assert not pk.has_missing
+    cap = 1 << 22
+
+    def step():
+        band.pack_population(hap_d, None, "hap", validate=False, out=pk)
+        return plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False)
+
+    ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    log(f"[bench] step {ms_step:.3f} ms, {out['n_sparse']} pairs with r^2 > 0.2, "
+        f"{int(out['nan_pairs'].sum())} NaN pairs")
+    plan.set_timing(True)
+    kms = []
+    for _ in range(max(3, args.steps)):
+        plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False)
+        kms.append(plan.kernel_ms())
+    plan.set_timing(False)
+    k_ms = float(np.mean(kms))
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_band_r2", macs)
+    log(f"[bench] k_band_r2 {k_ms:.3f} ms -> {roof['achieved']:.1f} int8 TOP/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ld_oracle as O
+        ii, jj = band_pair_sample(pos, np.inf, args.cpu_pairs or 60_000)
+        t2 = time.perf_counter()
+        hist = np.zeros((1, 100), dtype=np.int64)
+        for c0 in range(0, len(ii), 5000):
+            a, b = ii[c0:c0 + 5000], jj[c0:c0 + 5000]
+            c, _ = O.hap_counts(hap, a, b, None)
+            r2 = O.r_squared_exact(c)
+            hist += O.r2_histogram(r2, np.zeros(len(a), dtype=np.int64), 1, 100)[0]
+        cpu_s = time.perf_counter() - t2
+        cpu = {"value": len(ii) * n_hap / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
+               "sample": f"{len(ii)} random pairs of the {plan.n_pairs}, counts + r^2 + histogram, NumPy oracle "
+                         f"(single thread), {cpu_s:.1f} s"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
+    line = base_line(args, value, ms_step,
+                     f"dense all-pairs r^2: {n_var:,} SNPs x {n_hap:,} haplotypes, neutral SFS, no missing data; "
+                     "r^2 of every pair from exact int32 counts, 100-bin histogram on [0, 1] (NaN pairs counted "
+                     "apart) + sparse (i, j, r^2) list for r^2 > 0.2",
+                     {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
+                      "n_sparse": out["n_sparse"]})
+    line.update({"roofline": roof, "cpu_baseline": cpu})
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -391,7 +454,8 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {n} bytes 4 times")
         return
-    run = {"windowed_hap": run_windowed_hap, "genotype_bins": run_genotype_bins, "plain_hap": run_plain_hap}
+    run = {"windowed_hap": run_windowed_hap, "genotype_bins": run_genotype_bins, "plain_hap": run_plain_hap,
+           "dense_r2": run_dense_r2}
     line = run[args.workload](args, torch)
     print(json.dumps(line), flush=True)
 

@@ -109,3 +109,12 @@ def genotype_panel(seed=1003, n_ind=1000, n_var=50_000, span=1_000_000, missing=
     g = hwe_genotypes(n_ind, n_var, rng, missing=missing)
     pos = uniform_positions(n_var, span, rng)
     return g, pos
+
+
+def dense_r2_panel(seed=1004, n_hap=10_000, n_var=40_000, span=4_000_000):
+    """Dense all-pairs r^2: 40,000 SNPs x 10,000 haplotypes, neutral SFS as
+    the plain panel, no missing data, positions uniform over 4 Mb."""
+    rng = np.random.default_rng(seed)
+    hap = neutral_haplotypes(n_hap, n_var, rng)
+    pos = uniform_positions(n_var, span, rng)
+    return hap, pos
