@@ -220,6 +220,50 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                  int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                  double* r2_out, void* stream);
 
+/* ---- ragged region batches: many independent regions, one launch ------ */
+
+/* A band plan over n_regions independent regions (no pair crosses regions).
+ * pos (host) holds the regions' positions one after the other (n_var[r]
+ * values for region r, each sorted); key_coord likewise or NULL; edges /
+ * n_keys / origin / width describe each region's keys as in
+ * pgld_band_plan_create, and the plan's keys are region * n_keys + key.
+ * The reference loops over regions from Python (one moments_ld call each,
+ * moments_ld.py:40-203); this runs them as one persistent launch. */
+int pgld_batch_plan_create(int32_t n_regions, const int64_t* n_var, const double* pos, double max_dist,
+                           int32_t key_mode, const double* key_coord, const double* edges, int32_t n_keys,
+                           double origin, double width, pgld_band_plan** plan);
+/* Bind the packed planes of every region's populations (host arrays of
+ * n_regions * n_pops entries, region-major: entry r * n_pops + p), uploads the
+ * region table, orders the tiles for the persistent launch and sizes the
+ * workspace for the statistic set.  Synchronous; call before any capture. */
+int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                    const int32_t* n_samples);
+/* Fused counts -> statistics -> per-key sums for every region: sums is
+ * (n_regions * n_keys, n_stats) FP64, key_pairs (optional) the pair count
+ * per key.  Allocates nothing, records no memset: may be captured. */
+int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream);
+/* Schedule of the persistent launch: grid workgroups (0 = one per tile),
+ * order 0 = cost-descending tiles, 1 = natural; xcd_split = each XCD walks a
+ * contiguous eighth of the order. */
+int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split);
+/* Autotuner: time the schedule candidates on this plan, keep the fastest
+ * (also cached per shape class for the process); outputs may be NULL;
+ * *best_ms = -1 when the cache answered. */
+int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream,
+                        int32_t* grid, int32_t* order, int32_t* xcd_split, float* best_ms);
+
+/* HIP graph capture of a sequence of libpgld calls on one created stream:
+ * pgld_graph_begin(stream); ... calls on that stream ...; pgld_graph_end. The
+ * captured calls must not allocate or synchronize (pack with its buffers,
+ * pgld_batch_sums).  pgld_graph_launch replays the whole sequence as one
+ * submission.  (No counterpart in the reference.) */
+typedef struct pgld_graph pgld_graph;
+int pgld_graph_begin(void* stream);
+int pgld_graph_end(void* stream, pgld_graph** graph);
+int pgld_graph_launch(pgld_graph* graph, void* stream);
+int pgld_graph_destroy(pgld_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

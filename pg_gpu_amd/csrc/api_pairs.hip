@@ -1,5 +1,7 @@
 // Error handling, value scans, pair enumeration, pair keys, keyed sums and
 // the r / r^2 / D' kernels of libpgld.
+#include <algorithm>
+
 #include "common.h"
 #include "ld_math.h"
 #include "../../include/pgld.h"
@@ -18,6 +20,18 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* get_error() { return g_err.c_str(); }
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = v;
+}
+
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return PGLD_OK;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n, 256), 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return check_launch("k_fill_i32");
+}
 
 // ---------------------------------------------------------------------------
 // Value scan (has_missing / domain check)
@@ -182,7 +196,8 @@ int pgld_scan_rows(const int8_t* data, int64_t n_rows, int64_t n_var, const int3
   PGLD_REQUIRE(n_rows >= 0 && n_var >= 0, "pgld_scan_rows: negative size");
   if (!rows) n_sel = n_rows;
   hipStream_t s = as_stream(stream);
-  PGLD_TRY_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s), "pgld_scan_rows memset");
+  int rc = fill_i32(flags, 2, 0, s);
+  if (rc) return rc;
   int64_t total = n_sel * n_var;
   if (total == 0) return PGLD_OK;
   int64_t blocks = ceil_div(total, 256);

@@ -20,6 +20,11 @@
 // row, and a fixed-order per-key reduction over tiles.  Results are therefore
 // identical from run to run.
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "band_core.h"
@@ -95,21 +100,21 @@ constexpr int band_min_waves(int family, int P, int set) {
              : 1;
 }
 
+// One tile of the fused band pass.  key_base moves the tile's local keys
+// (distance bin or window) into the plan's key space: 0 for a single band,
+// region * n_keys_local for the tiles of a region batch.
 template <int FAMILY, bool MISSING, int P, int SET>
-__global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
-                                                                                            PopArgs pa) {
+__device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
+                                               int32_t key_base, unsigned char* smem) {
   using PL = Planes<FAMILY, MISSING>;
   using SS = StatSet<FAMILY, P, SET>;
   constexpr int NC = PL::NC;
   constexpr int NS = SS::N + 1;  // statistics + pair count
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  // layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
+  // smem layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
   __shared__ int s_wkey[BT];
   __shared__ double s_edges[MAX_LDS_EDGES];
 
-  const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  if (tile.n_slots <= 0) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   double* red = reinterpret_cast<double*>(smem);
   const int red_n = 4 * tile.n_slots * NS;
@@ -206,6 +211,7 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
           if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
           key = key_bin(d, edges, ba.n_keys);
         }
+        if (key >= 0) key += key_base;
       }
       const bool has = valid && key >= 0;
       const bool need = has && ck >= 0 && key != ck;
@@ -240,6 +246,50 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
   double* dst = ba.partials + tile.slot_base * NS;
   for (int t = tid; t < row_n; t += BAND_THREADS)
     dst[t] = ((red[t] + red[row_n + t]) + red[2 * row_n + t]) + red[3 * row_n + t];
+}
+
+template <int FAMILY, bool MISSING, int P, int SET>
+__global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
+                                                                                            PopArgs pa) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  if (tile.n_slots <= 0) return;
+  band_sums_tile<FAMILY, MISSING, P, SET>(ba, pa, tile, 0, smem);
+}
+
+// Region batch, persistent: workgroup b takes tiles order[b], order[b + grid],
+// ... .  Each tile's region supplies its populations' planes, its SNP range
+// (positions are concatenated over regions) and its key offset.  The per-key
+// reduction runs over slots in schedule-independent order, so the result does
+// not depend on the grid or the tile order.
+template <int FAMILY, bool MISSING, int P, int SET>
+__global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_batch_sums(
+    BandArgs ba, const RegionDesc* __restrict__ regions, const int32_t* __restrict__ order, int64_t n_tiles,
+    int32_t xcd_split) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  // xcd_split: workgroup b runs on XCD b % 8 (round-robin dispatch); each XCD
+  // walks its own contiguous eighth of the order so that neighbouring tiles
+  // (same region, shared plane rows) meet in one L2.
+  int64_t t0 = blockIdx.x, step = gridDim.x, t_end = n_tiles;
+  if (xcd_split && gridDim.x % N_XCD == 0) {
+    const int64_t x = blockIdx.x % N_XCD;
+    t0 = n_tiles * x / N_XCD + blockIdx.x / N_XCD;
+    step = gridDim.x / N_XCD;
+    t_end = n_tiles * (x + 1) / N_XCD;
+  }
+  for (int64_t t = t0; t < t_end; t += step) {
+    const BandTile tile = ba.tiles[order[t]];
+    if (tile.n_slots <= 0) continue;
+    const RegionDesc& rd = regions[tile.region];
+    BandArgs lb = ba;
+    lb.pos = ba.pos + rd.snp_base;
+    lb.kc = ba.kc + rd.snp_base;
+    lb.n_var = rd.n_var;
+    const PopArgs pa = rd.pa;
+    const int32_t key_base = rd.key_base;
+    __syncthreads();  // the previous tile's LDS is no longer read
+    band_sums_tile<FAMILY, MISSING, P, SET>(lb, pa, tile, key_base, smem);
+  }
 }
 
 // Counts mode: one population, per-pair counts in the band's (i, j) order.
@@ -429,11 +479,141 @@ static int bin_host(double d, const std::vector<double>& e) {
   return (int)(std::upper_bound(e.begin(), e.end(), d) - e.begin()) - 1;
 }
 
+// ---------------------------------------------------------------------------
+// Band schedule construction (host)
+// ---------------------------------------------------------------------------
+struct TileBuild {
+  std::vector<BandTile> tiles;
+  std::vector<BandTileHost> host;
+  std::vector<std::pair<int32_t, int64_t>> key_slot;  // (key, slot) in tile order
+  int64_t n_slots = 0;
+  int max_ts = 0;
+};
+
+static int check_plan_args(const double* pos, const double* key_coord, int64_t n_var, double max_dist,
+                           int32_t key_mode, const double* edges, int32_t n_keys, double width,
+                           std::vector<double>& e) {
+  PGLD_REQUIRE(max_dist >= 0.0, "band plan: negative max_dist");
+  PGLD_REQUIRE(key_mode == PGLD_KEY_BIN_ABS || key_mode == PGLD_KEY_BIN || key_mode == PGLD_KEY_WINDOW,
+               "band plan: unknown key mode %d", key_mode);
+  PGLD_REQUIRE(n_keys >= 1, "band plan: n_keys must be >= 1");
+  PGLD_REQUIRE(key_mode == PGLD_KEY_WINDOW ? width > 0.0 : edges != nullptr,
+               "band plan: windows need width > 0, bins need edges");
+  const double* kc = key_coord ? key_coord : pos;
+  for (int64_t v = 1; v < n_var; ++v) {
+    PGLD_REQUIRE(pos[v] >= pos[v - 1], "band plan: positions not sorted at %lld", (long long)v);
+    PGLD_REQUIRE(kc[v] >= kc[v - 1], "band plan: key coordinate decreases at %lld", (long long)v);
+  }
+  e.clear();
+  if (key_mode != PGLD_KEY_WINDOW) {
+    e.assign(edges, edges + n_keys + 1);
+    for (int k = 0; k < n_keys; ++k) PGLD_REQUIRE(e[k] < e[k + 1], "band plan: edges must increase");
+  }
+  return PGLD_OK;
+}
+
+// Tiles of one region (SNPs [0, n_var) of pos / kc): 64 x 64 tiles from each
+// 64-SNP row to the last column any of its SNPs reaches within max_dist, each
+// with the key range its pairs can fall in (keys shifted by key_base).
+// Returns the region's pair count; base (if given) receives the exclusive
+// prefix of per-SNP pair counts.
+static int64_t build_region(const double* pos, const double* kc, int64_t n_var, double max_dist, int32_t key_mode,
+                            const std::vector<double>& e, int32_t n_keys, double origin, double width, int32_t region,
+                            int32_t key_base, std::vector<int64_t>* base_out, TileBuild& tb) {
+  std::vector<int64_t> per(n_var), base(n_var + 1, 0);
+  for (int64_t v = 0; v < n_var; ++v) {
+    int64_t u = upper_index(pos, n_var, v, pos[v] + max_dist);
+    per[v] = std::max<int64_t>(0, u - v - 1);
+    base[v + 1] = base[v] + per[v];
+  }
+  auto key_i = [&](int64_t i) -> int64_t {  // window key of the left SNP
+    return (int64_t)std::floor((kc[i] - origin) / width);
+  };
+  const int64_t n_rows = ceil_div(n_var, BT);
+  for (int64_t r = 0; r < n_rows; ++r) {
+    const int64_t i0 = r * BT, il = std::min(i0 + BT - 1, n_var - 1);
+    if (base[il + 1] - base[i0] == 0) continue;
+    const int64_t jhi = std::max(il, il + per[il]);
+    for (int64_t tc = r; tc * BT <= jhi; ++tc) {
+      const int64_t j0 = tc * BT, jl = std::min(j0 + BT - 1, n_var - 1);
+      int64_t klo, khi;
+      if (key_mode == PGLD_KEY_WINDOW) {
+        klo = std::max<int64_t>(0, key_i(i0));
+        khi = std::min<int64_t>(n_keys - 1, key_i(il));
+      } else {
+        double dmin = std::max(0.0, kc[j0] - kc[il]);
+        double dmax = kc[jl] - kc[i0];
+        if (dmax < e[0] || dmin >= e[n_keys]) {
+          klo = 1;
+          khi = 0;
+        } else {
+          klo = dmin < e[0] ? 0 : bin_host(dmin, e);
+          khi = dmax >= e[n_keys] ? n_keys - 1 : bin_host(dmax, e);
+        }
+      }
+      BandTile t{};
+      t.i0 = (int32_t)i0;
+      t.j0 = (int32_t)j0;
+      t.key_lo = (int32_t)(klo + key_base);
+      t.n_slots = khi >= klo ? (int32_t)(khi - klo + 1) : 0;
+      t.slot_base = tb.n_slots;
+      t.region = region;
+      for (int q = 0; q < t.n_slots; ++q) tb.key_slot.push_back({(int32_t)(t.key_lo + q), tb.n_slots + q});
+      tb.n_slots += t.n_slots;
+      tb.max_ts = std::max(tb.max_ts, t.n_slots);
+      tb.tiles.push_back(t);
+      tb.host.push_back(BandTileHost{region, (int32_t)(il - i0 + 1), (int32_t)(jl - j0 + 1), (int32_t)(tc == r)});
+    }
+  }
+  if (base_out) base_out->assign(base.begin(), base.end() - 1);
+  return base[n_var];
+}
+
+static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc, int64_t n_var,
+                       const std::vector<double>& e, TileBuild& tb, const std::vector<int64_t>& base) {
+  plan->n_tiles = (int64_t)tb.tiles.size();
+  plan->n_slots = tb.n_slots;
+  plan->max_tile_slots = tb.max_ts;
+  const int32_t n_keys = plan->n_keys;
+  std::vector<int32_t> key_off(n_keys + 1, 0);
+  for (auto& ks : tb.key_slot) key_off[ks.first + 1]++;
+  for (int k = 0; k < n_keys; ++k) key_off[k + 1] += key_off[k];
+  std::vector<int64_t> key_list(tb.key_slot.size());
+  {
+    std::vector<int32_t> fill(key_off.begin(), key_off.end() - 1);
+    for (auto& ks : tb.key_slot) key_list[fill[ks.first]++] = ks.second;
+  }
+  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+    hipError_t err = hipMalloc(dst, bytes ? bytes : 8);
+    if (err == hipSuccess && src && bytes) err = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    return check_hip(err, "band plan upload");
+  };
+  int rc = PGLD_OK;
+  if (!rc) rc = up((void**)&plan->d_pos, pos, n_var * sizeof(double));
+  if (!rc) rc = up((void**)&plan->d_kc, kc, n_var * sizeof(double));
+  if (!rc) rc = up((void**)&plan->d_edges, e.empty() ? nullptr : e.data(), e.size() * sizeof(double));
+  if (!rc) rc = up(&plan->d_tiles, tb.tiles.data(), tb.tiles.size() * sizeof(BandTile));
+  if (!rc) rc = up((void**)&plan->d_pair_base, base.empty() ? nullptr : base.data(), base.size() * sizeof(int64_t));
+  if (!rc) rc = up((void**)&plan->d_key_off, key_off.data(), key_off.size() * sizeof(int32_t));
+  if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
+                   key_list.size() * sizeof(int64_t));
+  if (plan->n_regions > 0) plan->h_tiles.swap(tb.host);
+  return rc;
+}
+
 template <int FAMILY, bool MISSING, int P, int SET>
 static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
                         hipStream_t s) {
-  hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)plan->n_tiles), dim3(BAND_THREADS), lds,
-                     s, ba, pa);
+  if (plan->n_regions > 0) {
+    const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, plan->n_tiles) : plan->n_tiles;
+    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)grid), dim3(BAND_THREADS), lds, s,
+                       ba, reinterpret_cast<const RegionDesc*>(plan->d_regions),
+                       plan->lpt ? plan->d_order : plan->d_order + plan->n_tiles, plan->n_tiles,
+                       (int32_t)plan->xcd_split);
+  } else {
+    hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)plan->n_tiles), dim3(BAND_THREADS),
+                       lds, s, ba, pa);
+  }
 }
 
 typedef void (*SumsLauncher)(const pgld_band_plan*, const BandArgs&, const PopArgs&, size_t, hipStream_t);
@@ -482,9 +662,11 @@ int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t
   PGLD_REQUIRE(data && planes && marg && flags, "pgld_pack_planes: null pointer");
   const int npl = family == PGLD_FAMILY_HAP ? 2 : 3;
   hipStream_t s = as_stream(stream);
-  PGLD_TRY_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s), "pgld_pack_planes memset");
+  int rc = fill_i32(flags, 2, 0, s);
+  if (rc) return rc;
   if (n_var == 0) return PGLD_OK;
-  PGLD_TRY_HIP(hipMemsetAsync(marg, 0, (size_t)npl * n_var * sizeof(int32_t), s), "pgld_pack_planes memset");
+  rc = fill_i32(marg, (int64_t)npl * n_var, 0, s);
+  if (rc) return rc;
   if (k_pad == 0) return PGLD_OK;
   dim3 g((unsigned)ceil_div(n_var, PK_V), (unsigned)(k_pad / PK_S));
   const bool words = (n_var % 4 == 0) && (reinterpret_cast<uintptr_t>(data) % 4 == 0);
@@ -506,107 +688,70 @@ int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int
   *out = nullptr;
   PGLD_REQUIRE(n_var >= 0 && n_var < 0x7fffff00, "pgld_band_plan_create: bad n_var");
   PGLD_REQUIRE(n_var == 0 || pos, "pgld_band_plan_create: null pos");
-  PGLD_REQUIRE(max_dist >= 0.0, "pgld_band_plan_create: negative max_dist");
-  PGLD_REQUIRE(key_mode == PGLD_KEY_BIN_ABS || key_mode == PGLD_KEY_BIN || key_mode == PGLD_KEY_WINDOW,
-               "pgld_band_plan_create: unknown key mode %d", key_mode);
-  PGLD_REQUIRE(n_keys >= 1, "pgld_band_plan_create: n_keys must be >= 1");
-  PGLD_REQUIRE(key_mode == PGLD_KEY_WINDOW ? width > 0.0 : edges != nullptr,
-               "pgld_band_plan_create: windows need width > 0, bins need edges");
-  const double* kc = key_coord ? key_coord : pos;
-  for (int64_t v = 1; v < n_var; ++v) {
-    PGLD_REQUIRE(pos[v] >= pos[v - 1], "pgld_band_plan_create: positions not sorted at %lld", (long long)v);
-    PGLD_REQUIRE(kc[v] >= kc[v - 1], "pgld_band_plan_create: key coordinate decreases at %lld", (long long)v);
-  }
   std::vector<double> e;
-  if (key_mode != PGLD_KEY_WINDOW) {
-    e.assign(edges, edges + n_keys + 1);
-    for (int k = 0; k < n_keys; ++k)
-      PGLD_REQUIRE(e[k] < e[k + 1], "pgld_band_plan_create: edges must increase");
-  }
+  int rc = check_plan_args(pos, key_coord, n_var, max_dist, key_mode, edges, n_keys, width, e);
+  if (rc) return rc;
+  const double* kc = key_coord ? key_coord : pos;
   auto* plan = new pgld_band_plan();
   plan->n_var = n_var;
   plan->max_dist = max_dist;
   plan->key_mode = key_mode;
   plan->n_keys = n_keys;
+  plan->n_keys_local = n_keys;
   plan->origin = origin;
   plan->width = width;
-
-  std::vector<int64_t> per(n_var), base(n_var + 1, 0);
-  for (int64_t v = 0; v < n_var; ++v) {
-    int64_t u = upper_index(pos, n_var, v, pos[v] + max_dist);
-    per[v] = std::max<int64_t>(0, u - v - 1);
-    base[v + 1] = base[v] + per[v];
+  TileBuild tb;
+  std::vector<int64_t> base;
+  plan->n_pairs = build_region(pos, kc, n_var, max_dist, key_mode, e, n_keys, origin, width, 0, 0, &base, tb);
+  rc = upload_plan(plan, pos, kc, n_var, e, tb, base);
+  if (rc) {
+    pgld_band_plan_destroy(plan);
+    return rc;
   }
-  plan->n_pairs = base[n_var];
+  *out = plan;
+  return PGLD_OK;
+}
 
-  auto key_i = [&](int64_t i) -> int64_t {  // window key of the left SNP
-    return (int64_t)std::floor((kc[i] - origin) / width);
-  };
-  std::vector<BandTile> tiles;
-  std::vector<std::pair<int32_t, int64_t>> key_slot;  // (key, slot) in tile order
-  int64_t n_slots = 0;
-  int max_ts = 0;
-  const int64_t n_rows = ceil_div(n_var, BT);
-  for (int64_t r = 0; r < n_rows; ++r) {
-    const int64_t i0 = r * BT, il = std::min(i0 + BT - 1, n_var - 1);
-    if (base[il + 1] - base[i0] == 0) continue;
-    const int64_t jhi = std::max(il, il + per[il]);
-    for (int64_t tc = r; tc * BT <= jhi; ++tc) {
-      const int64_t j0 = tc * BT, jl = std::min(j0 + BT - 1, n_var - 1);
-      int64_t klo, khi;
-      if (key_mode == PGLD_KEY_WINDOW) {
-        klo = std::max<int64_t>(0, key_i(i0));
-        khi = std::min<int64_t>(n_keys - 1, key_i(il));
-      } else {
-        double dmin = std::max(0.0, kc[j0] - kc[il]);
-        double dmax = kc[jl] - kc[i0];
-        if (dmax < e[0] || dmin >= e[n_keys]) {
-          klo = 1;
-          khi = 0;
-        } else {
-          klo = dmin < e[0] ? 0 : bin_host(dmin, e);
-          khi = dmax >= e[n_keys] ? n_keys - 1 : bin_host(dmax, e);
-        }
-      }
-      BandTile t;
-      t.i0 = (int32_t)i0;
-      t.j0 = (int32_t)j0;
-      t.key_lo = (int32_t)klo;
-      t.n_slots = khi >= klo ? (int32_t)(khi - klo + 1) : 0;
-      t.slot_base = n_slots;
-      for (int q = 0; q < t.n_slots; ++q) key_slot.push_back({(int32_t)(klo + q), n_slots + q});
-      n_slots += t.n_slots;
-      max_ts = std::max(max_ts, t.n_slots);
-      tiles.push_back(t);
-    }
+int pgld_batch_plan_create(int32_t n_regions, const int64_t* n_var, const double* pos, double max_dist,
+                           int32_t key_mode, const double* key_coord, const double* edges, int32_t n_keys,
+                           double origin, double width, pgld_band_plan** out) {
+  PGLD_REQUIRE(out, "pgld_batch_plan_create: null output");
+  *out = nullptr;
+  PGLD_REQUIRE(n_regions >= 1 && n_var, "pgld_batch_plan_create: need n_regions >= 1 and n_var");
+  PGLD_REQUIRE((int64_t)n_regions * n_keys < 0x7fffffff, "pgld_batch_plan_create: too many keys");
+  std::vector<int64_t> rbase(n_regions + 1, 0);
+  for (int r = 0; r < n_regions; ++r) {
+    PGLD_REQUIRE(n_var[r] >= 0 && n_var[r] < 0x7fffff00, "pgld_batch_plan_create: bad n_var of region %d", r);
+    rbase[r + 1] = rbase[r] + n_var[r];
   }
-  plan->n_tiles = (int64_t)tiles.size();
-  plan->n_slots = n_slots;
-  plan->max_tile_slots = max_ts;
-  std::vector<int32_t> key_off(n_keys + 1, 0);
-  for (auto& ks : key_slot) key_off[ks.first + 1]++;
-  for (int k = 0; k < n_keys; ++k) key_off[k + 1] += key_off[k];
-  std::vector<int64_t> key_list(key_slot.size());
-  {
-    std::vector<int32_t> fill(key_off.begin(), key_off.end() - 1);
-    for (auto& ks : key_slot) key_list[fill[ks.first]++] = ks.second;
+  const int64_t total = rbase[n_regions];
+  PGLD_REQUIRE(total == 0 || pos, "pgld_batch_plan_create: null pos");
+  std::vector<double> e;
+  for (int r = 0; r < n_regions; ++r) {
+    int rc = check_plan_args(pos + rbase[r], key_coord ? key_coord + rbase[r] : nullptr, n_var[r], max_dist,
+                             key_mode, edges, n_keys, width, e);
+    if (rc) return rc;
   }
-
-  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
-    hipError_t err = hipMalloc(dst, bytes ? bytes : 8);
-    if (err == hipSuccess && src && bytes) err = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
-    return check_hip(err, "pgld_band_plan_create upload");
-  };
-  int rc = PGLD_OK;
-  std::vector<double> kcv(kc, kc + n_var);
-  if (!rc) rc = up((void**)&plan->d_pos, pos, n_var * sizeof(double));
-  if (!rc) rc = up((void**)&plan->d_kc, kcv.data(), n_var * sizeof(double));
-  if (!rc) rc = up((void**)&plan->d_edges, e.empty() ? nullptr : e.data(), e.size() * sizeof(double));
-  if (!rc) rc = up(&plan->d_tiles, tiles.data(), tiles.size() * sizeof(BandTile));
-  if (!rc) rc = up((void**)&plan->d_pair_base, base.data(), n_var * sizeof(int64_t));
-  if (!rc) rc = up((void**)&plan->d_key_off, key_off.data(), key_off.size() * sizeof(int32_t));
-  if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
-                   key_list.size() * sizeof(int64_t));
+  const double* kc = key_coord ? key_coord : pos;
+  auto* plan = new pgld_band_plan();
+  plan->n_var = total;
+  plan->max_dist = max_dist;
+  plan->key_mode = key_mode;
+  plan->n_keys = n_regions * n_keys;
+  plan->n_keys_local = n_keys;
+  plan->origin = origin;
+  plan->width = width;
+  plan->n_regions = n_regions;
+  plan->h_region_nvar.assign(n_var, n_var + n_regions);
+  plan->h_region_base.assign(rbase.begin(), rbase.end() - 1);
+  TileBuild tb;
+  int64_t n_pairs = 0;
+  for (int r = 0; r < n_regions; ++r)
+    n_pairs += build_region(pos + rbase[r], kc + rbase[r], n_var[r], max_dist, key_mode, e, n_keys, origin, width, r,
+                            r * n_keys, nullptr, tb);
+  plan->n_pairs = n_pairs;
+  std::vector<int64_t> no_base;
+  int rc = upload_plan(plan, pos, kc, total, e, tb, no_base);
   if (rc) {
     pgld_band_plan_destroy(plan);
     return rc;
@@ -625,8 +770,8 @@ int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n
 
 int pgld_band_plan_destroy(pgld_band_plan* plan) {
   if (!plan) return PGLD_OK;
-  void* ptrs[] = {plan->d_pos, plan->d_kc, plan->d_edges, plan->d_tiles, plan->d_pair_base,
-                  plan->d_key_off, plan->d_key_list, plan->d_partials};
+  void* ptrs[] = {plan->d_pos,     plan->d_kc,       plan->d_edges,    plan->d_tiles,  plan->d_pair_base,
+                  plan->d_key_off, plan->d_key_list, plan->d_partials, plan->d_regions, plan->d_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)
@@ -669,84 +814,308 @@ static int fill_pop_args(int32_t n_pops, const int8_t* const* planes, const int3
   return PGLD_OK;
 }
 
+}  // extern "C"
+
+namespace pgld {
+
+static SumsLauncher launcher_for(int32_t family, int32_t n_pops, bool miss, int32_t stat_set) {
+  if (family == PGLD_FAMILY_HAP)
+    return miss ? pick_launcher<PGLD_FAMILY_HAP, true>(n_pops, stat_set)
+                : pick_launcher<PGLD_FAMILY_HAP, false>(n_pops, stat_set);
+  return miss ? pick_launcher<PGLD_FAMILY_GENO, true>(n_pops, stat_set)
+              : pick_launcher<PGLD_FAMILY_GENO, false>(n_pops, stat_set);
+}
+
+// LDS bytes of one fused workgroup, or 0 when the tile's keys x statistics do
+// not fit (160 KiB of LDS per workgroup on gfx950, minus the static arrays).
+static size_t sums_lds_bytes(const pgld_band_plan* plan, int32_t family, int32_t n_pops, bool miss, int n_stats) {
+  const int ns_total = n_stats + 1;
+  const int nc = family == PGLD_FAMILY_HAP ? 4 : 9;
+  const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
+  const size_t stash_bytes = (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
+  const int n_planes_used = family == PGLD_FAMILY_HAP ? (miss ? 2 : 1) : (miss ? 3 : 2);
+  const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
+  if (red_bytes > (size_t)RED_LDS_BYTES || red_bytes + stash_bytes + op_bytes + STATIC_LDS_BYTES > 160 * 1024)
+    return 0;
+  return red_bytes + stash_bytes + op_bytes;
+}
+
+static int ensure_partials(pgld_band_plan* pl, int64_t ns_total) {
+  const size_t need = (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total;
+  if (pl->partials_cap >= need) return PGLD_OK;
+  if (pl->d_partials) PGLD_TRY_HIP(hipFree(pl->d_partials), "band workspace free");
+  pl->d_partials = nullptr;
+  pl->partials_cap = 0;
+  PGLD_TRY_HIP(hipMalloc((void**)&pl->d_partials, need * sizeof(double)), "band workspace");
+  pl->partials_cap = need;
+  return PGLD_OK;
+}
+
+// The fused pass + per-key reduction.  pa is used by single-band plans;
+// region batches read their populations from the bound region table.  With
+// allow_alloc false (region batches, which may be captured into a graph) the
+// workspace must already be large enough.
+static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const PopArgs& pa, int32_t with_missing,
+                    int32_t stat_set, double* sums, double* key_pairs, hipStream_t s, bool allow_alloc) {
+  const bool miss = with_missing != 0;
+  SumsLauncher L = launcher_for(family, n_pops, miss, stat_set);
+  if (!L) {
+    set_error("band sums: no fused kernel for family %d with %d populations and stat set %d", family, n_pops,
+              stat_set);
+    return PGLD_EUNSUPPORTED;
+  }
+  const int n_stats = n_stats_of(family, n_pops, stat_set);
+  const int ns_total = n_stats + 1;
+  const size_t lds = sums_lds_bytes(pl, family, n_pops, miss, n_stats);
+  if (!lds) {
+    set_error("band sums: a tile touches %d keys; %d statistics x that many keys exceed LDS", pl->max_tile_slots,
+              n_stats);
+    return PGLD_EUNSUPPORTED;
+  }
+  if (allow_alloc) {
+    int rc = ensure_partials(pl, ns_total);
+    if (rc) return rc;
+  } else if (pl->partials_cap < (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total) {
+    set_error("band sums: workspace not bound for this statistic set");
+    return PGLD_EINVAL;
+  }
+  // k_band_reduce writes every key (zero where no tile touches it), so the
+  // outputs need no clearing and the call records no memset node when
+  // captured into a graph.
+  BandArgs ba{};
+  ba.tiles = reinterpret_cast<const BandTile*>(pl->d_tiles);
+  ba.pos = pl->d_pos;
+  ba.kc = pl->d_kc;
+  ba.edges = pl->d_edges;
+  ba.n_var = pl->n_var;
+  ba.max_dist = pl->max_dist;
+  ba.origin = pl->origin;
+  ba.width = pl->width;
+  ba.key_mode = pl->key_mode;
+  ba.n_keys = pl->n_keys_local;
+  ba.partials = pl->d_partials;
+  if (pl->n_tiles > 0) {
+    if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "band sums event");
+    L(pl, ba, pa, lds, s);
+    int rc = check_launch("k_band_sums");
+    if (rc) return rc;
+    if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[1], s), "band sums event");
+  }
+  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)pl->n_keys), dim3(256), 0, s, pl->d_partials, pl->d_key_off,
+                     pl->d_key_list, ns_total, n_stats, sums, key_pairs);
+  return check_launch("k_band_reduce");
+}
+
+}  // namespace pgld
+
+extern "C" {
+
 int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, const int8_t* const* planes,
                    const int32_t* const* marg, const int64_t* k_pad, const int32_t* n_samples, int32_t with_missing,
                    int32_t stat_set, double* sums, double* key_pairs, void* stream) {
   PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples && sums, "pgld_band_sums: null pointer");
+  PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_sums: region batches run through pgld_batch_sums");
   PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_sums: bad family");
   PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_band_sums: n_pops must be 1..4");
   PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_band_sums: bad stat_set");
   PopArgs pa;
   int rc = fill_pop_args(n_pops, planes, marg, k_pad, n_samples, pa);
   if (rc) return rc;
-  SumsLauncher L = nullptr;
-  const bool miss = with_missing != 0;
-  if (family == PGLD_FAMILY_HAP)
-    L = miss ? pick_launcher<PGLD_FAMILY_HAP, true>(n_pops, stat_set)
-             : pick_launcher<PGLD_FAMILY_HAP, false>(n_pops, stat_set);
-  else
-    L = miss ? pick_launcher<PGLD_FAMILY_GENO, true>(n_pops, stat_set)
-             : pick_launcher<PGLD_FAMILY_GENO, false>(n_pops, stat_set);
-  if (!L) {
-    set_error("pgld_band_sums: no fused kernel for family %d with %d populations and stat set %d", family, n_pops,
-              stat_set);
-    return PGLD_EUNSUPPORTED;
-  }
-  const int n_stats = n_stats_of(family, n_pops, stat_set);
-  const int ns_total = n_stats + 1;
-  const int nc = family == PGLD_FAMILY_HAP ? 4 : 9;
-  const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
-  const size_t stash_bytes = (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
-  const int n_planes_used = family == PGLD_FAMILY_HAP ? (with_missing ? 2 : 1) : (with_missing ? 3 : 2);
-  const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
-  // 160 KiB of LDS per workgroup on gfx950, minus the kernel's static arrays
-  if (red_bytes > (size_t)RED_LDS_BYTES ||
-      red_bytes + stash_bytes + op_bytes + STATIC_LDS_BYTES > 160 * 1024) {
-    set_error("pgld_band_sums: a tile touches %d keys; %d statistics x that many keys exceed LDS",
-              plan->max_tile_slots, n_stats);
-    return PGLD_EUNSUPPORTED;
-  }
-  hipStream_t s = as_stream(stream);
-  auto* pl = const_cast<pgld_band_plan*>(plan);
-  const size_t need = (size_t)std::max<int64_t>(1, plan->n_slots) * ns_total;
-  if (pl->partials_cap < need) {
-    if (pl->d_partials) PGLD_TRY_HIP(hipFree(pl->d_partials), "pgld_band_sums free");
-    pl->d_partials = nullptr;
-    pl->partials_cap = 0;
-    PGLD_TRY_HIP(hipMalloc((void**)&pl->d_partials, need * sizeof(double)), "pgld_band_sums workspace");
-    pl->partials_cap = need;
-  }
-  // k_band_reduce writes every key (zero where no tile touches it), so the
-  // outputs need no clearing and the call records no memset node when
-  // captured into a graph.
-  BandArgs ba{};
-  ba.tiles = reinterpret_cast<const BandTile*>(plan->d_tiles);
-  ba.pos = plan->d_pos;
-  ba.kc = plan->d_kc;
-  ba.edges = plan->d_edges;
-  ba.n_var = plan->n_var;
-  ba.max_dist = plan->max_dist;
-  ba.origin = plan->origin;
-  ba.width = plan->width;
-  ba.key_mode = plan->key_mode;
-  ba.n_keys = plan->n_keys;
-  ba.partials = pl->d_partials;
-  if (plan->n_tiles > 0) {
-    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "pgld_band_sums event");
-    L(plan, ba, pa, red_bytes + stash_bytes + op_bytes, s);
-    rc = check_launch("k_band_sums");
+  return run_sums(const_cast<pgld_band_plan*>(plan), family, n_pops, pa, with_missing, stat_set, sums, key_pairs,
+                  as_stream(stream), true);
+}
+
+int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                    const int32_t* n_samples) {
+  PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples, "pgld_batch_bind: null pointer");
+  PGLD_REQUIRE(plan->n_regions > 0, "pgld_batch_bind: not a region batch plan");
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_batch_bind: bad family");
+  PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_batch_bind: n_pops must be 1..4");
+  PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_batch_bind: bad stat_set");
+  PGLD_REQUIRE(launcher_for(family, n_pops, true, stat_set), "pgld_batch_bind: no fused kernel for this set");
+  const int R = plan->n_regions;
+  std::vector<RegionDesc> rd(R);
+  std::vector<double> wsum(R, 0.0);  // MFMA work per tile pair of the region (sum of k_pad)
+  for (int r = 0; r < R; ++r) {
+    int rc = fill_pop_args(n_pops, planes + (size_t)r * n_pops, marg + (size_t)r * n_pops, k_pad + (size_t)r * n_pops,
+                           n_samples + (size_t)r * n_pops, rd[r].pa);
     if (rc) return rc;
-    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[1], s), "pgld_band_sums event");
+    rd[r].n_var = plan->h_region_nvar[r];
+    rd[r].snp_base = plan->h_region_base[r];
+    rd[r].key_base = r * plan->n_keys_local;
+    for (int p = 0; p < n_pops; ++p) wsum[r] += (double)k_pad[(size_t)r * n_pops + p];
   }
-  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)plan->n_keys), dim3(256), 0, s, pl->d_partials, plan->d_key_off,
-                     plan->d_key_list, ns_total, n_stats, sums, key_pairs);
-  return check_launch("k_band_reduce");
+  // tile orders: cost-descending (longest first over the persistent grid) and natural
+  const int64_t T = plan->n_tiles;
+  std::vector<int32_t> order(2 * std::max<int64_t>(T, 1));
+  std::vector<double> cost(T);
+  for (int64_t t = 0; t < T; ++t) {
+    const BandTileHost& h = plan->h_tiles[t];
+    const double pairs = h.diag ? 0.5 * h.rows * h.cols : (double)h.rows * h.cols;
+    cost[t] = (double)h.rows * h.cols * wsum[h.region] + 64.0 * pairs;  // K loop + epilogue
+    order[t] = (int32_t)t;
+    order[T + t] = (int32_t)t;
+  }
+  std::stable_sort(order.begin(), order.begin() + T, [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+  if (!plan->d_regions) PGLD_TRY_HIP(hipMalloc(&plan->d_regions, (size_t)R * sizeof(RegionDesc)), "batch regions");
+  PGLD_TRY_HIP(hipMemcpy(plan->d_regions, rd.data(), (size_t)R * sizeof(RegionDesc), hipMemcpyHostToDevice),
+               "batch regions upload");
+  if (!plan->d_order)
+    PGLD_TRY_HIP(hipMalloc((void**)&plan->d_order, order.size() * sizeof(int32_t)), "batch order");
+  PGLD_TRY_HIP(hipMemcpy(plan->d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+               "batch order upload");
+  int rc = ensure_partials(plan, n_stats_of(family, n_pops, stat_set) + 1);
+  if (rc) return rc;
+  plan->bound_family = family;
+  plan->bound_pops = n_pops;
+  plan->bound_set = stat_set;
+  return PGLD_OK;
+}
+
+int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream) {
+  PGLD_REQUIRE(plan && sums, "pgld_batch_sums: null pointer");
+  PGLD_REQUIRE(plan->n_regions > 0 && plan->bound_family >= 0, "pgld_batch_sums: plan is not a bound region batch");
+  PopArgs none{};
+  return run_sums(plan, plan->bound_family, plan->bound_pops, none, with_missing, plan->bound_set, sums, key_pairs,
+                  as_stream(stream), false);
+}
+
+int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split) {
+  PGLD_REQUIRE(plan && plan->n_regions > 0, "pgld_batch_schedule: not a region batch plan");
+  PGLD_REQUIRE(grid >= 0 && (order == 0 || order == 1), "pgld_batch_schedule: bad grid or order");
+  plan->grid = grid;
+  plan->lpt = order == 0;
+  plan->xcd_split = xcd_split != 0;
+  return PGLD_OK;
+}
+
+// Autotuner: times every schedule candidate of the persistent launch on this
+// plan (one run each after one warm-up, HIP events) and keeps the fastest.
+// Results are cached per process by (family, populations, set, missing data,
+// tile count and mean K) so a later plan of the same shape class skips timing.
+int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream,
+                        int32_t* grid_out, int32_t* order_out, int32_t* xcd_out, float* best_ms) {
+  PGLD_REQUIRE(plan && sums, "pgld_batch_autotune: null pointer");
+  PGLD_REQUIRE(plan->n_regions > 0 && plan->bound_family >= 0, "pgld_batch_autotune: plan is not a bound batch");
+  hipStream_t s = as_stream(stream);
+  int dev = 0, cus = 256;
+  PGLD_TRY_HIP(hipGetDevice(&dev), "pgld_batch_autotune device");
+  PGLD_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "pgld_batch_autotune CUs");
+  double mean_k = 0.0;
+  for (int64_t r = 0; r < plan->n_regions; ++r) mean_k += (double)plan->h_region_nvar[r];
+  char key[160];
+  snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%d", plan->bound_family, plan->bound_pops, plan->bound_set,
+           with_missing != 0, (int)std::log2((double)std::max<int64_t>(1, plan->n_tiles)),
+           (int)std::log2(1.0 + mean_k / plan->n_regions), cus);
+  static std::mutex mu;
+  static std::map<std::string, std::array<int32_t, 3>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      plan->grid = it->second[0];
+      plan->lpt = it->second[1] == 0;
+      plan->xcd_split = it->second[2] != 0;
+      if (grid_out) *grid_out = it->second[0];
+      if (order_out) *order_out = it->second[1];
+      if (xcd_out) *xcd_out = it->second[2];
+      if (best_ms) *best_ms = -1.0f;
+      return PGLD_OK;
+    }
+  }
+  struct Cand { int32_t grid, order, xcd; };
+  std::vector<Cand> cands;
+  for (int32_t g : {0, cus, 2 * cus, 4 * cus}) {
+    cands.push_back({g, 0, 0});  // cost-descending, strided
+    cands.push_back({g, 1, 0});  // natural, strided
+    cands.push_back({g, 1, 1});  // natural, one contiguous eighth per XCD
+  }
+  hipEvent_t e0, e1;
+  PGLD_TRY_HIP(hipEventCreate(&e0), "autotune event");
+  PGLD_TRY_HIP(hipEventCreate(&e1), "autotune event");
+  float best = 1e30f;
+  Cand win = cands[0];
+  int rc = PGLD_OK;
+  for (size_t c = 0; c < cands.size() && !rc; ++c) {
+    plan->grid = cands[c].grid;
+    plan->lpt = cands[c].order == 0;
+    plan->xcd_split = cands[c].xcd != 0;
+    if (c == 0) rc = pgld_batch_sums(plan, with_missing, sums, key_pairs, stream);  // warm-up
+    if (!rc) rc = check_hip(hipEventRecord(e0, s), "autotune event");
+    if (!rc) rc = pgld_batch_sums(plan, with_missing, sums, key_pairs, stream);
+    if (!rc) rc = check_hip(hipEventRecord(e1, s), "autotune event");
+    if (!rc) rc = check_hip(hipEventSynchronize(e1), "autotune sync");
+    float ms = 0.0f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "autotune elapsed");
+    if (!rc && ms < best) {
+      best = ms;
+      win = cands[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  plan->grid = win.grid;
+  plan->lpt = win.order == 0;
+  plan->xcd_split = win.xcd != 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = {win.grid, win.order, win.xcd};
+  }
+  if (grid_out) *grid_out = win.grid;
+  if (order_out) *order_out = win.order;
+  if (xcd_out) *xcd_out = win.xcd;
+  if (best_ms) *best_ms = best;
+  return PGLD_OK;
+}
+
+// ---- HIP graphs --------------------------------------------------------------
+struct pgld_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int pgld_graph_begin(void* stream) {
+  PGLD_REQUIRE(stream, "pgld_graph_begin: capture needs a created stream, not the null stream");
+  PGLD_TRY_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal), "pgld_graph_begin");
+  return PGLD_OK;
+}
+
+int pgld_graph_end(void* stream, pgld_graph** out) {
+  PGLD_REQUIRE(stream && out, "pgld_graph_end: null argument");
+  *out = nullptr;
+  auto* g = new pgld_graph();
+  int rc = check_hip(hipStreamEndCapture(as_stream(stream), &g->graph), "pgld_graph_end capture");
+  if (!rc) rc = check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "pgld_graph_end instantiate");
+  if (rc) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return PGLD_OK;
+}
+
+int pgld_graph_launch(pgld_graph* g, void* stream) {
+  PGLD_REQUIRE(g && g->exec, "pgld_graph_launch: null graph");
+  PGLD_TRY_HIP(hipGraphLaunch(g->exec, as_stream(stream)), "pgld_graph_launch");
+  return PGLD_OK;
+}
+
+int pgld_graph_destroy(pgld_graph* g) {
+  if (!g) return PGLD_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return PGLD_OK;
 }
 
 int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes, const int32_t* marg,
                      int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
                      int32_t* counts, void* stream) {
   PGLD_REQUIRE(plan && planes && marg && counts, "pgld_band_counts: null pointer");
+  PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_counts: not defined for region batches");
   PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_counts: bad family");
   PopArgs pa;
   int rc = fill_pop_args(1, &planes, &marg, &k_pad, &n_samples, pa);

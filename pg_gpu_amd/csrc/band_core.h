@@ -5,8 +5,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "common.h"
 #include "../../include/pgld.h"
+
+struct BandTileHost {  // tile geometry kept on the host for scheduling
+  int32_t region, rows, cols, diag;
+};
 
 struct pgld_band_plan {
   int64_t n_var = 0, n_pairs = 0, n_tiles = 0, n_slots = 0;
@@ -23,6 +29,17 @@ struct pgld_band_plan {
   size_t partials_cap = 0;         // doubles
   bool timing = false;             // record events around each k_band_sums launch
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // region batches (n_regions > 0): keys are region * n_keys_local + key
+  int32_t n_regions = 0, n_keys_local = 0;
+  std::vector<int64_t> h_region_nvar, h_region_base;
+  std::vector<BandTileHost> h_tiles;   // host copy of the schedule, for ordering
+  void* d_regions = nullptr;           // RegionDesc[n_regions], written by bind
+  int32_t* d_order = nullptr;          // [cost-descending order | natural order] of the tiles
+  int32_t bound_family = -1, bound_pops = 0, bound_set = -1;
+  int32_t grid = 0;                    // persistent grid (0 = one workgroup per tile)
+  bool lpt = true;                     // cost-descending tile order (else natural)
+  bool xcd_split = false;              // each XCD walks a contiguous eighth of the order
+  int64_t partials_ns = 0;             // statistics + 1 the workspace was sized for
 };
 
 namespace pgld {
@@ -39,8 +56,9 @@ constexpr int MAX_LDS_EDGES = 130;  // bin edges kept in LDS when n_keys < 130
 constexpr int STATIC_LDS_BYTES = 4 * 64 * 8 + 64 * 4 + MAX_LDS_EDGES * 8;
 
 struct BandTile {
-  int32_t i0, j0, key_lo, n_slots;
+  int32_t i0, j0, key_lo, n_slots;  // i0 / j0 are SNP indices within the tile's region
   int64_t slot_base;
+  int32_t region, pad_;
 };
 
 struct PopArgs {
@@ -48,6 +66,15 @@ struct PopArgs {
   const int32_t* marg[4];
   int64_t k_pad[4];
   int32_t n_samples[4];
+};
+
+// One region of a region batch: its populations' planes, its SNP range in the
+// plan's concatenated coordinates, and its first key.
+struct RegionDesc {
+  PopArgs pa;
+  int64_t n_var;
+  int64_t snp_base;
+  int32_t key_base, pad_;
 };
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,10 @@ inline int check_hip(hipError_t e, const char* what) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// p[0..n) = v with a kernel on stream s (used instead of hipMemsetAsync so that
+// a captured graph holds only kernel nodes).
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+
 }  // namespace pgld
 
 #define PGLD_REQUIRE(cond, ...)          \

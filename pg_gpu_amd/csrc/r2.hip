@@ -168,6 +168,7 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                  int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                  double* r2_out, void* stream) {
   PGLD_REQUIRE(plan && planes && marg, "pgld_band_r2: null pointer");
+  PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
   PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= n_samples && n_samples >= 0,
                "pgld_band_r2: k_pad must be a multiple of 128 >= n_samples");
   PGLD_REQUIRE(!hist || n_hist >= 1, "pgld_band_r2: n_hist must be >= 1");

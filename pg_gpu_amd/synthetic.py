@@ -118,3 +118,40 @@ def dense_r2_panel(seed=1004, n_hap=10_000, n_var=40_000, span=4_000_000):
     hap = neutral_haplotypes(n_hap, n_var, rng)
     pos = uniform_positions(n_var, span, rng)
     return hap, pos
+
+
+def ragged_regions(seed=1005, n_regions=512, m_lo=500, m_hi=20_000, s_lo=50, s_hi=5000, n_pops=4, missing=0.03,
+                   span=1_000_000, device=None):
+    """Independent regions of uneven size: SNP counts uniform in [m_lo, m_hi],
+    haplotype counts uniform in [s_lo, s_hi] split into n_pops contiguous
+    groups (population p = rows [p S / n_pops, (p + 1) S / n_pops)), MCAR
+    missing data, positions uniform over `span`.
+
+    On the host (device=None) each SNP carries exactly k derived alleles with
+    P(k) ~ 1/k; on a device the matrices are drawn there (torch) with
+    per-SNP frequency k / S, the same spectrum in expectation, so that ~10 GB
+    of input need not cross PCIe.  Returns (matrices, positions, rows)."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(m_lo, m_hi + 1, size=n_regions)
+    S = rng.integers(s_lo, s_hi + 1, size=n_regions)
+    positions = [uniform_positions(int(mr), span, rng) for mr in m]
+    rows = [[list(range(p * int(s) // n_pops, (p + 1) * int(s) // n_pops)) for p in range(n_pops)] for s in S]
+    datas = []
+    if device is None:
+        for mr, s in zip(m, S):
+            datas.append(neutral_haplotypes(int(s), int(mr), rng, missing=missing))
+    else:
+        import torch
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+        for mr, s in zip(m, S):
+            s, mr = int(s), int(mr)
+            kv = np.arange(1, s)
+            w = 1.0 / kv
+            k = rng.choice(kv, size=mr, p=w / w.sum())
+            p = torch.as_tensor(k / s, dtype=torch.float32, device=device)
+            h = (torch.rand((s, mr), generator=gen, device=device) < p[None, :]).to(torch.int8)
+            if missing > 0:
+                h[torch.rand((s, mr), generator=gen, device=device) < missing] = -1
+            datas.append(h)
+    return datas, positions, rows

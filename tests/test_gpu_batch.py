@@ -1,0 +1,137 @@
+"""GPU parity of ragged region batches: one persistent launch over many
+independent regions against the oracle and against the single-band path
+region by region (bit-identical: same tiles, same per-key slot order), under
+every schedule the autotuner may pick, and through a captured HIP graph
+replayed several times."""
+import numpy as np
+import pytest
+
+from oracle import ld_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from pg_gpu_amd import band, batch  # noqa: E402
+from pg_gpu_amd import synthetic as SY  # noqa: E402
+from pg_gpu_amd import _lib as L  # noqa: E402
+
+EDGES = np.logspace(0, 6, 17)
+
+
+def np_(t):
+    return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+
+def self_specs(P):
+    return [(f"{t}", [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(P)]
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    datas, pos, rows = SY.ragged_regions(seed=31, n_regions=9, m_lo=40, m_hi=400, s_lo=24, s_hi=90, span=50_000)
+    dev = L.device()
+    d_dev = [torch.as_tensor(d, device=dev) for d in datas]
+    rows_dev = [[torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in rr] for rr in rows]
+    return datas, pos, rows, d_dev, rows_dev
+
+
+def oracle_region(d, pos, rows, P):
+    ii, jj = O.pairs_within_distance(pos, np.inf)
+    cl = [O.hap_counts(d, ii, jj, r) for r in rows]
+    st = O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl], self_specs(P))
+    keys = O.bin_index(pos[jj] - pos[ii], EDGES)
+    return O.keyed_sums(keys, st, 16), O.keyed_sums(keys, np.abs(st), 16), np.bincount(keys[keys >= 0],
+                                                                                         minlength=16)[:16]
+
+
+def test_batch_self_stats_vs_oracle_and_single_band(small_batch):
+    datas, pos, rows, d_dev, rows_dev = small_batch
+    print("[batch] 9 regions, 4 populations, self statistics", flush=True)
+    b = batch.RegionBatch(pos, 1e6, EDGES)
+    packed = batch.pack_regions(d_dev, rows_dev)
+    b.bind(packed, stat_set="self")
+    got, kp = b.sums(with_missing=True, want_pairs=True)
+    got, kp = np_(got), np_(kp)
+    assert got.shape == (9, 16, 12)
+    for r in range(9):
+        want, scale, pairs = oracle_region(datas[r], pos[r], rows[r], 4)
+        assert np.all(np.abs(got[r] - want) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300), f"region {r}"
+        np.testing.assert_array_equal(kp[r], pairs)
+        single = band.BandPlan(pos[r], 1e6, edges=EDGES)
+        ref = np_(single.sums(packed[r], stat_set="self", with_missing=True))
+        np.testing.assert_array_equal(got[r], ref)
+
+
+def test_batch_schedules_and_autotune_are_bit_identical(small_batch):
+    _, pos, _, d_dev, rows_dev = small_batch
+    b = batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(d_dev, rows_dev), stat_set="self")
+    ref = np_(b.sums(True))
+    for grid in (0, 8, 256, 1024):
+        for order in ("cost", "natural"):
+            for xcd in (False, True):
+                b.schedule(grid, order, xcd)
+                np.testing.assert_array_equal(np_(b.sums(True)), ref)
+    choice = b.autotune(True)
+    assert choice["ms"] is not None and choice["ms"] > 0 and choice["grid"] >= 0
+    np.testing.assert_array_equal(np_(b.sums(True)), ref)
+    again = batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(d_dev, rows_dev), stat_set="self")
+    assert again.autotune(True)["ms"] is None  # answered from the cache
+
+
+def test_batch_moments_two_pops_and_genotypes():
+    datas, pos, rows = SY.ragged_regions(seed=32, n_regions=5, m_lo=60, m_hi=300, s_lo=40, s_hi=120, n_pops=2,
+                                         span=30_000)
+    dev = L.device()
+    d_dev = [torch.as_tensor(d, device=dev) for d in datas]
+    b = batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(d_dev, rows), stat_set="moments")
+    got = np_(b.sums(True))
+    for r in range(5):
+        ii, jj = O.pairs_within_distance(pos[r], np.inf)
+        cl = [O.hap_counts(datas[r], ii, jj, rw) for rw in rows[r]]
+        st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+        keys = O.bin_index(pos[r][jj] - pos[r][ii], EDGES)
+        want, scale = O.keyed_sums(keys, st, 16), O.keyed_sums(keys, np.abs(st), 16)
+        assert np.all(np.abs(got[r] - want) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300)
+    rng = np.random.default_rng(5)
+    gl = [SY.hwe_genotypes(int(rng.integers(30, 70)), len(p), rng, missing=0.04) for p in pos]
+    gb = batch.RegionBatch(pos, 1e6, EDGES).bind(
+        batch.pack_regions([torch.as_tensor(g, device=dev) for g in gl], [[None]] * 5, family="geno"),
+        stat_set="moments")
+    gg = np_(gb.sums(True))
+    for r in range(5):
+        ii, jj = O.pairs_within_distance(pos[r], np.inf)
+        c, nv = O.geno_counts(gl[r], ii, jj, None)
+        st = O.multi_pop_stats_geno([c], [nv])
+        keys = O.bin_index(pos[r][jj] - pos[r][ii], EDGES)
+        want, scale = O.keyed_sums(keys, st, 16), O.keyed_sums(keys, np.abs(st), 16)
+        assert np.all(np.abs(gg[r] - want) <= 1e-11 * np.maximum(scale, 1e-300) + 1e-300)
+
+
+def test_graph_replays_match_eager(small_batch):
+    """Pack every region + the batch pass captured as one graph; each replay,
+    with eager work and synchronizations in between, matches the eager run."""
+    _, pos, _, d_dev, rows_dev = small_batch
+    packed = batch.pack_regions(d_dev, rows_dev)
+    b = batch.RegionBatch(pos, 1e6, EDGES).bind(packed, stat_set="self")
+    eager = np_(b.sums(True))
+    out = torch.full((9, 16, 12), float("nan"), dtype=torch.float64, device=L.device())
+    kp = torch.full((9, 16), -1.0, dtype=torch.float64, device=L.device())
+
+    def step():
+        batch.pack_regions(d_dev, rows_dev, out=packed, validate=False)
+        b.sums(True, out=out, key_pairs=kp)
+
+    g = batch.Graph().capture(step)
+    for rep in range(4):
+        out.fill_(float("nan"))
+        kp.fill_(-1.0)
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device=L.device()).sum().item()  # eager work between replays
+        g.launch()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(np_(out), eager, err_msg=f"replay {rep}")
+        assert np.all(np_(kp) >= 0)
+    g.close()
