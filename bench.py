@@ -11,7 +11,10 @@ Other workloads (--workload): genotype_bins (50,000 SNPs x 1,000 diploids, 5%
 missing, all pairs in 1 Mb, 16 log distance bins), plain_hap (2,000 SNPs x
 200 haplotypes, per-pair r^2/DD/Dz/pi2 within 20 kb through the plain
 per-pair kernels), dense_r2 (40,000 SNPs x 10,000 haplotypes, r^2 of all
-7.9998e8 pairs, 100-bin histogram + sparse list of pairs with r^2 > 0.2).
+7.9998e8 pairs, 100-bin histogram + sparse list of pairs with r^2 > 0.2),
+ragged_regions (512 independent regions of 500-20,000 SNPs x 50-5,000
+haplotypes in 4 populations, 3% missing, all pairs, per-region 16-bin
+DD/Dz/pi2 sums per population in one graph-captured persistent launch).
 
 Prints one JSON line: value is Gpair*samples/s for the whole job; roofline is
 for the dominant kernel; cpu_baseline is the NumPy oracle on a bounded sample
@@ -46,7 +49,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="windowed_hap",
-                    choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2"])
+                    choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2", "ragged_regions"])
     ap.add_argument("--n-var", type=int, default=None)
     ap.add_argument("--n-samples", "--n-hap", dest="n_samples", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=1)
@@ -437,6 +440,111 @@ def run_dense_r2(args, torch):
     return line
 
 
+def run_ragged_regions(args, torch):
+    from pg_gpu_amd import band, batch, synthetic
+    from pg_gpu_amd import _lib as L
+
+    n_regions = args.n_var or 512  # --n-var sets the region count here
+    macs = 4
+    edges = np.logspace(0, 6, 17)
+    dev = L.device()
+    log("[bench] generating workload on the device")
+    datas, pos, rows = synthetic.ragged_regions(n_regions=n_regions, device=dev)
+    rows_d = [[torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in rr] for rr in rows]
+    S = np.array([d.shape[0] for d in datas], dtype=np.int64)
+    packed = batch.pack_regions(datas, rows_d)  # validates once
+    with_missing = any(p.has_missing for pl in packed for p in pl)
+    t0 = time.perf_counter()
+    b = batch.RegionBatch(pos, 1e6, edges).bind(packed, stat_set="self")
+    plan_s = time.perf_counter() - t0
+    per_region_pairs = np.array([len(p) * (len(p) - 1) // 2 for p in pos], dtype=np.int64)
+    assert int(per_region_pairs.sum()) == b.n_pairs
+    pair_samples = int((per_region_pairs * S).sum())
+    log(f"[bench] regions={n_regions} pairs={b.n_pairs} tiles={b.n_tiles} pair_samples={pair_samples:.4e} "
+        f"plan+bind {plan_s:.1f} s")
+    choice = b.autotune(with_missing)
+    log(f"[bench] autotuned schedule {choice}")
+    out = torch.empty((n_regions, 16, 12), dtype=torch.float64, device=dev)
+    kp = torch.empty((n_regions, 16), dtype=torch.float64, device=dev)
+
+    def step_body():
+        batch.pack_regions(datas, rows_d, out=packed, validate=False)
+        b.sums(with_missing, out=out, key_pairs=kp)
+
+    graph = batch.Graph().capture(step_body)
+
+    def step():
+        graph.launch()
+        return out
+
+    ms_step, res = timed_steps(torch, step, args.steps, args.warmup)
+    res = res.clone()
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    log(f"[bench] graph step {ms_step:.3f} ms (2,048 packs + one persistent pass)")
+    b.set_timing(True)
+    kms = []
+    for _ in range(max(3, min(args.steps, 5))):
+        b.sums(with_missing, out=out, key_pairs=kp)
+        kms.append(b.kernel_ms())
+    b.set_timing(False)
+    k_ms = float(np.mean(kms))
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_batch_sums", macs)
+    log(f"[bench] k_batch_sums {k_ms:.3f} ms -> {roof['achieved']:.1f} int8 TOP/s")
+
+    # the same work region by region: one band plan and one fused launch each
+    plain_ms = check = None
+    if not args.no_plain:
+        plans = [band.BandPlan(p, 1e6, edges=edges) for p in pos]
+        outs = [None] * n_regions
+
+        def per_region():
+            for r in range(n_regions):
+                outs[r] = plans[r].sums(packed[r], stat_set="self", with_missing=with_missing)
+        per_region()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        per_region()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t1) * 1e3
+        ref = torch.stack(outs).cpu().numpy()
+        check = float(np.max(np.abs(res.cpu().numpy() - ref)))
+        log(f"[bench] region-by-region launches {plain_ms:.1f} ms, max |batch - per-region| {check:.3e}")
+        del plans
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ld_oracle as O
+        specs = [(t, [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(4)]
+        t2 = time.perf_counter()
+        n_ps = 0
+        for r in range(4):
+            h = datas[r].cpu().numpy()
+            ii, jj = band_pair_sample(pos[r], np.inf, args.cpu_pairs or 300_000, seed=r)
+            for c0 in range(0, len(ii), 20_000):
+                a, bb = ii[c0:c0 + 20_000], jj[c0:c0 + 20_000]
+                cl = [O.hap_counts(h, a, bb, rw) for rw in rows[r]]
+                st = O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl], specs)
+                O.keyed_sums(O.bin_index(pos[r][bb] - pos[r][a], edges), st, 16)
+            n_ps += len(ii) * int(S[r])
+        cpu_s = time.perf_counter() - t2
+        cpu = {"value": n_ps / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_pairs or 300_000} random pairs from each of regions 0-3, 4-population counts + "
+                         f"12 self statistics + bin sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
+        log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
+    line = base_line(args, value, ms_step,
+                     f"ragged persistent batch: {n_regions} independent regions, SNPs uniform 500-20,000, "
+                     "haplotypes uniform 50-5,000 in 4 populations, 3% missing, positions uniform over 1 Mb, all "
+                     "pairs; per-region 16 log-bin DD/Dz/pi2 sums per population (FP64) from one HIP graph "
+                     "(packing + one autotuned persistent launch)",
+                     {"regions": n_regions, "pairs": b.n_pairs, "pair_samples": pair_samples, "tiles": b.n_tiles,
+                      "schedule": choice})
+    line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
+                 "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
+                 "plain": "one band plan and one fused launch per region", "batch_vs_plain_max_abs": check,
+                 "cpu_baseline": cpu})
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -455,7 +563,7 @@ def main():
         log(f"[bench] copied {n} bytes 4 times")
         return
     run = {"windowed_hap": run_windowed_hap, "genotype_bins": run_genotype_bins, "plain_hap": run_plain_hap,
-           "dense_r2": run_dense_r2}
+           "dense_r2": run_dense_r2, "ragged_regions": run_ragged_regions}
     line = run[args.workload](args, torch)
     print(json.dumps(line), flush=True)
 
