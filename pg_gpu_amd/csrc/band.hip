@@ -262,8 +262,14 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
 // (positions are concatenated over regions) and its key offset.  The per-key
 // reduction runs over slots in schedule-independent order, so the result does
 // not depend on the grid or the tile order.
+// The persistent loop adds live state; genotype and 4-population instances
+// take the whole register file (their LDS use allows one workgroup per CU).
+constexpr int batch_min_waves(int family, int P, int set) {
+  return (family == PGLD_FAMILY_HAP && P <= 3) ? band_min_waves(family, P, set) : 1;
+}
+
 template <int FAMILY, bool MISSING, int P, int SET>
-__global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_batch_sums(
+__global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void k_batch_sums(
     BandArgs ba, const RegionDesc* __restrict__ regions, const int32_t* __restrict__ order, int64_t n_tiles,
     int32_t xcd_split) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -285,10 +291,10 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
     lb.pos = ba.pos + rd.snp_base;
     lb.kc = ba.kc + rd.snp_base;
     lb.n_var = rd.n_var;
-    const PopArgs pa = rd.pa;
     const int32_t key_base = rd.key_base;
     __syncthreads();  // the previous tile's LDS is no longer read
-    band_sums_tile<FAMILY, MISSING, P, SET>(lb, pa, tile, key_base, smem);
+    // rd.pa stays in global memory: the population loop indexes it at run time
+    band_sums_tile<FAMILY, MISSING, P, SET>(lb, rd.pa, tile, key_base, smem);
   }
 }
 

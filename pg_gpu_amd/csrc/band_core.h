@@ -108,8 +108,7 @@ struct WaveAcc {
 // it contains barriers).  Per 128-byte K chunk the 64 A rows and 64 B rows of
 // every plane are copied with full-line loads (8 lanes per 128-byte line) into
 // LDS rows padded to 144 bytes, which makes the MFMA-layout ds_read_b128 reads
-// (lane l -> row l % 32) bank-conflict free.  The next chunk's global loads
-// are issued before the MFMAs of the current chunk.
+// (lane l -> row l % 32) bank-conflict free.
 constexpr int OP_ROW = 144;  // padded LDS row stride (bytes)
 
 template <int N>
@@ -141,18 +140,27 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
     for (int b = 0; b < N; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
-  v4i pre[ITEMS];
+  // register stages: the loads of chunk c + PF are in flight while the MFMAs
+  // of chunk c run (PF = 2 where registers allow: up to 2 planes per side)
+  constexpr int PF = N <= 2 ? 2 : 1;
+  v4i pre0[ITEMS], pre1[PF == 2 ? ITEMS : 1];
 #pragma unroll
-  for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m]);
+  for (int m = 0; m < ITEMS; ++m) pre0[m] = *reinterpret_cast<const v4i*>(src[m]);
+  if constexpr (PF == 2) {
+    if (KSTEP < k_pad) {
+#pragma unroll
+      for (int m = 0; m < ITEMS; ++m) pre1[m] = *reinterpret_cast<const v4i*>(src[m] + KSTEP);
+    }
+  }
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
-  for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) {
+  auto step = [&](v4i* pre, int64_t k0) {
     __syncthreads();  // the previous chunk has been read by every wave
 #pragma unroll
     for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dst[m]) = pre[m];
     __syncthreads();
-    if (k0 + KSTEP < k_pad) {
+    if (k0 + PF * KSTEP < k_pad) {
 #pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + KSTEP);
+      for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + PF * KSTEP);
     }
     if (wave_live) {
 #pragma unroll
@@ -170,6 +178,14 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
             acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc.c[a][b], 0, 0, 0);
       }
     }
+  };
+  if constexpr (PF == 2) {
+    for (int64_t k0 = 0; k0 < k_pad; k0 += 2 * KSTEP) {
+      step(pre0, k0);
+      if (k0 + KSTEP < k_pad) step(pre1, k0 + KSTEP);
+    }
+  } else {
+    for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) step(pre0, k0);
   }
 }
 
