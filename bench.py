@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--method", default="mfma", choices=["mfma", "popcount"],
+                    help="tile products of the dense r^2 workload: int8 MFMA or AND + popcount on bit planes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 1 GiB (the HBM-counter reference of the traffic figure)")
     return ap.parse_args()
@@ -398,8 +400,8 @@ def run_dense_r2(args, torch):
     cap = 1 << 22
 
     def step():
-        band.pack_population(hap_d, None, "hap", validate=False, out=pk)
-        return plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False)
+        p2 = band.pack_population(hap_d, None, "hap", validate=False, out=pk)
+        return plan.r2(p2, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False, method=args.method)
 
     ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
     value = pair_samples / (ms_step * 1e-3) / 1e9
@@ -408,12 +410,15 @@ def run_dense_r2(args, torch):
     plan.set_timing(True)
     kms = []
     for _ in range(max(3, args.steps)):
-        plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False)
+        plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False, method=args.method)
         kms.append(plan.kernel_ms())
     plan.set_timing(False)
     k_ms = float(np.mean(kms))
     roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_band_r2", macs)
-    log(f"[bench] k_band_r2 {k_ms:.3f} ms -> {roof['achieved']:.1f} int8 TOP/s")
+    if args.method == "popcount":  # VALU variant: report against the int8 peak all the same, say so
+        roof["ops"] = "pair*sample MACs x 2 as AND + popcount on the VALU (bit planes), against the int8 MFMA peak"
+        roof["kernel"] = "k_band_r2<popcount>"
+    log(f"[bench] k_band_r2 ({args.method}) {k_ms:.3f} ms -> {roof['achieved']:.1f} int8-equivalent TOP/s")
     cpu = None
     if not args.no_cpu:
         from oracle import ld_oracle as O
@@ -435,7 +440,7 @@ def run_dense_r2(args, torch):
                      "r^2 of every pair from exact int32 counts, 100-bin histogram on [0, 1] (NaN pairs counted "
                      "apart) + sparse (i, j, r^2) list for r^2 > 0.2",
                      {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
-                      "n_sparse": out["n_sparse"]})
+                      "n_sparse": out["n_sparse"], "method": args.method})
     line.update({"roofline": roof, "cpu_baseline": cpu})
     return line
 

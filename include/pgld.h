@@ -220,6 +220,25 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                  int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                  double* r2_out, void* stream);
 
+/* ---- bit-packed popcount variant of the tile products ------------------ */
+
+/* Bit planes from int8 0/1 planes (n_planes x n_var x k_pad, as written by
+ * pgld_pack_planes): row r of bits holds 32 * ceil(k_pad / 1024) uint32 words,
+ * word w = samples 32w .. 32w + 31 (bit b = sample 32w + b), zero padded. */
+int pgld_pack_bits(const int8_t* planes, int64_t n_planes, int64_t n_var, int64_t k_pad, uint32_t* bits,
+                   void* stream);
+/* pgld_band_counts / pgld_band_r2 with the products formed by AND + popcount
+ * on the VALU from bit planes instead of int8 MFMA (same outputs, bit for
+ * bit); k_pad is the int8 planes' padding the bit rows derive from.  Kept for
+ * the MFMA-versus-popcount comparison of the tile counts. */
+int pgld_band_counts_bits(const pgld_band_plan* plan, int32_t family, const uint32_t* bits, const int32_t* marg,
+                          int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
+                          int32_t* counts, void* stream);
+int pgld_band_r2_bits(const pgld_band_plan* plan, const uint32_t* bits, const int32_t* marg, int64_t k_pad,
+                      int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
+                      int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
+                      uint64_t* n_sparse, double* r2_out, void* stream);
+
 /* ---- ragged region batches: many independent regions, one launch ------ */
 
 /* A band plan over n_regions independent regions (no pair crosses regions).

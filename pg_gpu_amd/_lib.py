@@ -43,6 +43,10 @@ _SIGNATURES = {
     "pgld_band_plan_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], C.c_int),
     "pgld_band_plan_destroy": ([_vp], C.c_int),
     "pgld_band_plan_set_timing": ([_vp, _i32], C.c_int),
+    "pgld_pack_bits": ([_vp, _i64, _i64, _i64, _vp, _vp], C.c_int),
+    "pgld_band_counts_bits": ([_vp, _i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp], C.c_int),
+    "pgld_band_r2_bits": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+                          C.c_int),
     "pgld_batch_plan_create": ([_i32, _vp, _vp, _f64, _i32, _vp, _vp, _i32, _f64, _f64, C.POINTER(_vp)], C.c_int),
     "pgld_batch_bind": ([_vp, _i32, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32)], C.c_int),
     "pgld_batch_sums": ([_vp, _i32, _vp, _vp, _vp], C.c_int),

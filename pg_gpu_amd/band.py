@@ -33,6 +33,20 @@ class PackedPop:
     n_samples: int
     has_missing: object  # bool, or None when the flags were not read back
     flags: object = None  # int32 tensor [any missing, any out of domain]
+    bits: object = None   # uint32-as-int32 tensor of bit planes (popcount variant), made on demand
+    bits_fresh: bool = False
+
+    def bit_planes(self):
+        """Bit planes of this population (32 samples per word), rebuilt from
+        the int8 planes when they were repacked since."""
+        n_planes, n_var, kp = self.planes.shape
+        if self.bits is None:
+            self.bits = L.empty((n_planes, n_var, 32 * ((kp + 1023) // 1024)), np.int32)
+            self.bits_fresh = False
+        if not self.bits_fresh:
+            L.call("pgld_pack_bits", L.ptr(self.planes), n_planes, n_var, kp, L.ptr(self.bits), L.stream_handle())
+            self.bits_fresh = True
+        return self.bits
 
 
 def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, out=None):
@@ -67,13 +81,14 @@ def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, ou
         flags = L.empty((2,), np.int32)
     L.call("pgld_pack_planes", fam, L.ptr(g), n_rows, n_var, L.ptr(rows_t), n_sel, kp, L.ptr(planes),
            L.ptr(marg), L.ptr(flags), L.stream_handle())
+    bits = out.bits if (out is not None and out.planes is planes) else None
     if not validate:
-        return PackedPop(fam, planes, marg, kp, n_sel, None, flags)
+        return PackedPop(fam, planes, marg, kp, n_sel, None, flags, bits)
     f = flags.cpu().numpy()
     if f[1]:
         raise ValueError("input values outside {-1, 0, 1}" if fam == L.FAMILY_HAP
                          else "genotype values outside {-1, 0, 1, 2}")
-    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]), flags)
+    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]), flags, bits)
 
 
 class BandPlan:
@@ -164,7 +179,7 @@ class BandPlan:
         return (sums, kp) if want_pairs else sums
 
     def r2(self, pop, n_hist=100, thresh=0.2, sparse_cap=1 << 20, per_pair=False, histogram=True,
-           with_missing=None):
+           with_missing=None, method="mfma"):
         """r^2 of every band pair of one haplotype population through the MFMA
         tiles (pgld_band_r2).  Returns a dict with
           hist      int64 (n_keys, n_hist) pairs per r^2 bin [b/n, (b+1)/n),
@@ -187,7 +202,11 @@ class BandPlan:
         sp_r = L.empty((max(cap, 1),), np.float32)
         n_sp = L.empty((1,), np.int64)
         r2 = L.empty((self.n_pairs,), np.float64) if per_pair else None
-        L.call("pgld_band_r2", self._h, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad, pop.n_samples,
+        if method not in ("mfma", "popcount"):
+            raise ValueError("method is 'mfma' or 'popcount'")
+        src = pop.planes if method == "mfma" else pop.bit_planes()
+        L.call("pgld_band_r2" if method == "mfma" else "pgld_band_r2_bits", self._h, L.ptr(src), L.ptr(pop.marg),
+               pop.k_pad, pop.n_samples,
                int(bool(with_missing)), int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),
                L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.stream_handle())
         n_total = int(n_sp.cpu().item())
@@ -204,13 +223,18 @@ class BandPlan:
             out["r2"] = r2
         return out
 
-    def counts(self, pop, with_missing=None):
-        """Per-pair counts of one population through the MFMA path, in the
-        band's (i, j) pair order: int32 (n_pairs, 4) or (n_pairs, 9)."""
+    def counts(self, pop, with_missing=None, method="mfma"):
+        """Per-pair counts of one population through the tiled path (int8
+        MFMA, or AND + popcount on bit planes), in the band's (i, j) pair
+        order: int32 (n_pairs, 4) or (n_pairs, 9)."""
         if with_missing is None:
             with_missing = pop.has_missing
+        if method not in ("mfma", "popcount"):
+            raise ValueError("method is 'mfma' or 'popcount'")
         nc = 4 if pop.family == L.FAMILY_HAP else 9
         out = L.empty((self.n_pairs, nc), np.int32)
-        L.call("pgld_band_counts", self._h, pop.family, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad,
-               pop.n_samples, int(bool(with_missing)), None, L.ptr(out), L.stream_handle())
+        src = pop.planes if method == "mfma" else pop.bit_planes()
+        L.call("pgld_band_counts" if method == "mfma" else "pgld_band_counts_bits", self._h, pop.family, L.ptr(src),
+               L.ptr(pop.marg), pop.k_pad, pop.n_samples, int(bool(with_missing)), None, L.ptr(out),
+               L.stream_handle())
         return out

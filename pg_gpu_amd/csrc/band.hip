@@ -299,7 +299,7 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
 }
 
 // Counts mode: one population, per-pair counts in the band's (i, j) order.
-template <int FAMILY, bool MISSING>
+template <int FAMILY, bool MISSING, bool POPC>
 __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopArgs pa) {
   using PL = Planes<FAMILY, MISSING>;
   constexpr int NC = PL::NC;
@@ -310,8 +310,8 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopAr
   const int ri = tile.i0 + wr, rj = tile.j0 + wc;
   const bool wave_live = (int64_t)rj < ba.n_var && (int64_t)ri < ba.n_var && rj + 31 > ri;
   WaveAcc<PL::N> acc;
-  block_products<FAMILY, MISSING>(pa.planes[0], ba.n_var, pa.k_pad[0], tile.i0, tile.j0, wr, wc, wave_live, smem,
-                                  acc);
+  block_products<FAMILY, MISSING, POPC>(pa.planes[0], ba.n_var, pa.k_pad[0], tile.i0, tile.j0, wr, wc, wave_live,
+                                        smem, acc);
   if (!wave_live) return;
   const int64_t j = rj + (lane & 31);
   if (j >= ba.n_var) return;
@@ -470,6 +470,27 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
     if (mneg) atomicOr(&flags[0], 1);
     if (mbad) atomicOr(&flags[1], 1);
   }
+}
+
+// int8 0/1 planes -> bit planes: word w of a row holds samples 32w .. 32w+31
+// (bit b = sample 32w + b); rows padded with zero words to bit_row_bytes.
+__global__ void k_pack_bits(const int8_t* __restrict__ planes, int64_t n_rows, int64_t k_pad,
+                            uint32_t* __restrict__ bits, int64_t words) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows * words) return;
+  const int64_t row = t / words, w = t - row * words;
+  uint32_t out = 0;
+  if (32 * w < k_pad) {
+    const v4i* src = reinterpret_cast<const v4i*>(planes + row * k_pad + 32 * w);
+    const v4i lo = src[0], hi = src[1];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const uint32_t x = (uint32_t)(d < 4 ? lo[d] : hi[d - 4]);  // four 0/1 bytes
+      const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+      out |= nib << (4 * d);
+    }
+  }
+  bits[t] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -1117,7 +1138,7 @@ int pgld_graph_destroy(pgld_graph* g) {
   return PGLD_OK;
 }
 
-int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes, const int32_t* marg,
+static int band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes, bool popc, const int32_t* marg,
                      int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
                      int32_t* counts, void* stream) {
   PGLD_REQUIRE(plan && planes && marg && counts, "pgld_band_counts: null pointer");
@@ -1139,14 +1160,40 @@ int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* p
   const bool miss = with_missing != 0;
   const int npl = family == PGLD_FAMILY_HAP ? (miss ? 2 : 1) : (miss ? 3 : 2);
   const size_t lds = (size_t)2 * npl * BT * OP_ROW;
+#define PGLD_COUNTS(F, M, B) hipLaunchKernelGGL((k_band_counts<F, M, B>), g, b, lds, s, ba, pa)
   if (family == PGLD_FAMILY_HAP) {
-    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, true>), g, b, lds, s, ba, pa);
-    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_HAP, false>), g, b, lds, s, ba, pa);
+    if (popc) { if (miss) PGLD_COUNTS(PGLD_FAMILY_HAP, true, true); else PGLD_COUNTS(PGLD_FAMILY_HAP, false, true); }
+    else { if (miss) PGLD_COUNTS(PGLD_FAMILY_HAP, true, false); else PGLD_COUNTS(PGLD_FAMILY_HAP, false, false); }
   } else {
-    if (miss) hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, true>), g, b, lds, s, ba, pa);
-    else hipLaunchKernelGGL((k_band_counts<PGLD_FAMILY_GENO, false>), g, b, lds, s, ba, pa);
+    if (popc) { if (miss) PGLD_COUNTS(PGLD_FAMILY_GENO, true, true); else PGLD_COUNTS(PGLD_FAMILY_GENO, false, true); }
+    else { if (miss) PGLD_COUNTS(PGLD_FAMILY_GENO, true, false); else PGLD_COUNTS(PGLD_FAMILY_GENO, false, false); }
   }
+#undef PGLD_COUNTS
   return check_launch("k_band_counts");
+}
+
+int pgld_band_counts(const pgld_band_plan* plan, int32_t family, const int8_t* planes, const int32_t* marg,
+                     int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
+                     int32_t* counts, void* stream) {
+  return band_counts(plan, family, planes, false, marg, k_pad, n_samples, with_missing, pair_base, counts, stream);
+}
+
+int pgld_band_counts_bits(const pgld_band_plan* plan, int32_t family, const uint32_t* bits, const int32_t* marg,
+                          int64_t k_pad, int32_t n_samples, int32_t with_missing, const int64_t* pair_base,
+                          int32_t* counts, void* stream) {
+  return band_counts(plan, family, reinterpret_cast<const int8_t*>(bits), true, marg, k_pad, n_samples, with_missing,
+                     pair_base, counts, stream);
+}
+
+int pgld_pack_bits(const int8_t* planes, int64_t n_planes, int64_t n_var, int64_t k_pad, uint32_t* bits,
+                   void* stream) {
+  PGLD_REQUIRE(planes && bits, "pgld_pack_bits: null pointer");
+  PGLD_REQUIRE(n_planes >= 1 && n_var >= 0 && k_pad % 128 == 0, "pgld_pack_bits: bad shape");
+  const int64_t words = bit_row_bytes(k_pad) / 4, n = n_planes * n_var * words;
+  if (n == 0) return PGLD_OK;
+  hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, as_stream(stream), planes,
+                     n_planes * n_var, k_pad, bits, words);
+  return check_launch("k_pack_bits");
 }
 
 }  // extern "C"

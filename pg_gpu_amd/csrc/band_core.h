@@ -114,11 +114,21 @@ constexpr int OP_ROW = 144;  // padded LDS row stride (bytes)
 template <int N>
 __host__ __device__ constexpr int opbuf_bytes() { return 2 * N * BT * OP_ROW; }
 
-template <int FAMILY, bool MISSING>
+// Bit-plane rows for the popcount variant: 32 samples per uint32 word, rows
+// padded to a multiple of 128 bytes (1,024 samples) with zero words.
+__host__ __device__ inline int64_t bit_row_bytes(int64_t k_pad) { return 128 * ((k_pad + 1023) / 1024); }
+
+// POPC = false: int8 0/1 planes (row_bytes = k_pad), products on the int8
+// MFMA.  POPC = true: bit planes (row_bytes = bit_row_bytes(k_pad)), products
+// as AND + popcount on the VALU, accumulated into the same accumulator
+// layout as the MFMA (lane l -> column l % 32, rows acc_row(r, l)), so every
+// epilogue works with either.
+template <int FAMILY, bool MISSING, bool POPC = false>
 __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
                                                int i0, int j0, int wr, int wc, bool wave_live,
                                                unsigned char* __restrict__ opbuf,
                                                WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
+  if constexpr (POPC) k_pad = bit_row_bytes(k_pad);  // from here on: bytes per operand row
   using PL = Planes<FAMILY, MISSING>;
   constexpr int N = PL::N;
   constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
@@ -162,7 +172,35 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
 #pragma unroll
       for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + PF * KSTEP);
     }
-    if (wave_live) {
+    if (wave_live && POPC) {
+      // 8 groups of four 32-sample words per 128-byte chunk; the A rows of a
+      // half-wave are shared (LDS broadcast), B is this lane's column
+      const int half = lane >> 5;
+#pragma unroll 2
+      for (int q = 0; q < 8; ++q) {
+        v4i B[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b)
+          B[b] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + b) * BT + rb) * OP_ROW + 16 * q);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = wr + 8 * (r >> 2) + 4 * half + (r & 3);
+          v4i A[N];
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            A[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + row) * OP_ROW + 16 * q);
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+#pragma unroll
+            for (int b = 0; b < N; ++b) {
+              int c = acc.c[a][b][r];
+#pragma unroll
+              for (int w = 0; w < 4; ++w) c += __popc((unsigned)(A[a][w] & B[b][w]));
+              acc.c[a][b][r] = c;
+            }
+        }
+      }
+    } else if (wave_live) {
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
         v4i A[N], B[N];

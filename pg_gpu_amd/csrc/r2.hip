@@ -41,7 +41,7 @@ struct R2Args {
 
 constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
 
-template <bool MISSING>
+template <bool MISSING, bool POPC>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const int8_t* __restrict__ planes,
                                                              const int32_t* __restrict__ marg, int64_t k_pad,
                                                              int32_t S) {
@@ -75,7 +75,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   const bool wave_live = (int64_t)rj < ra.n_var && rj + 31 > ri && (int64_t)ri < ra.n_var &&
                          s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ra.max_dist;
   WaveAcc<PL::N> acc;
-  block_products<PGLD_FAMILY_HAP, MISSING>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem, acc);
+  block_products<PGLD_FAMILY_HAP, MISSING, POPC>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
+                                                 acc);
 
   if (wave_live) {
     const int jl = wc + (lane & 31);
@@ -163,10 +164,10 @@ using namespace pgld;
 
 extern "C" {
 
-int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
-                 int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
-                 int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
-                 double* r2_out, void* stream) {
+static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, const int32_t* marg, int64_t k_pad,
+                   int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
+                   int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
+                   double* r2_out, void* stream) {
   PGLD_REQUIRE(plan && planes && marg, "pgld_band_r2: null pointer");
   PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
   PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= n_samples && n_samples >= 0,
@@ -214,12 +215,33 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
   const size_t lds = miss ? (size_t)opbuf_bytes<2>() : (size_t)opbuf_bytes<1>();
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[0], s), "pgld_band_r2 event");
-  if (miss) hipLaunchKernelGGL(k_band_r2<true>, g, b, lds, s, ra, planes, marg, k_pad, n_samples);
-  else hipLaunchKernelGGL(k_band_r2<false>, g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+  if (popc) {
+    if (miss) hipLaunchKernelGGL((k_band_r2<true, true>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+    else hipLaunchKernelGGL((k_band_r2<false, true>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+  } else {
+    if (miss) hipLaunchKernelGGL((k_band_r2<true, false>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+    else hipLaunchKernelGGL((k_band_r2<false, false>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+  }
   rc = check_launch("k_band_r2");
   if (rc) return rc;
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[1], s), "pgld_band_r2 event");
   return PGLD_OK;
+}
+
+int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
+                 int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
+                 int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
+                 double* r2_out, void* stream) {
+  return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, n_hist, thresh, hist, sparse_i, sparse_j,
+                 sparse_r2, sparse_cap, n_sparse, r2_out, stream);
+}
+
+int pgld_band_r2_bits(const pgld_band_plan* plan, const uint32_t* bits, const int32_t* marg, int64_t k_pad,
+                      int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
+                      int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
+                      double* r2_out, void* stream) {
+  return band_r2(plan, reinterpret_cast<const int8_t*>(bits), true, marg, k_pad, n_samples, with_missing, n_hist,
+                 thresh, hist, sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, r2_out, stream);
 }
 
 }  // extern "C"

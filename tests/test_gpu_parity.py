@@ -336,3 +336,34 @@ def test_band_r2_all_pairs_bins_and_capacity():
     assert cut["n_sparse"] == n_hit and len(cut["sparse_i"]) == 7
     hits = set(zip(ii[r2 > 0.05].tolist(), jj[r2 > 0.05].tolist()))
     assert set(zip(np_(cut["sparse_i"]).tolist(), np_(cut["sparse_j"]).tolist())) <= hits
+
+
+# ---------------------------------------------------------------------------
+# bit-packed popcount variant: the same counts and r^2 as the MFMA tiles
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("family,missing", [("hap", True), ("hap", False), ("geno", True), ("geno", False)])
+def test_popcount_counts_equal_mfma_and_oracle(family, missing):
+    if family == "hap":
+        d, pos = hap_data(95, 1100, 300, 0.03 if missing else 0.0)  # > 1,024 samples: two bit-row chunks
+    else:
+        d, pos = geno_data(96, 700, 300, 0.04 if missing else 0.0)
+    plan = band.BandPlan(pos, 4000.0, window=1e9, n_keys=1)
+    pk = band.pack_population(d, None, family)
+    a = np_(plan.counts(pk, method="mfma"))
+    b = np_(plan.counts(pk, method="popcount"))
+    np.testing.assert_array_equal(a, b)
+    ii, jj = O.pairs_within_distance(pos, 4000.0)
+    exp = (O.hap_counts if family == "hap" else O.geno_counts)(d, ii, jj, None)[0]
+    np.testing.assert_array_equal(b, exp)
+
+
+@pytest.mark.parametrize("missing", [False, True])
+def test_popcount_r2_equals_mfma(missing):
+    d, pos = hap_data(97, 300, 600, 0.02 if missing else 0.0)
+    plan = band.BandPlan(pos, np.inf, edges=np.array([0.0, 2000.0, 10000.0, 1e9]))
+    pk = band.pack_population(d, None, "hap")
+    m = plan.r2(pk, n_hist=40, thresh=0.05, per_pair=True, method="mfma")
+    p = plan.r2(pk, n_hist=40, thresh=0.05, per_pair=True, method="popcount")
+    for k in ("hist", "nan_pairs", "sparse_i", "sparse_j", "sparse_r2", "r2"):
+        np.testing.assert_array_equal(np_(m[k]), np_(p[k]), err_msg=k)
+    assert m["n_sparse"] == p["n_sparse"]
