@@ -358,11 +358,27 @@ __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // Plane packing: (rows x variants) int8 -> [plane][variant][k_pad] 0/1 bytes
 // ---------------------------------------------------------------------------
-// Tile of 128 samples x 128 variants per block: every row segment is read as
-// one full 128-byte line (32 lanes x dword) when the row stride allows,
-// transposed through LDS, and each (plane, variant) gets one full 128-byte
-// line of sample bytes.
+// Tile of 128 samples x 128 variants per block.  Phase 1: every row segment is
+// read as one full 128-byte line (32 lanes x dword) into LDS.  Phase 2: each
+// thread takes 16 samples x 4 variants, turns four 4 x 4 byte blocks into
+// variant-major dwords with byte permutes, derives the plane bytes and the
+// domain / missing flags four samples at a time (SWAR on dwords) and writes
+// 16 sample bytes per (plane, variant): 8 lanes fill one 128-byte line.
 constexpr int PK_S = 128, PK_V = 128;
+constexpr int PK_ROW = PK_V / 4 + 1;  // LDS row stride in dwords
+
+// 4 x 4 byte transpose: r[i] holds bytes (v0..v3) of sample i; on return
+// c[j] holds bytes (s0..s3) of variant j.
+__device__ __forceinline__ void transpose4x4(const uint32_t r[4], uint32_t c[4]) {
+  const uint32_t t0 = __builtin_amdgcn_perm(r[1], r[0], 0x05010400u);
+  const uint32_t t1 = __builtin_amdgcn_perm(r[1], r[0], 0x07030602u);
+  const uint32_t t2 = __builtin_amdgcn_perm(r[3], r[2], 0x05010400u);
+  const uint32_t t3 = __builtin_amdgcn_perm(r[3], r[2], 0x07030602u);
+  c[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+  c[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+  c[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+  c[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
 
 template <int FAMILY, bool WORDS>
 __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
@@ -370,8 +386,7 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
                                               int8_t* __restrict__ planes, int32_t* __restrict__ marg,
                                               int32_t* __restrict__ flags) {
   constexpr int NPL = FAMILY == PGLD_FAMILY_HAP ? 2 : 3;
-  constexpr int maxv = FAMILY == PGLD_FAMILY_HAP ? 1 : 2;
-  __shared__ uint32_t tile[PK_S][PK_V / 4 + 1];  // [sample][variant quad], padded
+  __shared__ uint32_t tile[PK_S * PK_ROW];  // [sample][variant quad], padded
   __shared__ int msum[NPL][PK_V];
   __shared__ int64_t srow[PK_S];
   const int tid = threadIdx.x;
@@ -380,83 +395,96 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   for (int t = tid; t < NPL * PK_V; t += 256) msum[t / PK_V][t % PK_V] = 0;
   if (tid < PK_S) srow[tid] = k0 + tid < n_sel ? (int64_t)(rows ? rows[k0 + tid] : k0 + tid) * n_var : -1;
   __syncthreads();
-  int any_neg = 0, any_bad = 0;
-  // 32 threads per row (one dword = 4 variants each), 8 rows per pass; all 16
-  // loads of a thread are independent and issued back to back
-  const int q = tid & 31, rr = tid >> 5;
-  uint32_t words_in[PK_S / 8];
+  // phase 1: 32 threads per row (one dword = 4 variants each), 8 rows per
+  // pass; all 16 loads of a thread are independent and issued back to back
+  {
+    const int q = tid & 31, rr = tid >> 5;
+    uint32_t words_in[PK_S / 8];
 #pragma unroll
-  for (int u = 0; u < PK_S / 8; ++u) {
-    const int64_t base = srow[rr + 8 * u];
-    uint32_t word = 0xffffffffu;  // padding samples read as missing
-    if (base >= 0) {
-      const int64_t at = base + v0 + 4 * q;
-      if (WORDS && v0 + 4 * q + 3 < n_var) {
-        word = *reinterpret_cast<const uint32_t*>(data + at);
-      } else {
+    for (int u = 0; u < PK_S / 8; ++u) {
+      const int64_t base = srow[rr + 8 * u];
+      uint32_t word = 0xffffffffu;  // padding samples and variants read as missing
+      if (base >= 0) {
+        const int64_t at = base + v0 + 4 * q;
+        if (WORDS && v0 + 4 * q + 3 < n_var) {
+          word = *reinterpret_cast<const uint32_t*>(data + at);
+        } else {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const uint32_t byte = v0 + 4 * q + b < n_var ? (uint8_t)data[at + b] : 0xffu;
-          word = (word & ~(0xffu << (8 * b))) | (byte << (8 * b));
+          for (int b = 0; b < 4; ++b) {
+            const uint32_t byte = v0 + 4 * q + b < n_var ? (uint8_t)data[at + b] : 0xffu;
+            word = (word & ~(0xffu << (8 * b))) | (byte << (8 * b));
+          }
         }
       }
+      words_in[u] = word;
     }
-    words_in[u] = word;
-  }
 #pragma unroll
-  for (int u = 0; u < PK_S / 8; ++u) {
-    const uint32_t word = words_in[u];
-    if (srow[rr + 8 * u] >= 0) {
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        if (v0 + 4 * q + b < n_var) {
-          const int x = (int8_t)(word >> (8 * b));
-          any_neg |= x < 0;
-          any_bad |= (x < -1) | (x > maxv);
-        }
-      }
-    }
-    tile[rr + 8 * u][q] = word;
+    for (int u = 0; u < PK_S / 8; ++u) tile[(rr + 8 * u) * PK_ROW + q] = words_in[u];
   }
   __syncthreads();
-  // 8 lanes per (variant): each builds 16 sample bytes of every plane
-  for (int task = tid; task < PK_V * 8; task += 256) {
-    const int v = task >> 3, ch = (task & 7) * 16;
+  // phase 2: thread -> (16-sample group sg, variant quad vq); the 8 lanes of
+  // one variant quad cover its 128 samples
+  const int sg = tid & 7, vq = tid >> 3;
+  uint32_t cols[4][4];  // [variant j][sample dword i]: bytes = samples 16 sg + 4 i .. + 3
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t r[4], c[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r[e] = tile[(16 * sg + 4 * i + e) * PK_ROW + vq];
+    transpose4x4(r, c);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cols[j][i] = c[j];
+  }
+  // samples of this group that exist, as a byte mask per sample dword
+  const int64_t s_first = k0 + 16 * sg;
+  uint32_t smask[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) m |= (s_first + 4 * i + b < n_sel ? 0xffu : 0u) << (8 * b);
+    smask[i] = m;
+  }
+  constexpr uint32_t ONES = 0x01010101u;
+  int any_neg = 0, any_bad = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int v = 4 * vq + j;
     if (v0 + v >= n_var) continue;
-    uint32_t words[NPL][4];
+    uint32_t pw[NPL][4];
     int cnt[NPL];
 #pragma unroll
-    for (int p = 0; p < NPL; ++p) {
-      cnt[p] = 0;
+    for (int p = 0; p < NPL; ++p) cnt[p] = 0;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) words[p][t] = 0;
-    }
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      const int x = (int8_t)(tile[ch + b][v >> 2] >> (8 * (v & 3)));
-      uint32_t bits[NPL];
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t w = cols[j][i];
+      const uint32_t neg = (w >> 7) & ONES;    // 1 per negative byte
+      const uint32_t val = neg ^ ONES;         // 1 per byte >= 0
+      const uint32_t nonneg_m = val * 0xffu, neg_m = neg * 0xffu;
+      uint32_t bad = (~w & 0x7f7f7f7fu) & neg_m;  // negative but not -1
       if constexpr (FAMILY == PGLD_FAMILY_HAP) {
-        bits[0] = x == 1;
-        bits[1] = x >= 0;
+        bad |= (w & 0x7e7e7e7eu) & nonneg_m;     // > 1
+        pw[0][i] = w & val;                      // x == 1 (bit 0 of a valid 0/1 byte)
+        pw[1][i] = val;                          // x >= 0
       } else {
-        bits[0] = x >= 0;
-        bits[1] = x == 1;
-        bits[2] = x == 2;
+        bad |= ((w & 0x7c7c7c7cu) | (w & (w >> 1) & ONES)) & nonneg_m;  // > 2
+        pw[0][i] = val;                          // x >= 0
+        pw[1][i] = w & ~(w >> 1) & val;          // x == 1
+        pw[2][i] = (w >> 1) & ~w & val;          // x == 2
       }
+      any_neg |= (neg_m & smask[i]) != 0;
+      any_bad |= (bad & smask[i]) != 0;
 #pragma unroll
-      for (int p = 0; p < NPL; ++p) {
-        words[p][b >> 2] |= bits[p] << (8 * (b & 3));
-        cnt[p] += bits[p];
-      }
+      for (int p = 0; p < NPL; ++p) cnt[p] += __popc(pw[p][i]);
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) {
-      int4 val;
-      val.x = (int)words[p][0];
-      val.y = (int)words[p][1];
-      val.z = (int)words[p][2];
-      val.w = (int)words[p][3];
-      *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + ch) = val;
+      int4 val4;
+      val4.x = (int)pw[p][0];
+      val4.y = (int)pw[p][1];
+      val4.z = (int)pw[p][2];
+      val4.w = (int)pw[p][3];
+      *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + 16 * sg) = val4;
       if (cnt[p]) atomicAdd(&msum[p][v], cnt[p]);
     }
   }

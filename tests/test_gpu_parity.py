@@ -367,3 +367,37 @@ def test_popcount_r2_equals_mfma(missing):
     for k in ("hist", "nan_pairs", "sparse_i", "sparse_j", "sparse_r2", "r2"):
         np.testing.assert_array_equal(np_(m[k]), np_(p[k]), err_msg=k)
     assert m["n_sparse"] == p["n_sparse"]
+
+
+# ---------------------------------------------------------------------------
+# plane packing against its definition
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("family,n_var,n_rows,missing", [("hap", 301, 150, 0.05), ("hap", 256, 300, 0.0),
+                                                          ("geno", 259, 131, 0.05), ("geno", 128, 257, 0.0)])
+def test_pack_planes_definition(family, n_var, n_rows, missing):
+    if family == "hap":
+        d, _ = hap_data(101, n_rows, n_var, missing)
+    else:
+        d, _ = geno_data(102, n_rows, n_var, missing)
+    rows = np.sort(np.random.default_rng(3).choice(n_rows, size=n_rows - 7, replace=False))
+    pk = band.pack_population(d, rows, family)
+    x = d[rows].astype(np.int16)  # (samples, variants)
+    want = [(x == 1), (x >= 0)] if family == "hap" else [(x >= 0), (x == 1), (x == 2)]
+    planes, marg = np_(pk.planes), np_(pk.marg)
+    assert planes.shape == (len(want), n_var, band.k_pad_for(len(rows)))
+    for p, w in enumerate(want):
+        np.testing.assert_array_equal(planes[p, :, :len(rows)], w.T.astype(np.int8))
+        assert not planes[p, :, len(rows):].any()
+        np.testing.assert_array_equal(marg[p], w.sum(axis=0))
+    assert pk.has_missing == bool((x < 0).any())
+
+
+@pytest.mark.parametrize("family,bad", [("hap", 2), ("hap", -2), ("geno", 3), ("geno", -5)])
+def test_pack_planes_domain_error(family, bad):
+    d, _ = (hap_data if family == "hap" else geno_data)(103, 70, 90, 0.0)
+    d = d.copy()
+    d[13, 77] = bad
+    with pytest.raises(ValueError):
+        band.pack_population(d, None, family)
+    # the same value in a row outside the selected population is not an error
+    band.pack_population(d, [r for r in range(70) if r != 13], family)
