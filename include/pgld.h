@@ -220,6 +220,26 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                  int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                  double* r2_out, void* stream);
 
+/* Pair estimators of pgld_band_ld_pairs */
+#define PGLD_EST_R2 0       /* r^2 from the pair's counts (ld_statistics.r_squared); NaN where undefined */
+#define PGLD_EST_R2_NAIVE 1 /* frequency r^2, per-site p over all valid samples (pairwise_r2,
+                               _tile_r2_naive, ld_statistics.py:301-308); 0 where undefined */
+#define PGLD_EST_SIGMA_D2 2 /* unbiased D^2 / pi_2 (_tile_sigma_d2, ld_statistics.py:311-341);
+                               0 unless n >= 4 and pi_2 != 0 */
+
+/* pgld_band_r2 with a choice of estimator and two more optional outputs:
+ * values     FP64 per band pair in (i, j) order (as r2_out);
+ * snp_sums   FP64 (2 * n_var): [v] = sum over pairs (v, j > v), [n_var + v] =
+ *            sum over pairs (i < v, v) of the estimator (NaN pairs skipped);
+ * snp_counts int64 (2 * n_var): the number of pairs counted in each sum (for
+ *            PGLD_EST_SIGMA_D2 the pairs with n >= 4).
+ * The per-SNP sums are what ZnS and Omega need (ld_statistics.py:534-862)
+ * without an m x m matrix.  Histogram bins clamp values outside [0, 1]. */
+int pgld_band_ld_pairs(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
+                       int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
+                       uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
+                       uint64_t* n_sparse, double* values, double* snp_sums, int64_t* snp_counts, void* stream);
+
 /* ---- bit-packed popcount variant of the tile products ------------------ */
 
 /* Bit planes from int8 0/1 planes (n_planes x n_var x k_pad, as written by

@@ -991,3 +991,85 @@ def rogers_huff_r(dosage):
         rr = np.where(den > 0, cov / np.where(den > 0, den, 1.0), np.nan)
     iu = np.triu_indices(g.shape[1], k=1)
     return rr[iu]
+
+
+# ---------------------------------------------------------------------------
+# Dense r^2 / sigma_D^2 matrices, ZnS and Omega  (ld_statistics.py:249-862,
+# haplotype_matrix.py:1092-1170)
+# ---------------------------------------------------------------------------
+
+
+def segregating_sites(hap):
+    """Columns with 0 < derived count < valid count (ld_statistics.py:263-270)."""
+    h = np.asarray(hap)
+    dac = np.maximum(h, 0).astype(np.int64).sum(axis=0)
+    nv = (h >= 0).sum(axis=0)
+    return np.nonzero((dac > 0) & (dac < nv))[0]
+
+
+def naive_r2_matrix(hap):
+    """pairwise_r2 (haplotype_matrix.py:1092-1170): per-site p over every
+    valid sample, p_AB over the pair's valid samples, D = p_AB - p_i p_j,
+    r^2 = D^2 / (p_i q_i p_j q_j), 0 where that is not > 0; diagonal 0."""
+    h = np.asarray(hap)
+    v = (h >= 0).astype(np.float64)
+    hc = np.where(h >= 0, h, 0).astype(np.float64)
+    nv = v.sum(axis=0)
+    p = np.where(nv > 0, hc.sum(axis=0) / np.where(nv > 0, nv, 1.0), 0.0)
+    jn = v.T @ v
+    j11 = hc.T @ hc
+    pab = np.where(jn > 0, j11 / np.where(jn > 0, jn, 1.0), 0.0)
+    D = pab - np.outer(p, p)
+    den = np.outer(p * (1 - p), p * (1 - p))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r2 = np.where(den > 0, (D ** 2) / np.where(den > 0, den, 1.0), 0.0)
+    np.fill_diagonal(r2, 0.0)
+    return r2
+
+
+def sigma_d2_matrix(hap):
+    """Full sigma_D^2 matrix and its valid mask (n >= 4), diagonal excluded
+    (ld_statistics.py:734-758)."""
+    h = np.asarray(hap)
+    m = h.shape[1]
+    ii, jj = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    c, _ = hap_counts(h, ii.ravel(), jj.ravel(), None)
+    sig, valid = sigma_d2_hap_from_counts(c)
+    sig = sig.reshape(m, m)
+    valid = valid.reshape(m, m)
+    np.fill_diagonal(sig, 0.0)
+    np.fill_diagonal(valid, False)
+    return sig, valid
+
+
+def zns_from_matrix(M, valid=None):
+    """ZnS: mean over pairs i != j; with a valid mask, over valid pairs only
+    (ld_statistics.py:534-731)."""
+    M = np.asarray(M, dtype=np.float64)
+    m = M.shape[0]
+    if m < 2:
+        return 0.0
+    if valid is None:
+        return float((M.sum() - np.trace(M)) / (m * (m - 1)))
+    n = int(valid.sum())
+    return float(M[valid].sum() / n) if n else 0.0
+
+
+def omega_from_matrix(M):
+    """Kim and Nielsen's Omega over the upper triangle (ld_statistics.py:
+    761-862): max over split points l = 3 .. m-2 of mean within / mean across,
+    by explicit block sums (no prefix sums)."""
+    M = np.triu(np.asarray(M, dtype=np.float64), k=1)
+    m = M.shape[0]
+    if m < 5:
+        return 0.0
+    best = 0.0
+    for l in range(3, m - 1):
+        left = M[:l, :l].sum()
+        right = M[l:, l:].sum()
+        cross = M[:l, l:].sum()
+        n_w = l * (l - 1) // 2 + (m - l) * (m - l - 1) // 2
+        n_c = l * (m - l)
+        if n_w > 0 and n_c > 0 and cross > 0:
+            best = max(best, ((left + right) / n_w) / (cross / n_c))
+    return float(best)

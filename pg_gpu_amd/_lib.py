@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libpgld.so")
 FAMILY_HAP, FAMILY_GENO = 0, 1
 STAT_DD, STAT_DZ, STAT_PI2, STAT_PI2_SYM, STAT_SIGMA_D2 = 0, 1, 2, 3, 4
 KEY_BIN_ABS, KEY_BIN, KEY_WINDOW = 0, 1, 2
+EST_R2, EST_R2_NAIVE, EST_SIGMA_D2 = 0, 1, 2
 SET_MOMENTS, SET_SELF = 0, 1
 
 TERM_DTYPE = np.dtype([("type", "<i4"), ("pops", "<i4", (4,)), ("pad", "<i4"), ("weight", "<f8")])
@@ -43,6 +44,8 @@ _SIGNATURES = {
     "pgld_band_plan_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], C.c_int),
     "pgld_band_plan_destroy": ([_vp], C.c_int),
     "pgld_band_plan_set_timing": ([_vp, _i32], C.c_int),
+    "pgld_band_ld_pairs": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                            _vp, _vp], C.c_int),
     "pgld_pack_bits": ([_vp, _i64, _i64, _i64, _vp, _vp], C.c_int),
     "pgld_band_counts_bits": ([_vp, _i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "pgld_band_r2_bits": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],

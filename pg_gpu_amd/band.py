@@ -179,21 +179,34 @@ class BandPlan:
         return (sums, kp) if want_pairs else sums
 
     def r2(self, pop, n_hist=100, thresh=0.2, sparse_cap=1 << 20, per_pair=False, histogram=True,
-           with_missing=None, method="mfma"):
-        """r^2 of every band pair of one haplotype population through the MFMA
-        tiles (pgld_band_r2).  Returns a dict with
-          hist      int64 (n_keys, n_hist) pairs per r^2 bin [b/n, (b+1)/n),
-          nan_pairs int64 (n_keys,) pairs whose r^2 is NaN,
-          sparse_i, sparse_j, sparse_r2  pairs with r^2 > thresh, sorted by
+           with_missing=None, method="mfma", estimator="r2", snp_sums=False):
+        """r^2-family values of every band pair of one haplotype population
+        through the tiled counts (pgld_band_ld_pairs / pgld_band_r2_bits).
+
+        estimator: "r2" (from the pair's counts, NaN where undefined),
+        "naive" (per-site frequencies, 0 where undefined, as pairwise_r2) or
+        "sigma_d2" (unbiased D^2 / pi_2).  Returns a dict with
+          hist      int64 (n_keys, n_hist) pairs per bin [b/n, (b+1)/n),
+          nan_pairs int64 (n_keys,) pairs whose value is NaN,
+          sparse_i, sparse_j, sparse_r2  pairs with value > thresh, sorted by
                     (i, j) (at most sparse_cap are kept),
           n_sparse  number of such pairs (may exceed sparse_cap),
-          r2        FP64 per band pair in (i, j) order if per_pair."""
+          r2        FP64 per band pair in (i, j) order if per_pair,
+          row_sums, col_sums, row_counts, col_counts  per-SNP sums over the
+                    pairs (v, j > v) and (i < v, v) if snp_sums."""
         if pop.family != L.FAMILY_HAP:
             raise ValueError("r^2 is defined here for haplotype populations")
         if with_missing is None:
             with_missing = pop.has_missing
             if with_missing is None:
                 raise ValueError("with_missing must be given for populations packed with validate=False")
+        est = {"r2": L.EST_R2, "naive": L.EST_R2_NAIVE, "sigma_d2": L.EST_SIGMA_D2}.get(estimator)
+        if est is None:
+            raise ValueError("estimator is 'r2', 'naive' or 'sigma_d2'")
+        if method not in ("mfma", "popcount"):
+            raise ValueError("method is 'mfma' or 'popcount'")
+        if method == "popcount" and (est != L.EST_R2 or snp_sums):
+            raise ValueError("the popcount variant computes the 'r2' estimator only")
         T = L.torch()
         hist = L.empty((self.n_keys, n_hist + 1), np.int64) if histogram else None
         cap = int(sparse_cap) if sparse_cap else 0
@@ -202,13 +215,16 @@ class BandPlan:
         sp_r = L.empty((max(cap, 1),), np.float32)
         n_sp = L.empty((1,), np.int64)
         r2 = L.empty((self.n_pairs,), np.float64) if per_pair else None
-        if method not in ("mfma", "popcount"):
-            raise ValueError("method is 'mfma' or 'popcount'")
-        src = pop.planes if method == "mfma" else pop.bit_planes()
-        L.call("pgld_band_r2" if method == "mfma" else "pgld_band_r2_bits", self._h, L.ptr(src), L.ptr(pop.marg),
-               pop.k_pad, pop.n_samples,
-               int(bool(with_missing)), int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),
-               L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.stream_handle())
+        sums = L.empty((2 * self.n_var,), np.float64) if snp_sums else None
+        cnts = L.empty((2 * self.n_var,), np.int64) if snp_sums else None
+        if method == "mfma":
+            L.call("pgld_band_ld_pairs", self._h, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad, pop.n_samples,
+                   int(bool(with_missing)), est, int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),
+                   L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.ptr(sums), L.ptr(cnts), L.stream_handle())
+        else:
+            L.call("pgld_band_r2_bits", self._h, L.ptr(pop.bit_planes()), L.ptr(pop.marg), pop.k_pad,
+                   pop.n_samples, int(bool(with_missing)), int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i),
+                   L.ptr(sp_j), L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.stream_handle())
         n_total = int(n_sp.cpu().item())
         k = min(n_total, cap)
         si, sj, sr = sp_i[:k], sp_j[:k], sp_r[:k]
@@ -221,6 +237,9 @@ class BandPlan:
             out["nan_pairs"] = hist[:, n_hist]
         if per_pair:
             out["r2"] = r2
+        if snp_sums:
+            out["row_sums"], out["col_sums"] = sums[:self.n_var], sums[self.n_var:]
+            out["row_counts"], out["col_counts"] = cnts[:self.n_var], cnts[self.n_var:]
         return out
 
     def counts(self, pop, with_missing=None, method="mfma"):

@@ -652,6 +652,26 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   if (!rc) rc = up((void**)&plan->d_key_off, key_off.data(), key_off.size() * sizeof(int32_t));
   if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
                    key_list.size() * sizeof(int64_t));
+  if (!rc && plan->n_regions == 0) {
+    // tile lists by tile-row (contiguous in the schedule) and tile-column
+    const int64_t n_tr = ceil_div(std::max<int64_t>(n_var, 1), BT);
+    std::vector<int64_t> row_off(n_tr + 1, 0), col_off(n_tr + 1, 0);
+    for (const BandTile& t : tb.tiles) {
+      row_off[t.i0 / BT + 1]++;
+      col_off[t.j0 / BT + 1]++;
+    }
+    for (int64_t r = 0; r < n_tr; ++r) {
+      row_off[r + 1] += row_off[r];
+      col_off[r + 1] += col_off[r];
+    }
+    std::vector<int32_t> col_list(tb.tiles.size());
+    std::vector<int64_t> fill(col_off.begin(), col_off.end() - 1);
+    for (size_t t = 0; t < tb.tiles.size(); ++t) col_list[fill[tb.tiles[t].j0 / BT]++] = (int32_t)t;
+    rc = up((void**)&plan->d_row_off, row_off.data(), row_off.size() * sizeof(int64_t));
+    if (!rc) rc = up((void**)&plan->d_col_off, col_off.data(), col_off.size() * sizeof(int64_t));
+    if (!rc) rc = up((void**)&plan->d_col_list, col_list.empty() ? nullptr : col_list.data(),
+                     col_list.size() * sizeof(int32_t));
+  }
   if (plan->n_regions > 0) plan->h_tiles.swap(tb.host);
   return rc;
 }
@@ -825,8 +845,9 @@ int pgld_band_plan_info(const pgld_band_plan* plan, int64_t* n_tiles, int64_t* n
 
 int pgld_band_plan_destroy(pgld_band_plan* plan) {
   if (!plan) return PGLD_OK;
-  void* ptrs[] = {plan->d_pos,     plan->d_kc,       plan->d_edges,    plan->d_tiles,  plan->d_pair_base,
-                  plan->d_key_off, plan->d_key_list, plan->d_partials, plan->d_regions, plan->d_order};
+  void* ptrs[] = {plan->d_pos,      plan->d_kc,       plan->d_edges,     plan->d_tiles,    plan->d_pair_base,
+                  plan->d_key_off,  plan->d_key_list, plan->d_partials,  plan->d_regions,  plan->d_order,
+                  plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)

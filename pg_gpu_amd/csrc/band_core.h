@@ -40,6 +40,12 @@ struct pgld_band_plan {
   bool lpt = true;                     // cost-descending tile order (else natural)
   bool xcd_split = false;              // each XCD walks a contiguous eighth of the order
   int64_t partials_ns = 0;             // statistics + 1 the workspace was sized for
+  // per-SNP sums of pair values (single bands): tiles by tile-row / tile-column
+  int64_t* d_row_off = nullptr;        // tile-row r -> tiles [row_off[r], row_off[r + 1])
+  int64_t* d_col_off = nullptr;        // tile-column c -> col_list[col_off[c] .. col_off[c + 1])
+  int32_t* d_col_list = nullptr;
+  double* d_snp_part = nullptr;        // workspace: per tile 64 row + 64 column sums
+  int32_t* d_snp_cnt = nullptr;
 };
 
 namespace pgld {

@@ -36,12 +36,21 @@ struct R2Args {
   float* sp_r2;
   int64_t sp_cap;
   unsigned long long* sp_count;
-  double* r2_out;  // per band pair or null
+  double* r2_out;        // per band pair or null
+  double* snp_part;      // per tile: 64 row sums then 64 column sums, or null
+  int32_t* snp_cnt;      // per tile: 64 row counts then 64 column counts
+};
+
+// Per-pair estimators of the r^2 family
+enum R2Est : int32_t {
+  EST_R2 = 0,        // r^2 from the pair's own counts (ld_statistics.r_squared); NaN where undefined
+  EST_R2_NAIVE = 1,  // frequency r^2 with per-site p over all valid samples (_tile_r2_naive); 0 where undefined
+  EST_SIGMA_D2 = 2,  // unbiased D^2 / pi_2 (_tile_sigma_d2); 0 unless n >= 4 and pi_2 != 0
 };
 
 constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
 
-template <bool MISSING, bool POPC>
+template <bool MISSING, bool POPC, int EST>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const int8_t* __restrict__ planes,
                                                              const int32_t* __restrict__ marg, int64_t k_pad,
                                                              int32_t S) {
@@ -50,12 +59,20 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
   __shared__ int s_wkey[BT];
   __shared__ int s_hist[R2_MAX_HIST_LDS];
+  __shared__ double s_rsum[2][BT], s_csum[2][BT];  // [column half][row], [row half][column]
+  __shared__ int s_rcnt[2][BT], s_ccnt[2][BT];
 
   const BandTile tile = ra.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int hw = ra.n_hist + 1;
   const int h_n = ra.hist ? tile.n_slots * hw : 0;
   for (int t = tid; t < h_n; t += BAND_THREADS) s_hist[t] = 0;
+  if (ra.snp_part && tid < 2 * BT) {
+    (&s_rsum[0][0])[tid] = 0.0;
+    (&s_csum[0][0])[tid] = 0.0;
+    (&s_rcnt[0][0])[tid] = 0;
+    (&s_ccnt[0][0])[tid] = 0;
+  }
   if (tid < BT) {
     int64_t gi = min((int64_t)tile.i0 + tid, ra.n_var - 1), gj = min((int64_t)tile.j0 + tid, ra.n_var - 1);
     s_pos_i[tid] = ra.pos[gi];
@@ -81,6 +98,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   if (wave_live) {
     const int jl = wc + (lane & 31);
     const int64_t j = (int64_t)tile.j0 + jl;
+    double csum = 0.0;
+    int ccnt = 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int il = wr + acc_row(r, lane);
@@ -88,6 +107,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
       const bool valid = i < j && j < ra.n_var && s_pos_j[jl] - s_pos_i[il] <= ra.max_dist;
       float r2f = 0.0f;
       bool hit = false;
+      double sval = 0.0;
+      int scnt = 0;
       if (valid) {
         int64_t n11 = acc.c[0][0][r], nA, nB, n;
         if constexpr (MISSING) {
@@ -99,12 +120,45 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           nB = marg[j];
           n = S;
         }
-        const int64_t diff = n * n11 - nA * nB;
-        const double dd = (double)diff;
-        const double num = dd * dd;
-        const double den = (double)(nA * (n - nA)) * (double)(nB * (n - nB));
-        const double r2 = den > 0.0 ? num / den : __builtin_nan("");
-        const bool is_nan = !(den > 0.0);
+        double r2;
+        bool is_nan = false;
+        if constexpr (EST == EST_R2) {
+          const int64_t diff = n * n11 - nA * nB;
+          const double dd = (double)diff;
+          const double num = dd * dd;
+          const double den = (double)(nA * (n - nA)) * (double)(nB * (n - nB));
+          is_nan = !(den > 0.0);
+          r2 = den > 0.0 ? num / den : __builtin_nan("");
+          sval = is_nan ? 0.0 : r2;
+          scnt = !is_nan;
+        } else if constexpr (EST == EST_R2_NAIVE) {
+#pragma clang fp contract(off)
+          // per-site frequencies over every valid sample of the site, joint
+          // frequency over the pair's valid samples (the reference's order of
+          // operations: p_AB - p_i p_j, then D^2 / (p_i q_i p_j q_j))
+          const double vi = MISSING ? (double)marg[ra.n_var + i] : (double)S;
+          const double vj = MISSING ? (double)marg[ra.n_var + j] : (double)S;
+          const double pi = vi > 0.0 ? (double)marg[i] / vi : 0.0;
+          const double pj = vj > 0.0 ? (double)marg[j] / vj : 0.0;
+          const double pab = n > 0 ? (double)n11 / (double)n : 0.0;
+          const double D = pab - pi * pj;
+          const double den = (pi * (1.0 - pi)) * (pj * (1.0 - pj));
+          r2 = den > 0.0 ? (D * D) / den : 0.0;
+          sval = r2;
+          scnt = 1;
+        } else {
+          const double c1 = (double)n11, c2 = (double)(nA - n11), c3 = (double)(nB - n11);
+          const double c4 = (double)(n - nA - nB + n11);
+          const double dd_num = c1 * (c1 - 1.0) * c4 * (c4 - 1.0) + c2 * (c2 - 1.0) * c3 * (c3 - 1.0) -
+                                2.0 * c1 * c2 * c3 * c4;
+          const double pi2_num = (c1 + c2) * (c1 + c3) * (c2 + c4) * (c3 + c4) -
+                                 c1 * c4 * (-1.0 + c1 + 3.0 * c2 + 3.0 * c3 + c4) -
+                                 c2 * c3 * (-1.0 + 3.0 * c1 + c2 + c3 + 3.0 * c4);
+          const bool ok = n >= 4;
+          r2 = (ok && pi2_num != 0.0) ? dd_num / pi2_num : 0.0;
+          sval = r2;
+          scnt = ok;
+        }
         r2f = (float)r2;
         if (ra.hist) {
           int key;
@@ -125,6 +179,22 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
         hit = !is_nan && r2 > ra.thresh;
         if (ra.r2_out) ra.r2_out[ra.pair_base[i] + (j - i - 1)] = r2;
       }
+      if (ra.snp_part) {
+        // row sum over this half-wave's 32 columns (fixed butterfly order)
+        double rs = sval;
+        int rc = scnt;
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) {
+          rs += __shfl_xor(rs, o);
+          rc += __shfl_xor(rc, o);
+        }
+        if ((lane & 31) == 0) {
+          s_rsum[wc >> 5][il] = rs;
+          s_rcnt[wc >> 5][il] = rc;
+        }
+        csum += sval;
+        ccnt += scnt;
+      }
       if (ra.sp_count) {
         const unsigned long long m = __ballot(hit);
         if (m) {
@@ -143,6 +213,26 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
         }
       }
     }
+    if (ra.snp_part) {
+      // column sums: the 16 rows of each half, then both halves
+      csum += __shfl_xor(csum, 32);
+      ccnt += __shfl_xor(ccnt, 32);
+      if (lane < 32) {
+        s_csum[wr >> 5][jl] = csum;
+        s_ccnt[wr >> 5][jl] = ccnt;
+      }
+    }
+  }
+  if (ra.snp_part) {
+    __syncthreads();
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+    if (tid < BT) {
+      ra.snp_part[t * 2 * BT + tid] = s_rsum[0][tid] + s_rsum[1][tid];
+      ra.snp_cnt[t * 2 * BT + tid] = s_rcnt[0][tid] + s_rcnt[1][tid];
+    } else if (tid < 2 * BT) {
+      ra.snp_part[t * 2 * BT + tid] = s_csum[0][tid - BT] + s_csum[1][tid - BT];
+      ra.snp_cnt[t * 2 * BT + tid] = s_ccnt[0][tid - BT] + s_ccnt[1][tid - BT];
+    }
   }
   if (ra.hist) {
     __syncthreads();
@@ -158,6 +248,35 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   if (t < n) p[t] = 0ull;
 }
 
+// Per-SNP totals from the tiles' partial rows: sums[v] / counts[v] over the
+// pairs (v, j > v), sums[n_var + v] / counts[n_var + v] over the pairs
+// (i < v, v); tiles are visited in schedule order (deterministic).
+__global__ void k_snp_reduce(const double* __restrict__ part, const int32_t* __restrict__ cnt,
+                             const int64_t* __restrict__ row_off, const int64_t* __restrict__ col_off,
+                             const int32_t* __restrict__ col_list, int64_t n_var, double* __restrict__ sums,
+                             int64_t* __restrict__ counts) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_var) return;
+  const int64_t tr = v / BT, q = v % BT;
+  double s = 0.0;
+  int64_t c = 0;
+  for (int64_t t = row_off[tr]; t < row_off[tr + 1]; ++t) {
+    s += part[t * 2 * BT + q];
+    c += cnt[t * 2 * BT + q];
+  }
+  sums[v] = s;
+  if (counts) counts[v] = c;
+  s = 0.0;
+  c = 0;
+  for (int64_t e = col_off[tr]; e < col_off[tr + 1]; ++e) {
+    const int64_t t = col_list[e];
+    s += part[t * 2 * BT + BT + q];
+    c += cnt[t * 2 * BT + BT + q];
+  }
+  sums[n_var + v] = s;
+  if (counts) counts[n_var + v] = c;
+}
+
 }  // namespace pgld
 
 using namespace pgld;
@@ -165,9 +284,11 @@ using namespace pgld;
 extern "C" {
 
 static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, const int32_t* marg, int64_t k_pad,
-                   int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
-                   int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
-                   double* r2_out, void* stream) {
+                   int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
+                   uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
+                   uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream) {
+  PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_SIGMA_D2, "pgld_band_ld_pairs: unknown estimator %d",
+               estimator);
   PGLD_REQUIRE(plan && planes && marg, "pgld_band_r2: null pointer");
   PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
   PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= n_samples && n_samples >= 0,
@@ -211,37 +332,70 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
   ra.sp_cap = n_sparse ? sparse_cap : 0;
   ra.sp_count = reinterpret_cast<unsigned long long*>(n_sparse);
   ra.r2_out = r2_out;
+  auto* pl = const_cast<pgld_band_plan*>(plan);
+  if (snp_sums) {
+    if (!pl->d_snp_part) {
+      PGLD_TRY_HIP(hipMalloc((void**)&pl->d_snp_part, (size_t)plan->n_tiles * 2 * BT * sizeof(double)),
+                   "pgld_band_ld_pairs workspace");
+      PGLD_TRY_HIP(hipMalloc((void**)&pl->d_snp_cnt, (size_t)plan->n_tiles * 2 * BT * sizeof(int32_t)),
+                   "pgld_band_ld_pairs workspace");
+    }
+    ra.snp_part = pl->d_snp_part;
+    ra.snp_cnt = pl->d_snp_cnt;
+  }
   const bool miss = with_missing != 0;
   const size_t lds = miss ? (size_t)opbuf_bytes<2>() : (size_t)opbuf_bytes<1>();
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[0], s), "pgld_band_r2 event");
+#define PGLD_R2(M, B, E) hipLaunchKernelGGL((k_band_r2<M, B, E>), g, b, lds, s, ra, planes, marg, k_pad, n_samples)
+#define PGLD_R2_EST(M, B)                              \
+  do {                                                 \
+    if (estimator == EST_R2) PGLD_R2(M, B, EST_R2);    \
+    else if (estimator == EST_R2_NAIVE) PGLD_R2(M, B, EST_R2_NAIVE); \
+    else PGLD_R2(M, B, EST_SIGMA_D2);                  \
+  } while (0)
   if (popc) {
-    if (miss) hipLaunchKernelGGL((k_band_r2<true, true>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
-    else hipLaunchKernelGGL((k_band_r2<false, true>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+    if (miss) PGLD_R2_EST(true, true); else PGLD_R2_EST(false, true);
   } else {
-    if (miss) hipLaunchKernelGGL((k_band_r2<true, false>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
-    else hipLaunchKernelGGL((k_band_r2<false, false>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);
+    if (miss) PGLD_R2_EST(true, false); else PGLD_R2_EST(false, false);
   }
+#undef PGLD_R2_EST
+#undef PGLD_R2
   rc = check_launch("k_band_r2");
   if (rc) return rc;
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[1], s), "pgld_band_r2 event");
-  return PGLD_OK;
+  if (snp_sums) {
+    hipLaunchKernelGGL(k_snp_reduce, dim3((unsigned)ceil_div(plan->n_var, 256)), dim3(256), 0, s, pl->d_snp_part,
+                       pl->d_snp_cnt, plan->d_row_off, plan->d_col_off, plan->d_col_list, plan->n_var, snp_sums,
+                       snp_counts);
+    rc = check_launch("k_snp_reduce");
+  }
+  return rc;
 }
 
 int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
                  int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
                  int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                  double* r2_out, void* stream) {
-  return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, n_hist, thresh, hist, sparse_i, sparse_j,
-                 sparse_r2, sparse_cap, n_sparse, r2_out, stream);
+  return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, EST_R2, n_hist, thresh, hist, sparse_i,
+                 sparse_j, sparse_r2, sparse_cap, n_sparse, r2_out, nullptr, nullptr, stream);
+}
+
+int pgld_band_ld_pairs(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
+                       int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
+                       uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
+                       uint64_t* n_sparse, double* values, double* snp_sums, int64_t* snp_counts, void* stream) {
+  return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, estimator, n_hist, thresh, hist,
+                 sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, values, snp_sums, snp_counts, stream);
 }
 
 int pgld_band_r2_bits(const pgld_band_plan* plan, const uint32_t* bits, const int32_t* marg, int64_t k_pad,
                       int32_t n_samples, int32_t with_missing, int32_t n_hist, double thresh, uint64_t* hist,
                       int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap, uint64_t* n_sparse,
                       double* r2_out, void* stream) {
-  return band_r2(plan, reinterpret_cast<const int8_t*>(bits), true, marg, k_pad, n_samples, with_missing, n_hist,
-                 thresh, hist, sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, r2_out, stream);
+  return band_r2(plan, reinterpret_cast<const int8_t*>(bits), true, marg, k_pad, n_samples, with_missing, EST_R2,
+                 n_hist, thresh, hist, sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, r2_out, nullptr, nullptr,
+                 stream);
 }
 
 }  // extern "C"

@@ -353,6 +353,75 @@ class HaplotypeMatrix:
             out[key] = OrderedDict((n, float(v)) for n, v in zip(names, vals))
         return out
 
+    def pairwise_r2(self, estimator: str = "r2"):
+        """Pairwise r^2 of every variant pair as a dense (m, m) float64 GPU
+        tensor: per-site frequencies over all valid samples, joint frequency
+        over the pair's valid samples, 0 where undefined, zero diagonal
+        (haplotype_matrix.py:1139-1170).  Counts through the MFMA tiles."""
+        if estimator != "r2":
+            raise NotImplementedError("only the 'r2' estimator is in this build") if estimator == "rogers_huff" \
+                else ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+        self.transfer_to_gpu()
+        T = L.torch()
+        m = self.num_variants
+        out = T.zeros((m, m), dtype=T.float64, device=L.device())
+        if m < 2:
+            return out
+        plan = B.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
+        pk = B.pack_population(self._haplotypes, None, "hap")
+        vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="naive")["r2"]
+        iu = T.triu_indices(m, m, offset=1, device=out.device)
+        out[iu[0], iu[1]] = vals
+        out[iu[1], iu[0]] = vals
+        return out
+
+    def windowed_r_squared(self, bp_bins, percentile=50, pop=None, estimator: str = "r2"):
+        """Percentiles of r^2 (from each pair's counts; NaN pairs excluded)
+        per distance bin [b_k, b_{k+1}), and the pair count per bin
+        (haplotype_matrix.py:1233-1317).  r^2 of the pairs within the last
+        edge come from the MFMA tiles; the percentiles are taken on the
+        host, as the reference does."""
+        if estimator != "r2":
+            raise NotImplementedError("only the 'r2' estimator is in this build") if estimator == "rogers_huff" \
+                else ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+        self.transfer_to_gpu()
+        bins = np.asarray(bp_bins, dtype=np.float64)
+        n_bins = len(bins) - 1
+        rows = None
+        if pop is not None:
+            if self._sample_sets is None or pop not in self._sample_sets:
+                raise KeyError(f"Population key {pop} must exist in sample_sets")
+            rows = self._sample_sets[pop]
+        pos = self._pos_np().astype(np.float64)
+        plan = B.BandPlan(pos, float(bins[-1]), edges=bins)
+        pk = B.pack_population(self._haplotypes, rows, "hap")
+        vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="r2")["r2"]
+        keys = L.empty((plan.n_pairs,), np.int32)
+        pos_d = L.to_device(pos, np.float64)
+        edges_d = L.to_device(bins, np.float64)
+        off = 0
+        for ii, jj in LP.iter_pairs_within_distance(pos, float(bins[-1]), 10**8):
+            n = int(ii.shape[0])
+            L.call("pgld_pair_keys", L.ptr(pos_d), L.ptr(ii), L.ptr(jj), n, L.KEY_BIN, L.ptr(edges_d), n_bins, 0.0,
+                   1.0, L.ptr(keys[off:off + n]), L.stream_handle())
+            off += n
+        r2 = vals.cpu().numpy()
+        k = keys.cpu().numpy()
+        ok = (k >= 0) & ~np.isnan(r2)
+        r2, k = r2[ok], k[ok]
+        pct = np.atleast_1d(percentile)
+        result = np.full((n_bins, len(pct)), np.nan)
+        counts = np.bincount(k, minlength=n_bins)[:n_bins].astype(int)
+        order = np.argsort(k, kind="stable")
+        r2s, ks = r2[order], k[order]
+        starts = np.searchsorted(ks, np.arange(n_bins + 1))
+        for b in range(n_bins):
+            if counts[b] > 0:
+                result[b] = np.percentile(r2s[starts[b]:starts[b + 1]], pct)
+        if result.shape[1] == 1:
+            result = result[:, 0]
+        return result, counts
+
     def windowed_ld_sums(self, window, max_dist=None, pops=None, origin=0.0, n_windows=None,
                          stat_set="moments"):
         """FP64 sums of the ld_names(P) statistics per window of the left SNP

@@ -148,3 +148,129 @@ def pi2_two_pops(counts, pop_indices, n_valid1=None, n_valid2=None):
     warnings.warn("pi2_two_pops() is deprecated.", DeprecationWarning, stacklevel=2)
     nv = (n_valid1, n_valid2) if n_valid1 is not None and n_valid2 is not None else None
     return pi2(counts, populations=pop_indices, n_valid=nv)
+
+
+# ---------------------------------------------------------------------------
+# ZnS and Omega (ld_statistics.py:534-862)
+# ---------------------------------------------------------------------------
+
+def _resolve_ld_estimator(estimator, is_hap_matrix):
+    """'auto' -> 'sigma_d2' for a HaplotypeMatrix, 'rogers_huff' otherwise
+    (ld_statistics.py:344-363)."""
+    if estimator == "auto":
+        return "sigma_d2" if is_hap_matrix else "rogers_huff"
+    if estimator not in ("r2", "sigma_d2", "rogers_huff"):
+        raise ValueError(f"Unknown estimator: {estimator!r} (expected one of 'auto', 'r2', 'sigma_d2', "
+                         "'rogers_huff')")
+    return estimator
+
+
+def _segregating(mat, missing_data):
+    """Sites kept for ZnS / Omega: optionally only sites without missing data,
+    then sites with 0 < derived count < valid count (ld_statistics.py:249-278)."""
+    mat.transfer_to_gpu()
+    if missing_data == "exclude":
+        alt, valid = (x.cpu().numpy() for x in mat.allele_counts())
+        mat = mat.get_subset(np.nonzero(valid == mat.num_haplotypes)[0])
+    if mat.num_variants == 0:
+        return mat
+    alt, valid = (x.cpu().numpy() for x in mat.allele_counts())
+    seg = np.nonzero((alt > 0) & (alt < valid))[0]
+    if len(seg) < mat.num_variants:
+        mat = mat.get_subset(seg)
+    return mat
+
+
+def _snp_sums(mat, estimator):
+    """Per-SNP row / column sums of the estimator over all pairs of the
+    matrix (one all-pairs band, MFMA tiles)."""
+    from . import band as B
+    pos = np.arange(mat.num_variants, dtype=np.float64)  # all pairs: positions play no part
+    plan = B.BandPlan(pos, np.inf, window=1e300, n_keys=1)
+    pk = B.pack_population(mat.haplotypes, None, "hap")
+    out = plan.r2(pk, histogram=False, sparse_cap=0, estimator=estimator, snp_sums=True)
+    return {k: out[k].cpu().numpy() for k in ("row_sums", "col_sums", "row_counts", "col_counts")}
+
+
+def zns(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
+    """Kelly's ZnS, the mean pairwise LD over all SNP pairs
+    (ld_statistics.py:679-731).  For a HaplotypeMatrix the pairs are never
+    materialised: one pass of the MFMA tiles yields per-SNP sums.
+    estimator 'sigma_d2' (the 'auto' choice) averages the unbiased
+    D^2 / pi_2 over pairs with n >= 4; 'r2' averages the frequency r^2 over
+    all pairs.  A precomputed square r^2 array is averaged off the diagonal."""
+    from .genotype_matrix import GenotypeMatrix
+    from .haplotype_matrix import HaplotypeMatrix
+    is_hm = isinstance(r2_matrix_or_matrix, HaplotypeMatrix)
+    est = _resolve_ld_estimator(estimator, is_hm)
+    if is_hm:
+        if est == "rogers_huff":
+            raise NotImplementedError("the rogers_huff estimator is outside this build")
+        mat = _segregating(r2_matrix_or_matrix, missing_data)
+        m = mat.num_variants
+        if m < 2:
+            return 0.0
+        s = _snp_sums(mat, "sigma_d2" if est == "sigma_d2" else "naive")
+        if est == "sigma_d2":
+            n = int(s["row_counts"].sum())
+            return float(s["row_sums"].sum() / n) if n else 0.0
+        return float(2.0 * s["row_sums"].sum() / (m * (m - 1)))
+    if isinstance(r2_matrix_or_matrix, GenotypeMatrix):
+        raise NotImplementedError("ZnS of a GenotypeMatrix (dosage correlation) is outside this build")
+    if est == "sigma_d2":
+        raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix, not a pre-computed r² array")
+    M = np.asarray(L.to_numpy(r2_matrix_or_matrix), dtype=np.float64)
+    m = M.shape[0]
+    if m < 2:
+        return 0.0
+    return float((M.sum() - np.trace(M)) / (m * (m - 1)))
+
+
+def _omega_from_sums(row_sums, col_sums):
+    """Omega from per-SNP upper-triangle sums: for a split l, the left block
+    sum is the prefix of column sums below l, the cross block the prefix of
+    row sums below l minus the left block (ld_statistics.py:808-862)."""
+    m = len(row_sums)
+    if m < 5:
+        return 0.0
+    left_c = np.concatenate([[0.0], np.cumsum(col_sums)])
+    rows_c = np.concatenate([[0.0], np.cumsum(row_sums)])
+    total = rows_c[-1]
+    l = np.arange(3, m - 1)
+    left = left_c[l]
+    cross = rows_c[l] - left
+    right = total - left - cross
+    n_left = l * (l - 1) // 2
+    n_right = (m - l) * (m - l - 1) // 2
+    n_cross = l * (m - l)
+    n_w = n_left + n_right
+    ok = (n_w > 0) & (n_cross > 0) & (cross > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        vals = np.where(ok, ((left + right) / np.where(n_w > 0, n_w, 1)) /
+                        np.where(ok, cross / np.where(n_cross > 0, n_cross, 1), 1.0), 0.0)
+    return float(vals.max()) if len(vals) else 0.0
+
+
+def omega(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
+    """Kim and Nielsen's Omega: the largest ratio of mean LD within the two
+    blocks of a split to mean LD across it, over the upper triangle
+    (ld_statistics.py:761-862).  For a HaplotypeMatrix one pass of the MFMA
+    tiles yields the per-SNP sums the split ratios need."""
+    from .genotype_matrix import GenotypeMatrix
+    from .haplotype_matrix import HaplotypeMatrix
+    is_hm = isinstance(r2_matrix_or_matrix, HaplotypeMatrix)
+    est = _resolve_ld_estimator(estimator, is_hm)
+    if is_hm:
+        if est == "rogers_huff":
+            raise NotImplementedError("the rogers_huff estimator is outside this build")
+        mat = _segregating(r2_matrix_or_matrix, missing_data)
+        if mat.num_variants < 5:
+            return 0.0
+        s = _snp_sums(mat, "sigma_d2" if est == "sigma_d2" else "naive")
+        return _omega_from_sums(s["row_sums"], s["col_sums"])
+    if isinstance(r2_matrix_or_matrix, GenotypeMatrix):
+        raise NotImplementedError("Omega of a GenotypeMatrix (dosage correlation) is outside this build")
+    if est == "sigma_d2":
+        raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix")
+    M = np.triu(np.asarray(L.to_numpy(r2_matrix_or_matrix), dtype=np.float64), k=1)
+    return _omega_from_sums(M.sum(axis=1), M.sum(axis=0))

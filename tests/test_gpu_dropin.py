@@ -227,3 +227,89 @@ def test_sigma_d2_geno():
         LSG.sigma_d2_geno(GenotypeMatrix(gmiss, np.arange(60) * 10))
     with pytest.raises(TypeError):
         LSG.sigma_d2_geno(g)
+
+
+# ---------------------------------------------------------------------------
+# r^2-family estimators, ZnS, Omega, pairwise_r2, windowed_r_squared
+# ---------------------------------------------------------------------------
+from pg_gpu_amd import band, ld_statistics as LS  # noqa: E402
+
+
+def _ld_panel(seed, n_hap=90, n_var=150, missing=0.03):
+    rng = np.random.default_rng(seed)
+    h = SY.neutral_haplotypes(n_hap, n_var, rng, missing=missing)
+    h[:, 7] = 0  # a monomorphic site
+    return h
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.04])
+def test_ld_pair_estimators_exact(missing):
+    h = _ld_panel(61, missing=missing)
+    m = h.shape[1]
+    plan = band.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
+    pk = band.pack_population(h, None, "hap")
+    ii, jj = np.triu_indices(m, k=1)
+    c, _ = O.hap_counts(h, ii, jj)
+    naive = O.naive_r2_matrix(h)
+    sig, sval = O.sigma_d2_matrix(h)
+    for est, want, counted in (("r2", O.r_squared_exact(c), None),
+                               ("naive", naive[ii, jj], np.ones(len(ii), bool)),
+                               ("sigma_d2", sig[ii, jj], sval[ii, jj])):
+        out = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator=est, snp_sums=True)
+        got = np_(out["r2"])
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(want), err_msg=est)
+        ok = ~np.isnan(want)
+        np.testing.assert_array_equal(got[ok], want[ok], err_msg=est)
+        if counted is None:
+            counted = ok
+        vals = np.where(counted, np.nan_to_num(want), 0.0)
+        Mv = np.zeros((m, m))
+        Mc = np.zeros((m, m), dtype=np.int64)
+        Mv[ii, jj] = vals
+        Mc[ii, jj] = counted
+        np.testing.assert_allclose(np_(out["row_sums"]), Mv.sum(axis=1), rtol=1e-12, atol=1e-12, err_msg=est)
+        np.testing.assert_allclose(np_(out["col_sums"]), Mv.sum(axis=0), rtol=1e-12, atol=1e-12, err_msg=est)
+        np.testing.assert_array_equal(np_(out["row_counts"]), Mc.sum(axis=1))
+        np.testing.assert_array_equal(np_(out["col_counts"]), Mc.sum(axis=0))
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.04])
+def test_zns_omega_pairwise_r2(missing):
+    h = _ld_panel(62, n_hap=70, n_var=120, missing=missing)
+    hm = HaplotypeMatrix(h, np.arange(h.shape[1]) * 100)
+    seg = O.segregating_sites(h)
+    hs = h[:, seg]
+    sig, valid = O.sigma_d2_matrix(hs)
+    naive = O.naive_r2_matrix(hs)
+    assert LS.zns(hm) == pytest.approx(O.zns_from_matrix(sig, valid), rel=1e-10)
+    assert LS.zns(hm, estimator="r2") == pytest.approx(O.zns_from_matrix(naive), rel=1e-10)
+    assert LS.omega(hm) == pytest.approx(O.omega_from_matrix(sig), rel=1e-10)
+    assert LS.omega(hm, estimator="r2") == pytest.approx(O.omega_from_matrix(naive), rel=1e-10)
+    # precomputed arrays go through the same host formulas
+    assert LS.zns(naive) == pytest.approx(O.zns_from_matrix(naive), rel=1e-12)
+    assert LS.omega(naive) == pytest.approx(O.omega_from_matrix(naive), rel=1e-12)
+    full = np_(HaplotypeMatrix(h, np.arange(h.shape[1]) * 100).pairwise_r2())
+    np.testing.assert_array_equal(full, O.naive_r2_matrix(h))
+    if missing == 0.0:
+        ex = HaplotypeMatrix(h, np.arange(h.shape[1]) * 100)
+        assert LS.zns(ex, missing_data="exclude") == pytest.approx(O.zns_from_matrix(sig, valid), rel=1e-10)
+
+
+def test_windowed_r_squared():
+    h = _ld_panel(63, n_hap=80, n_var=300, missing=0.02)
+    rng = np.random.default_rng(4)
+    pos = SY.uniform_positions(300, 60000, rng)
+    hm = HaplotypeMatrix(h, pos, sample_sets={"a": list(range(40)), "b": list(range(40, 80))})
+    bins = [0, 500, 2000, 8000, 30000]
+    res, cnt = hm.windowed_r_squared(bins, percentile=[10, 50, 90], pop="b")
+    ii, jj = np.triu_indices(300, k=1)
+    c, nv = O.hap_counts(h, ii, jj, list(range(40, 80)))
+    r2 = O.r_squared(c, nv)
+    k = O.bin_index(pos[jj] - pos[ii], np.array(bins, dtype=float))
+    ok = (k >= 0) & (k < 4) & ~np.isnan(r2)
+    for b in range(4):
+        sel = r2[ok & (k == b)]
+        assert cnt[b] == len(sel)
+        np.testing.assert_allclose(res[b], np.percentile(sel, [10, 50, 90]), rtol=1e-12, atol=1e-15)
+    res1, _ = hm.windowed_r_squared(bins)
+    assert res1.shape == (4,)

@@ -256,3 +256,39 @@ def test_r_squared_exact_matches_reference_form():
     h, nan = O.r2_histogram(np.array([0.0, 0.0099, 0.01, 0.5, 1.0, np.nan, 0.3]),
                             np.array([0, 0, 0, 1, 1, 1, -1]), 2, 100)
     assert h[0, 0] == 2 and h[0, 1] == 1 and h[1, 50] == 1 and h[1, 99] == 1 and list(nan) == [0, 1]
+
+
+def test_naive_r2_and_sigma_matrices_pins():
+    """Dense matrices of the ZnS / Omega path: without missing data the naive
+    r^2 equals the count-based r^2 (both are D^2 / (p q p q) on the same
+    frequencies) and sigma_D^2 equals the tile ratio of the pinned DD and
+    pi_2 numerators; Omega's explicit block sums are checked by brute force
+    over pairs."""
+    rng = np.random.default_rng(17)
+    h = (rng.random((40, 12)) < rng.uniform(0.1, 0.9, size=12)).astype(np.int8)
+    R = O.naive_r2_matrix(h)
+    ii, jj = np.triu_indices(12, k=1)
+    c, _ = O.hap_counts(h, ii, jj)
+    w = O.r_squared(c)
+    np.testing.assert_allclose(R[ii, jj], np.where(np.isnan(w), 0.0, w), rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(R, R.T)
+    S, valid = O.sigma_d2_matrix(h)
+    s2, _ = O.sigma_d2_hap_from_counts(c)
+    np.testing.assert_allclose(S[ii, jj], s2, rtol=1e-13, atol=1e-15)
+    assert O.zns_from_matrix(R) == pytest.approx(2 * R[ii, jj].sum() / (12 * 11))
+    # Omega by brute force over pairs
+    m = 12
+    best = 0.0
+    for l in range(3, m - 1):
+        w_s = w_n = c_s = c_n = 0
+        for a, b in zip(ii, jj):
+            if (a < l) == (b < l):
+                w_s += R[a, b]
+                w_n += 1
+            else:
+                c_s += R[a, b]
+                c_n += 1
+        if c_s > 0:
+            best = max(best, (w_s / w_n) / (c_s / c_n))
+    assert O.omega_from_matrix(R) == pytest.approx(best, rel=1e-12)
+    assert list(O.segregating_sites(np.array([[0, 1, -1], [0, 0, 1], [0, 1, 1]]))) == [1]
