@@ -226,8 +226,14 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                                _tile_r2_naive, ld_statistics.py:301-308); 0 where undefined */
 #define PGLD_EST_SIGMA_D2 2 /* unbiased D^2 / pi_2 (_tile_sigma_d2, ld_statistics.py:311-341);
                                0 unless n >= 4 and pi_2 != 0 */
+#define PGLD_EST_RH_R 3     /* genotype planes, no missing data: signed Rogers-Huff r of 0/1/2 dosages
+                               (_tile_rogers_huff_r, ld_statistics.py:410-443); NaN for a constant site */
+#define PGLD_EST_DIP_R2 4   /* genotype planes: dosage r^2 about site means over valid samples
+                               (_r2_matrix_diploid, ld_statistics.py:975-1021); 0 where undefined */
 
-/* pgld_band_r2 with a choice of estimator and two more optional outputs:
+/* pgld_band_r2 with a choice of estimator and two more optional outputs
+ * (planes / marg are haplotype planes for estimators 0-2, genotype planes
+ * for 3-4):
  * values     FP64 per band pair in (i, j) order (as r2_out);
  * snp_sums   FP64 (2 * n_var): [v] = sum over pairs (v, j > v), [n_var + v] =
  *            sum over pairs (i < v, v) of the estimator (NaN pairs skipped);

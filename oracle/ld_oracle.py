@@ -1073,3 +1073,23 @@ def omega_from_matrix(M):
         if n_w > 0 and n_c > 0 and cross > 0:
             best = max(best, ((left + right) / n_w) / (cross / n_c))
     return float(best)
+
+
+def dosage_r2_matrix(geno):
+    """_r2_matrix_diploid (ld_statistics.py:975-1021): dosages centred on
+    each site's mean over its valid samples, missing entries zeroed after
+    centring, r^2 = cov^2 / (var_i var_j), 0 where that denominator is 0;
+    diagonal 0."""
+    g = np.asarray(geno)
+    v = (g >= 0).astype(np.float64)
+    gc = np.where(g >= 0, g, 0).astype(np.float64)
+    nv = v.sum(axis=0)
+    mean = np.where(nv > 0, gc.sum(axis=0) / np.where(nv > 0, nv, 1.0), 0.0)
+    gn = (gc - mean[None, :]) * v
+    var = (gn ** 2).sum(axis=0)
+    cov = gn.T @ gn
+    den = np.sqrt(np.outer(var, var))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r2 = np.where(den > 0, (cov / np.where(den > 0, den, 1.0)) ** 2, 0.0)
+    np.fill_diagonal(r2, 0.0)
+    return r2

@@ -184,8 +184,10 @@ class BandPlan:
         through the tiled counts (pgld_band_ld_pairs / pgld_band_r2_bits).
 
         estimator: "r2" (from the pair's counts, NaN where undefined),
-        "naive" (per-site frequencies, 0 where undefined, as pairwise_r2) or
-        "sigma_d2" (unbiased D^2 / pi_2).  Returns a dict with
+        "naive" (per-site frequencies, 0 where undefined, as pairwise_r2),
+        "sigma_d2" (unbiased D^2 / pi_2); for a genotype population
+        "rogers_huff_r" (signed dosage r, no missing data) or "dosage_r2"
+        (dosage r^2 about site means).  Returns a dict with
           hist      int64 (n_keys, n_hist) pairs per bin [b/n, (b+1)/n),
           nan_pairs int64 (n_keys,) pairs whose value is NaN,
           sparse_i, sparse_j, sparse_r2  pairs with value > thresh, sorted by
@@ -194,15 +196,18 @@ class BandPlan:
           r2        FP64 per band pair in (i, j) order if per_pair,
           row_sums, col_sums, row_counts, col_counts  per-SNP sums over the
                     pairs (v, j > v) and (i < v, v) if snp_sums."""
-        if pop.family != L.FAMILY_HAP:
-            raise ValueError("r^2 is defined here for haplotype populations")
+        geno_est = estimator in ("rogers_huff_r", "dosage_r2")
+        if (pop.family == L.FAMILY_GENO) != geno_est:
+            raise ValueError("estimators 'r2', 'naive', 'sigma_d2' take haplotype planes; 'rogers_huff_r' and "
+                             "'dosage_r2' take genotype planes")
         if with_missing is None:
             with_missing = pop.has_missing
             if with_missing is None:
                 raise ValueError("with_missing must be given for populations packed with validate=False")
-        est = {"r2": L.EST_R2, "naive": L.EST_R2_NAIVE, "sigma_d2": L.EST_SIGMA_D2}.get(estimator)
+        est = {"r2": L.EST_R2, "naive": L.EST_R2_NAIVE, "sigma_d2": L.EST_SIGMA_D2, "rogers_huff_r": L.EST_RH_R,
+               "dosage_r2": L.EST_DIP_R2}.get(estimator)
         if est is None:
-            raise ValueError("estimator is 'r2', 'naive' or 'sigma_d2'")
+            raise ValueError("estimator is 'r2', 'naive', 'sigma_d2', 'rogers_huff_r' or 'dosage_r2'")
         if method not in ("mfma", "popcount"):
             raise ValueError("method is 'mfma' or 'popcount'")
         if method == "popcount" and (est != L.EST_R2 or snp_sums):

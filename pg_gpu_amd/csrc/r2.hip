@@ -46,15 +46,17 @@ enum R2Est : int32_t {
   EST_R2 = 0,        // r^2 from the pair's own counts (ld_statistics.r_squared); NaN where undefined
   EST_R2_NAIVE = 1,  // frequency r^2 with per-site p over all valid samples (_tile_r2_naive); 0 where undefined
   EST_SIGMA_D2 = 2,  // unbiased D^2 / pi_2 (_tile_sigma_d2); 0 unless n >= 4 and pi_2 != 0
+  EST_RH_R = 3,      // genotypes: signed Rogers-Huff r of 0/1/2 dosages; NaN where a site is constant
+  EST_DIP_R2 = 4,    // genotypes: dosage r^2 over valid samples, site means (_r2_matrix_diploid); 0 where undefined
 };
 
 constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
 
-template <bool MISSING, bool POPC, int EST>
+template <int FAMILY, bool MISSING, bool POPC, int EST>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const int8_t* __restrict__ planes,
                                                              const int32_t* __restrict__ marg, int64_t k_pad,
                                                              int32_t S) {
-  using PL = Planes<PGLD_FAMILY_HAP, MISSING>;
+  using PL = Planes<FAMILY, MISSING>;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
   __shared__ int s_wkey[BT];
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   const bool wave_live = (int64_t)rj < ra.n_var && rj + 31 > ri && (int64_t)ri < ra.n_var &&
                          s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ra.max_dist;
   WaveAcc<PL::N> acc;
-  block_products<PGLD_FAMILY_HAP, MISSING, POPC>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
+  block_products<FAMILY, MISSING, POPC>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
                                                  acc);
 
   if (wave_live) {
@@ -109,7 +111,70 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
       bool hit = false;
       double sval = 0.0;
       int scnt = 0;
-      if (valid) {
+      if (valid && FAMILY == PGLD_FAMILY_GENO) {
+        // dosage g = e1 + 2 e2; pair-valid sums from the plane products,
+        // site sums from the per-SNP plane sums [v, e1, e2]
+        int64_t gg, gib, gjb, nb;
+        if constexpr (MISSING) {
+          gg = (int64_t)acc.c[1][1][r] + 2 * (int64_t)acc.c[1][2][r] + 2 * (int64_t)acc.c[2][1][r] +
+               4 * (int64_t)acc.c[2][2][r];
+          gib = (int64_t)acc.c[1][0][r] + 2 * (int64_t)acc.c[2][0][r];  // g_i over samples valid at j
+          gjb = (int64_t)acc.c[0][1][r] + 2 * (int64_t)acc.c[0][2][r];
+          nb = acc.c[0][0][r];
+        } else {
+          gg = (int64_t)acc.c[0][0][r] + 2 * (int64_t)acc.c[0][1][r] + 2 * (int64_t)acc.c[1][0][r] +
+               4 * (int64_t)acc.c[1][1][r];
+          gib = 0;
+          gjb = 0;
+          nb = S;
+        }
+        const int64_t si = (int64_t)marg[ra.n_var + i] + 2 * (int64_t)marg[2 * ra.n_var + i];
+        const int64_t sj = (int64_t)marg[ra.n_var + j] + 2 * (int64_t)marg[2 * ra.n_var + j];
+        const int64_t qi = (int64_t)marg[ra.n_var + i] + 4 * (int64_t)marg[2 * ra.n_var + i];
+        const int64_t qj = (int64_t)marg[ra.n_var + j] + 4 * (int64_t)marg[2 * ra.n_var + j];
+        const int64_t ni = MISSING ? (int64_t)marg[i] : (int64_t)S, nj = MISSING ? (int64_t)marg[j] : (int64_t)S;
+        if constexpr (!MISSING) {
+          gib = si;
+          gjb = sj;
+        }
+        double val;
+        bool is_nan = false;
+        if constexpr (EST == EST_RH_R) {
+          // n cov = n sum(g_i g_j) - sum(g_i) sum(g_j); n ssd = n sum(g^2) - sum(g)^2 (exact)
+          const double cn = (double)(nb * gg - si * sj);
+          const double den = (double)(nb * qi - si * si) * (double)(nb * qj - sj * sj);
+          is_nan = !(den > 0.0);
+          val = den > 0.0 ? cn / sqrt(den) : __builtin_nan("");
+        } else {
+          // (cov n_i n_j)^2 / ((var_i n_i)(var_j n_j) n_i n_j), cov and var about the site means
+          const int64_t c = gg * ni * nj - sj * gib * ni - si * gjb * nj + si * sj * nb;
+          const int64_t vi = ni * qi - si * si, vj = nj * qj - sj * sj;
+          const double den = (double)vi * (double)vj * (double)ni * (double)nj;
+          const double cd = (double)c;
+          val = (vi > 0 && vj > 0 && ni > 0 && nj > 0) ? (cd * cd) / den : 0.0;
+        }
+        r2f = (float)val;
+        sval = is_nan ? 0.0 : val;
+        scnt = !is_nan;
+        if (ra.r2_out) ra.r2_out[ra.pair_base[i] + (j - i - 1)] = val;
+        hit = !is_nan && val > ra.thresh;
+        if (ra.hist) {
+          int key;
+          if (ra.key_mode == PGLD_KEY_WINDOW) {
+            key = s_wkey[il];
+          } else {
+            double d = s_kc_j[jl] - s_kc_i[il];
+            if (ra.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
+            key = key_bin(d, ra.edges, ra.n_keys);
+          }
+          const int slot = key - tile.key_lo;
+          if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
+            int b = ra.n_hist;
+            if (!is_nan) b = max(0, min((int)(val * (double)ra.n_hist), ra.n_hist - 1));
+            atomicAdd(&s_hist[slot * hw + b], 1);
+          }
+        }
+      } else if (valid && FAMILY == PGLD_FAMILY_HAP) {
         int64_t n11 = acc.c[0][0][r], nA, nB, n;
         if constexpr (MISSING) {
           nA = acc.c[0][1][r];
@@ -172,7 +237,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           const int slot = key - tile.key_lo;
           if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
             int b = ra.n_hist;
-            if (!is_nan) b = min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1);
+            if (!is_nan) b = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
             atomicAdd(&s_hist[slot * hw + b], 1);
           }
         }
@@ -287,8 +352,10 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
                    int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
                    uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
                    uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream) {
-  PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_SIGMA_D2, "pgld_band_ld_pairs: unknown estimator %d",
+  PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_DIP_R2, "pgld_band_ld_pairs: unknown estimator %d",
                estimator);
+  PGLD_REQUIRE(!(estimator == EST_RH_R && with_missing),
+               "pgld_band_ld_pairs: Rogers-Huff r needs genotypes without missing values");
   PGLD_REQUIRE(plan && planes && marg, "pgld_band_r2: null pointer");
   PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
   PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= n_samples && n_samples >= 0,
@@ -344,17 +411,25 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
     ra.snp_cnt = pl->d_snp_cnt;
   }
   const bool miss = with_missing != 0;
-  const size_t lds = miss ? (size_t)opbuf_bytes<2>() : (size_t)opbuf_bytes<1>();
+  const bool geno = estimator == EST_RH_R || estimator == EST_DIP_R2;
+  const int npl = geno ? (miss ? 3 : 2) : (miss ? 2 : 1);
+  const size_t lds = (size_t)2 * npl * BT * OP_ROW;
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[0], s), "pgld_band_r2 event");
-#define PGLD_R2(M, B, E) hipLaunchKernelGGL((k_band_r2<M, B, E>), g, b, lds, s, ra, planes, marg, k_pad, n_samples)
-#define PGLD_R2_EST(M, B)                              \
-  do {                                                 \
-    if (estimator == EST_R2) PGLD_R2(M, B, EST_R2);    \
-    else if (estimator == EST_R2_NAIVE) PGLD_R2(M, B, EST_R2_NAIVE); \
-    else PGLD_R2(M, B, EST_SIGMA_D2);                  \
+#define PGLD_R2(F, M, B, E) \
+  hipLaunchKernelGGL((k_band_r2<F, M, B, E>), g, b, lds, s, ra, planes, marg, k_pad, n_samples)
+#define PGLD_R2_EST(M, B)                                                        \
+  do {                                                                           \
+    if (estimator == EST_R2) PGLD_R2(PGLD_FAMILY_HAP, M, B, EST_R2);             \
+    else if (estimator == EST_R2_NAIVE) PGLD_R2(PGLD_FAMILY_HAP, M, B, EST_R2_NAIVE); \
+    else PGLD_R2(PGLD_FAMILY_HAP, M, B, EST_SIGMA_D2);                           \
   } while (0)
-  if (popc) {
+  if (estimator == EST_RH_R) {
+    PGLD_R2(PGLD_FAMILY_GENO, false, false, EST_RH_R);
+  } else if (estimator == EST_DIP_R2) {
+    if (miss) PGLD_R2(PGLD_FAMILY_GENO, true, false, EST_DIP_R2);
+    else PGLD_R2(PGLD_FAMILY_GENO, false, false, EST_DIP_R2);
+  } else if (popc) {
     if (miss) PGLD_R2_EST(true, true); else PGLD_R2_EST(false, true);
   } else {
     if (miss) PGLD_R2_EST(true, false); else PGLD_R2_EST(false, false);

@@ -358,12 +358,21 @@ class HaplotypeMatrix:
         tensor: per-site frequencies over all valid samples, joint frequency
         over the pair's valid samples, 0 where undefined, zero diagonal
         (haplotype_matrix.py:1139-1170).  Counts through the MFMA tiles."""
-        if estimator != "r2":
-            raise NotImplementedError("only the 'r2' estimator is in this build") if estimator == "rogers_huff" \
-                else ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+        if estimator not in ("r2", "rogers_huff"):
+            raise ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
         self.transfer_to_gpu()
         T = L.torch()
         m = self.num_variants
+        if estimator == "rogers_huff":
+            from .ld_statistics import rogers_huff_r
+            r = rogers_huff_r(self)
+            out = T.zeros((m, m), dtype=T.float64, device=L.device())
+            if m >= 2:
+                v = T.nan_to_num(r * r, nan=0.0)
+                iu = T.triu_indices(m, m, offset=1, device=out.device)
+                out[iu[0], iu[1]] = v
+                out[iu[1], iu[0]] = v
+            return out
         out = T.zeros((m, m), dtype=T.float64, device=L.device())
         if m < 2:
             return out
@@ -381,9 +390,11 @@ class HaplotypeMatrix:
         (haplotype_matrix.py:1233-1317).  r^2 of the pairs within the last
         edge come from the MFMA tiles; the percentiles are taken on the
         host, as the reference does."""
-        if estimator != "r2":
-            raise NotImplementedError("only the 'r2' estimator is in this build") if estimator == "rogers_huff" \
-                else ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+        if estimator not in ("r2", "rogers_huff"):
+            raise ValueError(f"Unknown estimator: {estimator!r} (expected 'r2' or 'rogers_huff')")
+        if estimator == "rogers_huff" and pop is not None:
+            raise NotImplementedError("windowed_r_squared(estimator='rogers_huff', pop=...) is not yet implemented; "
+                                      "pass the full matrix or subset to the desired haplotypes first.")
         self.transfer_to_gpu()
         bins = np.asarray(bp_bins, dtype=np.float64)
         n_bins = len(bins) - 1
@@ -394,8 +405,14 @@ class HaplotypeMatrix:
             rows = self._sample_sets[pop]
         pos = self._pos_np().astype(np.float64)
         plan = B.BandPlan(pos, float(bins[-1]), edges=bins)
-        pk = B.pack_population(self._haplotypes, rows, "hap")
-        vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="r2")["r2"]
+        if estimator == "rogers_huff":
+            from .ld_statistics import _dosage_matrix
+            pk = B.pack_population(_dosage_matrix(self).genotypes, None, "geno")
+            r = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="rogers_huff_r")["r2"]
+            vals = r * r
+        else:
+            pk = B.pack_population(self._haplotypes, rows, "hap")
+            vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="r2")["r2"]
         keys = L.empty((plan.n_pairs,), np.int32)
         pos_d = L.to_device(pos, np.float64)
         edges_d = L.to_device(bins, np.float64)

@@ -181,6 +181,17 @@ def _segregating(mat, missing_data):
     return mat
 
 
+def _genotype_sites(gm, missing_data):
+    """GenotypeMatrix sites for ZnS / Omega: all sites, or with
+    missing_data='exclude' those without missing genotypes
+    (ld_statistics.py:940-951); no segregating filter, as the reference."""
+    gm.transfer_to_gpu()
+    if missing_data == "exclude":
+        alt, valid = (x.cpu().numpy() for x in gm.allele_counts())
+        gm = gm.get_subset(np.nonzero(valid == gm.num_individuals)[0])
+    return gm
+
+
 def _snp_sums(mat, estimator):
     """Per-SNP row / column sums of the estimator over all pairs of the
     matrix (one all-pairs band, MFMA tiles)."""
@@ -216,7 +227,14 @@ def zns(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
             return float(s["row_sums"].sum() / n) if n else 0.0
         return float(2.0 * s["row_sums"].sum() / (m * (m - 1)))
     if isinstance(r2_matrix_or_matrix, GenotypeMatrix):
-        raise NotImplementedError("ZnS of a GenotypeMatrix (dosage correlation) is outside this build")
+        if est == "sigma_d2":
+            raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix, not a pre-computed r² array")
+        gm = _genotype_sites(r2_matrix_or_matrix, missing_data)
+        m = gm.num_variants
+        if m < 2:
+            return 0.0
+        s = _all_pairs_values(gm, "dosage_r2", per_pair=False, snp_sums=True)
+        return float(2.0 * s["row_sums"].sum().item() / (m * (m - 1)))
     if est == "sigma_d2":
         raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix, not a pre-computed r² array")
     M = np.asarray(L.to_numpy(r2_matrix_or_matrix), dtype=np.float64)
@@ -269,8 +287,69 @@ def omega(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
         s = _snp_sums(mat, "sigma_d2" if est == "sigma_d2" else "naive")
         return _omega_from_sums(s["row_sums"], s["col_sums"])
     if isinstance(r2_matrix_or_matrix, GenotypeMatrix):
-        raise NotImplementedError("Omega of a GenotypeMatrix (dosage correlation) is outside this build")
+        if est == "sigma_d2":
+            raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix")
+        gm = _genotype_sites(r2_matrix_or_matrix, missing_data)
+        if gm.num_variants < 5:
+            return 0.0
+        s = _all_pairs_values(gm, "dosage_r2", per_pair=False, snp_sums=True)
+        return _omega_from_sums(s["row_sums"].cpu().numpy(), s["col_sums"].cpu().numpy())
     if est == "sigma_d2":
         raise ValueError("estimator='sigma_d2' requires a HaplotypeMatrix")
     M = np.triu(np.asarray(L.to_numpy(r2_matrix_or_matrix), dtype=np.float64), k=1)
     return _omega_from_sums(M.sum(axis=1), M.sum(axis=0))
+
+
+# ---------------------------------------------------------------------------
+# Rogers-Huff r of diploid dosages (ld_statistics.py:366-531)
+# ---------------------------------------------------------------------------
+
+def _dosage_matrix(matrix):
+    """GenotypeMatrix of 0/1/2 dosages: a HaplotypeMatrix is paired by
+    adjacent rows (haplotypes 2s, 2s + 1); missing values and an odd
+    haplotype count raise ValueError, other inputs TypeError
+    (ld_statistics.py:366-407)."""
+    from .genotype_matrix import GenotypeMatrix
+    from .haplotype_matrix import HaplotypeMatrix
+    if isinstance(matrix, HaplotypeMatrix):
+        matrix.transfer_to_gpu()
+        if matrix.count_missing() > 0:
+            raise ValueError("rogers_huff_r: input HaplotypeMatrix contains missing values (-1). Rogers-Huff r "
+                             "expects strict 0/1/2 dosage input; drop or impute missing sites first.")
+        if matrix.num_haplotypes % 2 != 0:
+            raise ValueError(f"rogers_huff_r: HaplotypeMatrix has an odd number of haplotypes "
+                             f"({matrix.num_haplotypes}); cannot pair into diploids.")
+        return GenotypeMatrix.from_haplotype_matrix(matrix)
+    if isinstance(matrix, GenotypeMatrix):
+        matrix.transfer_to_gpu()
+        if bool((matrix.genotypes < 0).any()):
+            raise ValueError("rogers_huff_r: input GenotypeMatrix contains missing values (-1). Rogers-Huff r "
+                             "expects strict 0/1/2 dosage input; drop or impute missing sites first.")
+        return matrix
+    raise TypeError(f"rogers_huff_r: expected HaplotypeMatrix or GenotypeMatrix; got {type(matrix).__name__}")
+
+
+def _all_pairs_values(gm, estimator, per_pair=True, snp_sums=False):
+    from . import band as B
+    m = gm.num_variants
+    plan = B.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
+    pk = B.pack_population(gm.genotypes, None, "geno")
+    return plan.r2(pk, histogram=False, sparse_cap=0, per_pair=per_pair, estimator=estimator, snp_sums=snp_sums)
+
+
+def rogers_huff_r(matrix, tile_size: Optional[int] = None):
+    """Signed Rogers-Huff (2008) r of every variant pair, condensed upper
+    triangle (0,1), (0,2), ..., (n-2,n-1); NaN where either site is constant
+    (ld_statistics.py:492-521).  The dosage products come from the MFMA tiles
+    on genotype planes and r from exact integer sums; tile_size is accepted
+    for signature compatibility."""
+    gm = _dosage_matrix(matrix)
+    if gm.num_variants < 2:
+        return L.torch().empty((0,), dtype=L.torch().float64, device=L.device())
+    return _all_pairs_values(gm, "rogers_huff_r")["r2"]
+
+
+def rogers_huff_r_squared(matrix, tile_size: Optional[int] = None):
+    """rogers_huff_r(matrix) ** 2 (ld_statistics.py:524-531)."""
+    r = rogers_huff_r(matrix, tile_size)
+    return r * r

@@ -313,3 +313,60 @@ def test_windowed_r_squared():
         np.testing.assert_allclose(res[b], np.percentile(sel, [10, 50, 90]), rtol=1e-12, atol=1e-15)
     res1, _ = hm.windowed_r_squared(bins)
     assert res1.shape == (4,)
+
+
+def test_rogers_huff_r():
+    rng = np.random.default_rng(71)
+    g = SY.hwe_genotypes(45, 80, rng)
+    g[:, 3] = 1  # constant site -> NaN
+    gm = GenotypeMatrix(g, np.arange(80) * 10)
+    got = np_(LS.rogers_huff_r(gm))
+    want = O.rogers_huff_r(g)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_allclose(got[ok], want[ok], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(np_(LS.rogers_huff_r_squared(gm))[ok], want[ok] ** 2, rtol=1e-12, atol=1e-14)
+    h = SY.neutral_haplotypes(60, 50, rng)
+    hm = HaplotypeMatrix(h, np.arange(50) * 10)
+    w2 = O.rogers_huff_r(h[0::2] + h[1::2])
+    g2 = np_(LS.rogers_huff_r(hm))
+    np.testing.assert_array_equal(np.isnan(g2), np.isnan(w2))
+    np.testing.assert_allclose(g2[~np.isnan(w2)], w2[~np.isnan(w2)], rtol=1e-12, atol=1e-14)
+    full = np_(hm.pairwise_r2(estimator="rogers_huff"))
+    ii, jj = np.triu_indices(50, k=1)
+    np.testing.assert_allclose(full[ii, jj], np.nan_to_num(w2 ** 2), rtol=1e-12, atol=1e-14)
+    assert np.all(np.diag(full) == 0)
+    bad = h.copy()
+    bad[0, 0] = -1
+    with pytest.raises(ValueError):
+        LS.rogers_huff_r(HaplotypeMatrix(bad, np.arange(50) * 10))
+    with pytest.raises(ValueError):
+        LS.rogers_huff_r(HaplotypeMatrix(h[:59], np.arange(50) * 10))
+    with pytest.raises(TypeError):
+        LS.rogers_huff_r(g)
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.05])
+def test_genotype_zns_omega(missing):
+    rng = np.random.default_rng(72)
+    g = SY.hwe_genotypes(50, 90, rng, missing=missing)
+    gm = GenotypeMatrix(g, np.arange(90) * 10)
+    R = O.dosage_r2_matrix(g)
+    assert LS.zns(gm) == pytest.approx(O.zns_from_matrix(R), rel=1e-10)
+    assert LS.omega(gm) == pytest.approx(O.omega_from_matrix(R), rel=1e-10)
+
+
+def test_windowed_r_squared_rogers_huff():
+    rng = np.random.default_rng(73)
+    h = SY.neutral_haplotypes(60, 200, rng)
+    pos = SY.uniform_positions(200, 40000, rng)
+    hm = HaplotypeMatrix(h, pos)
+    bins = [0, 1000, 5000, 20000]
+    res, cnt = hm.windowed_r_squared(bins, percentile=[25, 75], estimator="rogers_huff")
+    ii, jj = np.triu_indices(200, k=1)
+    r2 = O.rogers_huff_r(h[0::2] + h[1::2]) ** 2
+    k = O.bin_index(pos[jj] - pos[ii], np.array(bins, dtype=float))
+    for b in range(3):
+        sel = r2[(k == b) & ~np.isnan(r2)]
+        assert cnt[b] == len(sel)
+        np.testing.assert_allclose(res[b], np.percentile(sel, [25, 75]), rtol=1e-11, atol=1e-14)

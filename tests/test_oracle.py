@@ -292,3 +292,18 @@ def test_naive_r2_and_sigma_matrices_pins():
             best = max(best, (w_s / w_n) / (c_s / c_n))
     assert O.omega_from_matrix(R) == pytest.approx(best, rel=1e-12)
     assert list(O.segregating_sites(np.array([[0, 1, -1], [0, 0, 1], [0, 1, 1]]))) == [1]
+
+
+def test_dosage_r2_equals_rogers_huff_without_missing():
+    """Without missing data the dosage r^2 of the ZnS path is the square of
+    the Rogers-Huff r (both are Pearson correlations of the same dosages),
+    and Rogers-Huff r of a site with itself shifted is 1."""
+    rng = np.random.default_rng(23)
+    g = rng.integers(0, 3, size=(30, 9)).astype(np.int8)
+    g[:, 4] = 1  # a constant site: r undefined, r^2 reported as 0
+    r = O.rogers_huff_r(g)
+    R2 = O.dosage_r2_matrix(g)
+    ii, jj = np.triu_indices(9, k=1)
+    np.testing.assert_allclose(R2[ii, jj], np.nan_to_num(r ** 2), rtol=1e-12, atol=1e-15)
+    h = np.column_stack([g[:, 0], g[:, 0]])
+    assert O.rogers_huff_r(h)[0] == pytest.approx(1.0)
