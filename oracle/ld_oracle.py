@@ -1093,3 +1093,24 @@ def dosage_r2_matrix(geno):
         r2 = np.where(den > 0, (cov / np.where(den > 0, den, 1.0)) ** 2, 0.0)
     np.fill_diagonal(r2, 0.0)
     return r2
+
+
+def locate_unlinked(hap, size=100, step=20, threshold=0.1):
+    """Greedy sliding-window LD pruning of haplotype_matrix.py:1172-1231 on
+    the naive r^2 of the window's still-unlinked variants."""
+    h = np.asarray(hap)
+    m = h.shape[1]
+    loc = np.ones(m, dtype=bool)
+    for w_start in range(0, m, step):
+        w_end = min(w_start + size, m)
+        active = np.nonzero(loc[w_start:w_end])[0] + w_start
+        if len(active) <= 1:
+            continue
+        r2 = naive_r2_matrix(h[:, active])
+        for i in range(len(active)):
+            if not loc[active[i]]:
+                continue
+            for j in range(i + 1, len(active)):
+                if loc[active[j]] and r2[i, j] > threshold:
+                    loc[active[j]] = False
+    return loc

@@ -384,6 +384,31 @@ class HaplotypeMatrix:
         out[iu[1], iu[0]] = vals
         return out
 
+    def locate_unlinked(self, size=100, step=20, threshold=0.1):
+        """Variants in approximate linkage equilibrium: windows of `size`
+        variants every `step`; within a window, walking the still-unlinked
+        variants in order, each one drops every later variant whose
+        frequency r^2 with it exceeds `threshold` (haplotype_matrix.py:
+        1172-1231).  The window's r^2 matrix comes from the MFMA tiles; the
+        greedy pruning runs on the host, as in the reference."""
+        self.transfer_to_gpu()
+        m = self.num_variants
+        loc = np.ones(m, dtype=bool)
+        for w_start in range(0, m, step):
+            w_end = min(w_start + size, m)
+            active_idx = np.nonzero(loc[w_start:w_end])[0] + w_start
+            if len(active_idx) <= 1:
+                continue
+            r2 = self.get_subset(active_idx).pairwise_r2().cpu().numpy()
+            n_active = len(active_idx)
+            for i in range(n_active):
+                if not loc[active_idx[i]]:
+                    continue
+                for j in range(i + 1, n_active):
+                    if loc[active_idx[j]] and r2[i, j] > threshold:
+                        loc[active_idx[j]] = False
+        return loc
+
     def windowed_r_squared(self, bp_bins, percentile=50, pop=None, estimator: str = "r2"):
         """Percentiles of r^2 (from each pair's counts; NaN pairs excluded)
         per distance bin [b_k, b_{k+1}), and the pair count per bin

@@ -370,3 +370,15 @@ def test_windowed_r_squared_rogers_huff():
         sel = r2[(k == b) & ~np.isnan(r2)]
         assert cnt[b] == len(sel)
         np.testing.assert_allclose(res[b], np.percentile(sel, [25, 75]), rtol=1e-11, atol=1e-14)
+
+
+def test_locate_unlinked():
+    rng = np.random.default_rng(74)
+    base = SY.neutral_haplotypes(50, 40, rng, missing=0.02)
+    h = np.repeat(base, 3, axis=1)  # runs of identical sites: strongly linked
+    h[:, 1::3] = SY.neutral_haplotypes(50, 40, rng, missing=0.02)
+    hm = HaplotypeMatrix(h, np.arange(120) * 50)
+    for size, step, thr in ((30, 10, 0.1), (12, 5, 0.5)):
+        got = hm.locate_unlinked(size=size, step=step, threshold=thr)
+        np.testing.assert_array_equal(got, O.locate_unlinked(h, size, step, thr))
+    assert not got.all() and got.any()
