@@ -289,8 +289,10 @@ int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_
  * per key.  Allocates nothing, records no memset: may be captured. */
 int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream);
 /* Schedule of the persistent launch: grid workgroups (0 = one per tile),
- * order 0 = cost-descending tiles, 1 = natural; xcd_split = each XCD walks a
- * contiguous eighth of the order. */
+ * order 0 = cost-descending tiles, 1 = natural, 2 = L2-blocked (8 x 8 tile
+ * blocks of one region dealt to the 8 XCDs by work and interleaved so that
+ * position p runs on XCD p % 8; use grid 0 or a multiple of 8); xcd_split =
+ * each XCD walks a contiguous eighth of the order. */
 int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split);
 /* Autotuner: time the schedule candidates on this plan, keep the fastest
  * (also cached per shape class for the process); outputs may be NULL;

@@ -101,8 +101,14 @@ class RegionBatch:
                stream if stream is not None else L.stream_handle())
         return (out, key_pairs) if (want_pairs or key_pairs is not None) else out
 
+    _ORDERS = ("cost", "natural", "blocked")
+
     def schedule(self, grid=0, order="cost", xcd_split=False):
-        L.call("pgld_batch_schedule", self._h, int(grid), 0 if order == "cost" else 1, int(bool(xcd_split)))
+        """grid 0 = one workgroup per tile; order "cost" (descending work),
+        "natural" or "blocked" (8 x 8 tile blocks per XCD for L2 reuse)."""
+        if order not in self._ORDERS:
+            raise ValueError(f"order must be one of {self._ORDERS}")
+        L.call("pgld_batch_schedule", self._h, int(grid), self._ORDERS.index(order), int(bool(xcd_split)))
 
     def autotune(self, with_missing):
         """Time the schedule candidates, keep the fastest; returns the choice."""
@@ -110,7 +116,7 @@ class RegionBatch:
         g, o, x, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
         L.call("pgld_batch_autotune", self._h, int(bool(with_missing)), L.ptr(out), None, L.stream_handle(),
                C.byref(g), C.byref(o), C.byref(x), C.byref(ms))
-        return {"grid": g.value, "order": "cost" if o.value == 0 else "natural", "xcd_split": bool(x.value),
+        return {"grid": g.value, "order": self._ORDERS[o.value], "xcd_split": bool(x.value),
                 "ms": None if ms.value < 0 else float(ms.value)}
 
     def set_timing(self, enable=True):

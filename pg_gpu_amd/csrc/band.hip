@@ -25,6 +25,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "band_core.h"
@@ -135,8 +136,16 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   const bool edges_in_lds = ba.key_mode != PGLD_KEY_WINDOW && ba.n_keys < MAX_LDS_EDGES;
   if (edges_in_lds)
     for (int t = tid; t <= ba.n_keys; t += BAND_THREADS) s_edges[t] = ba.edges[t];
+  // Every band pair of the tile has its bin in the tile's key range [klo, khi]
+  // (the plan bounds it by the tile's smallest and largest distance) or none:
+  // only those bins are searched, and a one-bin tile needs two compares.
+  const int klo = tile.key_lo - key_base, khi = klo + tile.n_slots - 1;
+  double e_lo = 0.0, e_hi = 0.0;
+  if (ba.key_mode != PGLD_KEY_WINDOW) {
+    e_lo = ba.edges[klo];
+    e_hi = ba.edges[khi + 1];
+  }
   __syncthreads();
-  const double* edges = edges_in_lds ? s_edges : ba.edges;
 
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int ri = tile.i0 + wr, rj = tile.j0 + wc;
@@ -146,57 +155,46 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   bool wave_live = (int64_t)rj < ba.n_var && rj + 31 > ri && (int64_t)ri < ba.n_var &&
                    s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ba.max_dist;
 
-  unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
-  WaveAcc<PL::N> acc;
-#pragma unroll 1
-  for (int p = 0; p < P; ++p) {
-    block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live, opbuf,
-                                    acc);
-    if (p == P - 1 || !wave_live) continue;
-    // all populations but the last park their counts in LDS as int16
-    const int64_t j = rj + (lane & 31);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int64_t i = ri + acc_row(r, lane);
-      int c[NC];
-      elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(j, ba.n_var - 1),
-                                   pa.n_samples[p], c);
-#pragma unroll
-      for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
-    }
-  }
-
-  // ---- epilogue: statistics and keyed running sums ----
-  double lacc[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) lacc[s] = 0.0;
-  int ck = -1;
   const int jl = wc + (lane & 31);
   const int64_t j = (int64_t)tile.j0 + jl;
   double* wred = red + w * tile.n_slots * NS;
 
-  auto flush = [&](bool flag) {
-    unsigned long long part = __ballot(flag);
-    while (part) {
-      const int leader = __ffsll((long long)part) - 1;
-      const int kk = __shfl(ck, leader);
-      const bool mine = flag && ck == kk;
-      part &= ~__ballot(mine);
-      const int slot = kk - tile.key_lo;
+  // Walks the wave's 32 x 32 sub-tile in accumulator order, evaluates M
+  // statistics of every band pair with eval(r, i, out), keys the pair and keeps
+  // per-lane running sums that are folded into this wave's LDS row at key
+  // changes: statistic s goes to column col0 + s * cstride, the pair count (if
+  // `count`) to the last column.
+  auto epilogue = [&](auto m_tag, int col0, int cstride, bool count, auto&& eval) {
+    constexpr int M = decltype(m_tag)::value;
+    double lacc[M + 1];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        double x = wave_sum(mine ? lacc[s] : 0.0);
-        if (lane == 0 && slot >= 0 && slot < tile.n_slots) wred[slot * NS + s] += x;
+    for (int s = 0; s <= M; ++s) lacc[s] = 0.0;
+    int ck = -1;
+    auto flush = [&](bool flag) {
+      unsigned long long part = __ballot(flag);
+      while (part) {
+        const int leader = __ffsll((long long)part) - 1;
+        const int kk = __shfl(ck, leader);
+        const bool mine = flag && ck == kk;
+        part &= ~__ballot(mine);
+        const int slot = kk - tile.key_lo;
+        const bool own = lane == 0 && slot >= 0 && slot < tile.n_slots;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          double x = wave_sum(mine ? lacc[s] : 0.0);
+          if (own) wred[slot * NS + col0 + s * cstride] += x;
+        }
+        if (count) {
+          double x = wave_sum(mine ? lacc[M] : 0.0);
+          if (own) wred[slot * NS + NS - 1] += x;
+        }
       }
-    }
-    if (flag) {
+      if (flag) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) lacc[s] = 0.0;
-      ck = -1;
-    }
-  };
-
-  if (wave_live) {
+        for (int s = 0; s <= M; ++s) lacc[s] = 0.0;
+        ck = -1;
+      }
+    };
 #pragma unroll 1
     for (int r = 0; r < 16; ++r) {
       const int il = wr + acc_row(r, lane);
@@ -209,7 +207,12 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         } else {
           double d = s_kc_j[jl] - s_kc_i[il];
           if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
-          key = key_bin(d, edges, ba.n_keys);
+          if (!(d >= e_lo) || !(d < e_hi))
+            key = -1;
+          else if (klo == khi)
+            key = klo;
+          else
+            key = edges_in_lds ? key_bin_in(d, s_edges, klo, khi + 1) : key_bin_in(d, ba.edges, klo, khi + 1);
         }
         if (key >= 0) key += key_base;
       }
@@ -218,6 +221,56 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       if (__ballot(need)) flush(need);
       if (has) {
         ck = key;
+        double out[M];
+        eval(r, i, out);
+#pragma unroll
+        for (int s = 0; s < M; ++s) lacc[s] += out[s];
+        lacc[M] += 1.0;
+      }
+    }
+    flush(ck >= 0);
+  };
+
+  WaveAcc<PL::N> acc;
+  if constexpr (SET == PGLD_SET_SELF && P > 1) {
+    // Independent per-population statistics (column s * P + p): each
+    // population's products are followed at once by its own epilogue, so no
+    // counts are parked in LDS and more workgroups fit on a CU.
+    using S1 = StatSet<FAMILY, 1, PGLD_SET_SELF>;
+    unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
+#pragma unroll 1
+    for (int p = 0; p < P; ++p) {
+      block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live,
+                                      opbuf, acc);
+      if (wave_live)
+        epilogue(std::integral_constant<int, S1::N>{}, p, P, p == 0, [&](int r, int64_t i, double* out) {
+          int c[NC];
+          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, i, j, pa.n_samples[p], c);
+          typename PopVal<FAMILY>::T V = PopVal<FAMILY>::make(c);
+          S1::eval(&V, out);
+        });
+    }
+  } else {
+    unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
+#pragma unroll 1
+    for (int p = 0; p < P; ++p) {
+      block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live,
+                                      opbuf, acc);
+      if (p == P - 1 || !wave_live) continue;
+      // all populations but the last park their counts in LDS as int16
+      const int64_t jj = rj + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = ri + acc_row(r, lane);
+        int c[NC];
+        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(jj, ba.n_var - 1),
+                                     pa.n_samples[p], c);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
+      }
+    }
+    if (wave_live)
+      epilogue(std::integral_constant<int, SS::N>{}, 0, 1, true, [&](int r, int64_t i, double* out) {
         typename PopVal<FAMILY>::T V[P];
 #pragma unroll
         for (int p = 0; p < P - 1; ++p) {
@@ -226,19 +279,11 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
           for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
           V[p] = PopVal<FAMILY>::make(c);
         }
-        {
-          int c[NC];
-          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
-          V[P - 1] = PopVal<FAMILY>::make(c);
-        }
-        double out[SS::N];
+        int c[NC];
+        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
+        V[P - 1] = PopVal<FAMILY>::make(c);
         SS::eval(V, out);
-#pragma unroll
-        for (int s = 0; s < SS::N; ++s) lacc[s] += out[s];
-        lacc[NS - 1] += 1.0;
-      }
-    }
-    flush(ck >= 0);
+      });
   }
   __syncthreads();
   // fixed-order sum over the four wave rows into this tile's partial row
@@ -262,10 +307,11 @@ __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void 
 // (positions are concatenated over regions) and its key offset.  The per-key
 // reduction runs over slots in schedule-independent order, so the result does
 // not depend on the grid or the tile order.
-// The persistent loop adds live state; genotype and 4-population instances
-// take the whole register file (their LDS use allows one workgroup per CU).
+// The persistent loop adds live state; genotype and 4-population joint
+// (moments) instances take the whole register file (their LDS use allows one
+// workgroup per CU).
 constexpr int batch_min_waves(int family, int P, int set) {
-  return (family == PGLD_FAMILY_HAP && P <= 3) ? band_min_waves(family, P, set) : 1;
+  return (family == PGLD_FAMILY_HAP && (P <= 3 || set == PGLD_SET_SELF)) ? band_min_waves(family, P, set) : 1;
 }
 
 template <int FAMILY, bool MISSING, int P, int SET>
@@ -284,7 +330,9 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     t_end = n_tiles * (x + 1) / N_XCD;
   }
   for (int64_t t = t0; t < t_end; t += step) {
-    const BandTile tile = ba.tiles[order[t]];
+    const int32_t ot = order[t];
+    if (ot < 0) continue;  // padding of the XCD-blocked order
+    const BandTile tile = ba.tiles[ot];
     if (tile.n_slots <= 0) continue;
     const RegionDesc& rd = regions[tile.region];
     BandArgs lb = ba;
@@ -617,7 +665,8 @@ static int64_t build_region(const double* pos, const double* kc, int64_t n_var, 
       tb.n_slots += t.n_slots;
       tb.max_ts = std::max(tb.max_ts, t.n_slots);
       tb.tiles.push_back(t);
-      tb.host.push_back(BandTileHost{region, (int32_t)(il - i0 + 1), (int32_t)(jl - j0 + 1), (int32_t)(tc == r)});
+      tb.host.push_back(BandTileHost{region, (int32_t)(il - i0 + 1), (int32_t)(jl - j0 + 1), (int32_t)(tc == r),
+                                     (int32_t)r, (int32_t)tc});
     }
   }
   if (base_out) base_out->assign(base.begin(), base.end() - 1);
@@ -680,11 +729,12 @@ template <int FAMILY, bool MISSING, int P, int SET>
 static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
                         hipStream_t s) {
   if (plan->n_regions > 0) {
-    const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, plan->n_tiles) : plan->n_tiles;
+    const int m = plan->order_mode;
+    const int64_t len = plan->order_len[m];
+    const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
     hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)grid), dim3(BAND_THREADS), lds, s,
-                       ba, reinterpret_cast<const RegionDesc*>(plan->d_regions),
-                       plan->lpt ? plan->d_order : plan->d_order + plan->n_tiles, plan->n_tiles,
-                       (int32_t)plan->xcd_split);
+                       ba, reinterpret_cast<const RegionDesc*>(plan->d_regions), plan->d_order + plan->order_off[m],
+                       len, (int32_t)plan->xcd_split);
   } else {
     hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)plan->n_tiles), dim3(BAND_THREADS),
                        lds, s, ba, pa);
@@ -723,7 +773,57 @@ static int n_stats_of(int family, int P, int set) {
 
 }  // namespace pgld
 
+
 using namespace pgld;
+
+// L2-blocked tile order for a region batch.  Workgroup b of a launch runs on
+// XCD b % 8, and each XCD holds about 64 workgroups at once; so the tiles are
+// cut into 8 x 8 blocks of one region (8 A and 8 B operand panels serve 64
+// tiles from one L2), the blocks are dealt to the XCD with the least work so
+// far (regions by descending work per tile, largest first), and the eight
+// per-XCD streams are interleaved position by position.  Streams that run out
+// are padded with -1 so that position p stays on XCD p % 8; the padding is
+// skipped by the kernel.  Persistent grids that are a multiple of 8 keep the
+// same mapping (workgroup b walks positions b, b + grid, ...).
+static std::vector<int32_t> blocked_order(const std::vector<BandTileHost>& h, const std::vector<double>& wsum,
+                                          const std::vector<double>& cost) {
+  constexpr int SB = 8;
+  const int64_t T = (int64_t)h.size();
+  const int R = (int)wsum.size();
+  std::vector<std::vector<int32_t>> by_region(R);
+  for (int64_t t = 0; t < T; ++t) by_region[h[t].region].push_back((int32_t)t);
+  std::vector<int32_t> rorder(R);
+  for (int r = 0; r < R; ++r) rorder[r] = r;
+  std::stable_sort(rorder.begin(), rorder.end(), [&](int32_t a, int32_t b) { return wsum[a] > wsum[b]; });
+  std::vector<std::vector<int32_t>> stream(N_XCD);
+  std::vector<double> load(N_XCD, 0.0);
+  for (int32_t r : rorder) {
+    std::vector<int32_t>& ts = by_region[r];
+    // region tiles are row-major; a stable sort by block keeps rows within a block
+    std::stable_sort(ts.begin(), ts.end(), [&](int32_t a, int32_t b) {
+      const int32_t ba = h[a].ti / SB, bb = h[b].ti / SB;
+      if (ba != bb) return ba < bb;
+      return h[a].tj / SB < h[b].tj / SB;
+    });
+    for (size_t s = 0; s < ts.size();) {
+      size_t e = s;
+      double c = 0.0;
+      while (e < ts.size() && h[ts[e]].ti / SB == h[ts[s]].ti / SB && h[ts[e]].tj / SB == h[ts[s]].tj / SB)
+        c += cost[ts[e++]];
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      stream[x].insert(stream[x].end(), ts.begin() + s, ts.begin() + e);
+      load[x] += c;
+      s = e;
+    }
+  }
+  size_t longest = 0;
+  for (auto& st : stream) longest = std::max(longest, st.size());
+  std::vector<int32_t> out;
+  out.reserve(longest * N_XCD);
+  for (size_t q = 0; q < longest; ++q)
+    for (int x = 0; x < N_XCD; ++x) out.push_back(q < stream[x].size() ? stream[x][q] : -1);
+  return out;
+}
 
 extern "C" {
 
@@ -904,11 +1004,14 @@ static SumsLauncher launcher_for(int32_t family, int32_t n_pops, bool miss, int3
 
 // LDS bytes of one fused workgroup, or 0 when the tile's keys x statistics do
 // not fit (160 KiB of LDS per workgroup on gfx950, minus the static arrays).
-static size_t sums_lds_bytes(const pgld_band_plan* plan, int32_t family, int32_t n_pops, bool miss, int n_stats) {
+static size_t sums_lds_bytes(const pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set, bool miss,
+                             int n_stats) {
   const int ns_total = n_stats + 1;
   const int nc = family == PGLD_FAMILY_HAP ? 4 : 9;
   const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
-  const size_t stash_bytes = (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
+  // per-population statistics need no parked counts (each population's
+  // epilogue follows its own products)
+  const size_t stash_bytes = stat_set == PGLD_SET_SELF ? 0 : (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
   const int n_planes_used = family == PGLD_FAMILY_HAP ? (miss ? 2 : 1) : (miss ? 3 : 2);
   const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
   if (red_bytes > (size_t)RED_LDS_BYTES || red_bytes + stash_bytes + op_bytes + STATIC_LDS_BYTES > 160 * 1024)
@@ -942,7 +1045,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
   }
   const int n_stats = n_stats_of(family, n_pops, stat_set);
   const int ns_total = n_stats + 1;
-  const size_t lds = sums_lds_bytes(pl, family, n_pops, miss, n_stats);
+  const size_t lds = sums_lds_bytes(pl, family, n_pops, stat_set, miss, n_stats);
   if (!lds) {
     set_error("band sums: a tile touches %d keys; %d statistics x that many keys exceed LDS", pl->max_tile_slots,
               n_stats);
@@ -1022,25 +1125,38 @@ int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_
     rd[r].key_base = r * plan->n_keys_local;
     for (int p = 0; p < n_pops; ++p) wsum[r] += (double)k_pad[(size_t)r * n_pops + p];
   }
-  // tile orders: cost-descending (longest first over the persistent grid) and natural
+  // tile orders: cost-descending (longest first over the persistent grid),
+  // natural, and L2-blocked per XCD
   const int64_t T = plan->n_tiles;
-  std::vector<int32_t> order(2 * std::max<int64_t>(T, 1));
   std::vector<double> cost(T);
+  std::vector<int32_t> order;
+  order.reserve(3 * T + 64);
   for (int64_t t = 0; t < T; ++t) {
     const BandTileHost& h = plan->h_tiles[t];
     const double pairs = h.diag ? 0.5 * h.rows * h.cols : (double)h.rows * h.cols;
     cost[t] = (double)h.rows * h.cols * wsum[h.region] + 64.0 * pairs;  // K loop + epilogue
-    order[t] = (int32_t)t;
-    order[T + t] = (int32_t)t;
+    order.push_back((int32_t)t);
   }
-  std::stable_sort(order.begin(), order.begin() + T, [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+  for (int64_t t = 0; t < T; ++t) order.push_back((int32_t)t);
+  const std::vector<int32_t> blocked = blocked_order(plan->h_tiles, wsum, cost);
+  order.insert(order.end(), blocked.begin(), blocked.end());
+  plan->order_off[0] = 0;
+  plan->order_off[1] = T;
+  plan->order_off[2] = 2 * T;
+  plan->order_len[0] = plan->order_len[1] = T;
+  plan->order_len[2] = (int64_t)blocked.size();
+  if (order.empty()) order.push_back(-1);
+  if (plan->d_order) {
+    (void)hipFree(plan->d_order);
+    plan->d_order = nullptr;
+  }
+  PGLD_TRY_HIP(hipMalloc((void**)&plan->d_order, order.size() * sizeof(int32_t)), "batch order");
+  PGLD_TRY_HIP(hipMemcpy(plan->d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+               "batch order upload");
   if (!plan->d_regions) PGLD_TRY_HIP(hipMalloc(&plan->d_regions, (size_t)R * sizeof(RegionDesc)), "batch regions");
   PGLD_TRY_HIP(hipMemcpy(plan->d_regions, rd.data(), (size_t)R * sizeof(RegionDesc), hipMemcpyHostToDevice),
                "batch regions upload");
-  if (!plan->d_order)
-    PGLD_TRY_HIP(hipMalloc((void**)&plan->d_order, order.size() * sizeof(int32_t)), "batch order");
-  PGLD_TRY_HIP(hipMemcpy(plan->d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice),
-               "batch order upload");
   int rc = ensure_partials(plan, n_stats_of(family, n_pops, stat_set) + 1);
   if (rc) return rc;
   plan->bound_family = family;
@@ -1059,9 +1175,9 @@ int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, do
 
 int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split) {
   PGLD_REQUIRE(plan && plan->n_regions > 0, "pgld_batch_schedule: not a region batch plan");
-  PGLD_REQUIRE(grid >= 0 && (order == 0 || order == 1), "pgld_batch_schedule: bad grid or order");
+  PGLD_REQUIRE(grid >= 0 && order >= 0 && order <= 2, "pgld_batch_schedule: bad grid or order");
   plan->grid = grid;
-  plan->lpt = order == 0;
+  plan->order_mode = order;
   plan->xcd_split = xcd_split != 0;
   return PGLD_OK;
 }
@@ -1091,7 +1207,7 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
     auto it = cache.find(key);
     if (it != cache.end()) {
       plan->grid = it->second[0];
-      plan->lpt = it->second[1] == 0;
+      plan->order_mode = it->second[1];
       plan->xcd_split = it->second[2] != 0;
       if (grid_out) *grid_out = it->second[0];
       if (order_out) *order_out = it->second[1];
@@ -1106,6 +1222,7 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
     cands.push_back({g, 0, 0});  // cost-descending, strided
     cands.push_back({g, 1, 0});  // natural, strided
     cands.push_back({g, 1, 1});  // natural, one contiguous eighth per XCD
+    cands.push_back({g, 2, 0});  // 8 x 8 tile blocks dealt to the XCDs
   }
   hipEvent_t e0, e1;
   PGLD_TRY_HIP(hipEventCreate(&e0), "autotune event");
@@ -1115,7 +1232,7 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
   int rc = PGLD_OK;
   for (size_t c = 0; c < cands.size() && !rc; ++c) {
     plan->grid = cands[c].grid;
-    plan->lpt = cands[c].order == 0;
+    plan->order_mode = cands[c].order;
     plan->xcd_split = cands[c].xcd != 0;
     if (c == 0) rc = pgld_batch_sums(plan, with_missing, sums, key_pairs, stream);  // warm-up
     if (!rc) rc = check_hip(hipEventRecord(e0, s), "autotune event");
@@ -1133,7 +1250,7 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   plan->grid = win.grid;
-  plan->lpt = win.order == 0;
+  plan->order_mode = win.order;
   plan->xcd_split = win.xcd != 0;
   {
     std::lock_guard<std::mutex> lk(mu);

@@ -12,6 +12,7 @@
 
 struct BandTileHost {  // tile geometry kept on the host for scheduling
   int32_t region, rows, cols, diag;
+  int32_t ti, tj;  // tile row / column (64-SNP blocks) within the region
 };
 
 struct pgld_band_plan {
@@ -34,10 +35,12 @@ struct pgld_band_plan {
   std::vector<int64_t> h_region_nvar, h_region_base;
   std::vector<BandTileHost> h_tiles;   // host copy of the schedule, for ordering
   void* d_regions = nullptr;           // RegionDesc[n_regions], written by bind
-  int32_t* d_order = nullptr;          // [cost-descending order | natural order] of the tiles
+  int32_t* d_order = nullptr;          // tile orders, see order_mode (-1 entries are skipped)
   int32_t bound_family = -1, bound_pops = 0, bound_set = -1;
   int32_t grid = 0;                    // persistent grid (0 = one workgroup per tile)
-  bool lpt = true;                     // cost-descending tile order (else natural)
+  int32_t order_mode = 0;              // 0 cost-descending, 1 natural, 2 L2-blocked per XCD
+  int64_t order_off[3] = {0, 0, 0};    // start of each order in d_order
+  int64_t order_len[3] = {0, 0, 0};    // its length (the blocked order holds -1 padding)
   bool xcd_split = false;              // each XCD walks a contiguous eighth of the order
   int64_t partials_ns = 0;             // statistics + 1 the workspace was sized for
   // per-SNP sums of pair values (single bands): tiles by tile-row / tile-column
@@ -47,6 +50,9 @@ struct pgld_band_plan {
   double* d_snp_part = nullptr;        // workspace: per tile 64 row + 64 column sums
   int32_t* d_snp_cnt = nullptr;
 };
+
+// Address space 1 (global memory) on AMDGPU.
+#define GLOBAL_AS __attribute__((address_space(1)))
 
 namespace pgld {
 
@@ -139,7 +145,10 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   constexpr int N = PL::N;
   constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
   const int tid = threadIdx.x, lane = tid & 63;
-  const int8_t* src[ITEMS];
+  // plane pointers may come from a region table in memory; the explicit
+  // global address space keeps the loads off the flat path (whose counter
+  // would make every LDS wait also wait for the prefetch)
+  const GLOBAL_AS int8_t* src[ITEMS];
   int dst[ITEMS];
 #pragma unroll
   for (int m = 0; m < ITEMS; ++m) {
@@ -147,7 +156,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
     const int q = e & 7, row = (e >> 3) & (BT - 1), rest = e >> 9;
     const int a = rest % N, side = rest / N;
     const int64_t snp = min((int64_t)(side ? j0 : i0) + row, n_var - 1);
-    src[m] = planes + ((int64_t)(PL::FIRST + a) * n_var + snp) * k_pad + 16 * q;
+    src[m] = (const GLOBAL_AS int8_t*)planes + ((int64_t)(PL::FIRST + a) * n_var + snp) * k_pad + 16 * q;
     dst[m] = ((side * N + a) * BT + row) * OP_ROW + 16 * q;
   }
 #pragma unroll
@@ -161,11 +170,11 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   constexpr int PF = N <= 2 ? 2 : 1;
   v4i pre0[ITEMS], pre1[PF == 2 ? ITEMS : 1];
 #pragma unroll
-  for (int m = 0; m < ITEMS; ++m) pre0[m] = *reinterpret_cast<const v4i*>(src[m]);
+  for (int m = 0; m < ITEMS; ++m) pre0[m] = *(const GLOBAL_AS v4i*)(src[m]);
   if constexpr (PF == 2) {
     if (KSTEP < k_pad) {
 #pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre1[m] = *reinterpret_cast<const v4i*>(src[m] + KSTEP);
+      for (int m = 0; m < ITEMS; ++m) pre1[m] = *(const GLOBAL_AS v4i*)(src[m] + KSTEP);
     }
   }
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
@@ -176,7 +185,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
     __syncthreads();
     if (k0 + PF * KSTEP < k_pad) {
 #pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre[m] = *reinterpret_cast<const v4i*>(src[m] + k0 + PF * KSTEP);
+      for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + k0 + PF * KSTEP);
     }
     if (wave_live && POPC) {
       // 8 groups of four 32-sample words per 128-byte chunk; the A rows of a
@@ -237,8 +246,9 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
 // column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
 template <int FAMILY, bool MISSING>
 __device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING>::N>& acc, int r,
-                                            const int32_t* __restrict__ marg, int64_t n_var, int64_t i, int64_t j,
+                                            const int32_t* __restrict__ marg_, int64_t n_var, int64_t i, int64_t j,
                                             int32_t S, int* out) {
+  const GLOBAL_AS int32_t* marg = (const GLOBAL_AS int32_t*)marg_;
   if constexpr (FAMILY == PGLD_FAMILY_HAP) {
     int hh, hv, vh, vv;
     hh = acc.c[0][0][r];
@@ -296,6 +306,15 @@ __device__ __forceinline__ double wave_sum(double x) {
 __device__ __forceinline__ int32_t key_bin(double d, const double* edges, int32_t n_keys) {
   if (!(d >= edges[0]) || !(d < edges[n_keys])) return -1;
   int lo = 0, hi = n_keys;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (edges[mid] <= d) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Bin of d among bins [lo, hi) given edges[lo] <= d < edges[hi].
+__device__ __forceinline__ int32_t key_bin_in(double d, const double* edges, int lo, int hi) {
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (edges[mid] <= d) lo = mid; else hi = mid;

@@ -70,7 +70,7 @@ def test_batch_schedules_and_autotune_are_bit_identical(small_batch):
     b = batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(d_dev, rows_dev), stat_set="self")
     ref = np_(b.sums(True))
     for grid in (0, 8, 256, 1024):
-        for order in ("cost", "natural"):
+        for order in ("cost", "natural", "blocked"):
             for xcd in (False, True):
                 b.schedule(grid, order, xcd)
                 np.testing.assert_array_equal(np_(b.sums(True)), ref)
