@@ -979,8 +979,8 @@ static int fill_pop_args(int32_t n_pops, const int8_t* const* planes, const int3
   pa = PopArgs{};
   for (int p = 0; p < n_pops; ++p) {
     PGLD_REQUIRE(planes[p] && marg[p], "band: null planes/marg for pop %d", p);
-    PGLD_REQUIRE(k_pad[p] % KSTEP == 0 && k_pad[p] >= n_samples[p] && n_samples[p] >= 0,
-                 "band: k_pad must be a multiple of 128 >= n_samples (pop %d)", p);
+    PGLD_REQUIRE(k_pad[p] % KSTEP == 0 && k_pad[p] >= KSTEP && k_pad[p] >= n_samples[p] && n_samples[p] >= 0,
+                 "band: k_pad must be a positive multiple of 128 >= n_samples (pop %d)", p);
     PGLD_REQUIRE(n_samples[p] <= 32767, "band: at most 32767 samples per population");
     pa.planes[p] = planes[p];
     pa.marg[p] = marg[p];

@@ -169,24 +169,24 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   // of chunk c run (PF = 2 where registers allow: up to 2 planes per side)
   constexpr int PF = N <= 2 ? 2 : 1;
   v4i pre0[ITEMS], pre1[PF == 2 ? ITEMS : 1];
+  // Loads are issued unconditionally (a chunk past the end re-reads the last
+  // chunk and is never used) so that every path keeps the same number of
+  // loads in flight and the wait before a chunk's LDS store covers only that
+  // chunk, not the younger prefetch.  k_pad is a multiple of KSTEP.
+  auto load = [&](v4i* pre, int64_t k) {
+    const int64_t kk = k < k_pad ? k : k_pad - KSTEP;
 #pragma unroll
-  for (int m = 0; m < ITEMS; ++m) pre0[m] = *(const GLOBAL_AS v4i*)(src[m]);
-  if constexpr (PF == 2) {
-    if (KSTEP < k_pad) {
-#pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre1[m] = *(const GLOBAL_AS v4i*)(src[m] + KSTEP);
-    }
-  }
+    for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + kk);
+  };
+  load(pre0, 0);
+  if constexpr (PF == 2) load(pre1, KSTEP);
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
   auto step = [&](v4i* pre, int64_t k0) {
     __syncthreads();  // the previous chunk has been read by every wave
 #pragma unroll
     for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dst[m]) = pre[m];
     __syncthreads();
-    if (k0 + PF * KSTEP < k_pad) {
-#pragma unroll
-      for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + k0 + PF * KSTEP);
-    }
+    load(pre, k0 + PF * KSTEP);
     if (wave_live && POPC) {
       // 8 groups of four 32-sample words per 128-byte chunk; the A rows of a
       // half-wave are shared (LDS broadcast), B is this lane's column
@@ -233,10 +233,12 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
     }
   };
   if constexpr (PF == 2) {
-    for (int64_t k0 = 0; k0 < k_pad; k0 += 2 * KSTEP) {
+    int64_t k0 = 0;
+    for (; k0 + 2 * KSTEP <= k_pad; k0 += 2 * KSTEP) {
       step(pre0, k0);
-      if (k0 + KSTEP < k_pad) step(pre1, k0 + KSTEP);
+      step(pre1, k0 + KSTEP);
     }
+    if (k0 < k_pad) step(pre0, k0);
   } else {
     for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) step(pre0, k0);
   }

@@ -358,8 +358,8 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
                "pgld_band_ld_pairs: Rogers-Huff r needs genotypes without missing values");
   PGLD_REQUIRE(plan && planes && marg, "pgld_band_r2: null pointer");
   PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
-  PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= n_samples && n_samples >= 0,
-               "pgld_band_r2: k_pad must be a multiple of 128 >= n_samples");
+  PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= KSTEP && k_pad >= n_samples && n_samples >= 0,
+               "pgld_band_r2: k_pad must be a positive multiple of 128 >= n_samples");
   PGLD_REQUIRE(!hist || n_hist >= 1, "pgld_band_r2: n_hist must be >= 1");
   PGLD_REQUIRE(!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2_MAX_HIST_LDS,
                "pgld_band_r2: %d keys per tile x %d histogram bins exceed LDS", plan->max_tile_slots, n_hist + 1);
