@@ -216,19 +216,25 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
         }
       }
     } else if (wave_live) {
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        v4i A[N], B[N];
+      // operand fragments of substep st + 1 are read while the MFMAs of
+      // substep st run
+      v4i A[2][N], B[2][N];
+      auto frag = [&](int st, v4i* a_, v4i* b_) {
 #pragma unroll
         for (int a = 0; a < N; ++a) {
-          A[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
-          B[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
+          a_[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
+          b_[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
         }
+      };
+      frag(0, A[0], B[0]);
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        if (st < 3) frag(st + 1, A[(st + 1) & 1], B[(st + 1) & 1]);
 #pragma unroll
         for (int a = 0; a < N; ++a)
 #pragma unroll
           for (int b = 0; b < N; ++b)
-            acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc.c[a][b], 0, 0, 0);
+            acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
       }
     }
   };
