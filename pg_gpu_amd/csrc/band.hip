@@ -702,6 +702,34 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
                    key_list.size() * sizeof(int64_t));
   if (!rc && plan->n_regions == 0) {
+    // 2 x 2 groups of tiles in row-major order of the groups
+    std::vector<std::pair<int64_t, int32_t>> keyed(tb.tiles.size());
+    const int64_t n_sc = ceil_div(std::max<int64_t>(n_var, 1), 2 * BT) + 1;
+    for (size_t t = 0; t < tb.tiles.size(); ++t) {
+      const int64_t ti = tb.tiles[t].i0 / BT, tj = tb.tiles[t].j0 / BT;
+      keyed[t] = {(ti >> 1) * n_sc + (tj >> 1), (int32_t)t};
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    std::vector<SuperTile> sup;
+    for (size_t k = 0; k < keyed.size();) {
+      SuperTile st{};
+      st.i0 = (int32_t)((keyed[k].first / n_sc) * 2 * BT);
+      st.j0 = (int32_t)((keyed[k].first % n_sc) * 2 * BT);
+      for (int q = 0; q < 4; ++q) st.t[q] = -1;
+      const int64_t key = keyed[k].first;
+      for (; k < keyed.size() && keyed[k].first == key; ++k) {
+        const BandTile& bt = tb.tiles[keyed[k].second];
+        st.t[((bt.i0 - st.i0) / BT) * 2 + (bt.j0 - st.j0) / BT] = keyed[k].second;
+      }
+      sup.push_back(st);
+    }
+    plan->n_super = (int64_t)sup.size();
+    rc = up(&plan->d_super, sup.empty() ? nullptr : sup.data(), sup.size() * sizeof(SuperTile));
+  }
+  if (!rc && plan->n_regions == 0) {
     // tile lists by tile-row (contiguous in the schedule) and tile-column
     const int64_t n_tr = ceil_div(std::max<int64_t>(n_var, 1), BT);
     std::vector<int64_t> row_off(n_tr + 1, 0), col_off(n_tr + 1, 0);
@@ -947,7 +975,8 @@ int pgld_band_plan_destroy(pgld_band_plan* plan) {
   if (!plan) return PGLD_OK;
   void* ptrs[] = {plan->d_pos,      plan->d_kc,       plan->d_edges,     plan->d_tiles,    plan->d_pair_base,
                   plan->d_key_off,  plan->d_key_list, plan->d_partials,  plan->d_regions,  plan->d_order,
-                  plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt};
+                  plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt,
+                  plan->d_super};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)

@@ -47,6 +47,8 @@ struct pgld_band_plan {
   int64_t* d_row_off = nullptr;        // tile-row r -> tiles [row_off[r], row_off[r + 1])
   int64_t* d_col_off = nullptr;        // tile-column c -> col_list[col_off[c] .. col_off[c + 1])
   int32_t* d_col_list = nullptr;
+  void* d_super = nullptr;             // SuperTile[n_super] (single plans): 2 x 2 groups of tiles
+  int64_t n_super = 0;
   double* d_snp_part = nullptr;        // workspace: per tile 64 row + 64 column sums
   int32_t* d_snp_cnt = nullptr;
 };
@@ -340,6 +342,14 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
   const int64_t per = n / N_XCD, rem = n % N_XCD;
   return x * per + (x < rem ? x : rem) + slot;
 }
+
+// 128 x 128 block of the pair matrix: the (up to) four 64 x 64 plan tiles
+// (i0 + 64 (q >> 1), j0 + 64 (q & 1)), q = 0..3, as indices into the plan's
+// tile list (-1 where the band has no such tile).
+struct SuperTile {
+  int32_t i0, j0;
+  int32_t t[4];
+};
 
 struct BandArgs {
   const BandTile* tiles;

@@ -52,6 +52,56 @@ enum R2Est : int32_t {
 
 constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
 
+// r^2-family value of a haplotype pair from its counts (n11 = h_i.h_j, nA, nB
+// the pair-valid allele counts, n the pair-valid sample count); is_nan marks
+// an undefined r^2, sval / scnt the pair's contribution to per-SNP sums.
+template <bool MISSING, int EST>
+__device__ __forceinline__ double hap_pair_value(int64_t n11, int64_t nA, int64_t nB, int64_t n,
+                                                 const int32_t* __restrict__ marg, int64_t n_var, int64_t i,
+                                                 int64_t j, int32_t S, bool& is_nan, double& sval, int& scnt) {
+  double r2;
+  is_nan = false;
+  if constexpr (EST == EST_R2) {
+    const int64_t diff = n * n11 - nA * nB;
+    const double dd = (double)diff;
+    const double num = dd * dd;
+    const double den = (double)(nA * (n - nA)) * (double)(nB * (n - nB));
+    is_nan = !(den > 0.0);
+    r2 = den > 0.0 ? num / den : __builtin_nan("");
+    sval = is_nan ? 0.0 : r2;
+    scnt = !is_nan;
+  } else if constexpr (EST == EST_R2_NAIVE) {
+#pragma clang fp contract(off)
+    // per-site frequencies over every valid sample of the site, joint
+    // frequency over the pair's valid samples (the reference's order of
+    // operations: p_AB - p_i p_j, then D^2 / (p_i q_i p_j q_j))
+    const double vi = MISSING ? (double)marg[n_var + i] : (double)S;
+    const double vj = MISSING ? (double)marg[n_var + j] : (double)S;
+    const double pi = vi > 0.0 ? (double)marg[i] / vi : 0.0;
+    const double pj = vj > 0.0 ? (double)marg[j] / vj : 0.0;
+    const double pab = n > 0 ? (double)n11 / (double)n : 0.0;
+    const double D = pab - pi * pj;
+    const double den = (pi * (1.0 - pi)) * (pj * (1.0 - pj));
+    r2 = den > 0.0 ? (D * D) / den : 0.0;
+    sval = r2;
+    scnt = 1;
+  } else {
+    const double c1 = (double)n11, c2 = (double)(nA - n11), c3 = (double)(nB - n11);
+    const double c4 = (double)(n - nA - nB + n11);
+    const double dd_num = c1 * (c1 - 1.0) * c4 * (c4 - 1.0) + c2 * (c2 - 1.0) * c3 * (c3 - 1.0) -
+                          2.0 * c1 * c2 * c3 * c4;
+    const double pi2_num = (c1 + c2) * (c1 + c3) * (c2 + c4) * (c3 + c4) -
+                           c1 * c4 * (-1.0 + c1 + 3.0 * c2 + 3.0 * c3 + c4) -
+                           c2 * c3 * (-1.0 + 3.0 * c1 + c2 + c3 + 3.0 * c4);
+    const bool ok = n >= 4;
+    r2 = (ok && pi2_num != 0.0) ? dd_num / pi2_num : 0.0;
+    sval = r2;
+    scnt = ok;
+  }
+  return r2;
+}
+
+
 template <int FAMILY, bool MISSING, bool POPC, int EST>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const int8_t* __restrict__ planes,
                                                              const int32_t* __restrict__ marg, int64_t k_pad,
@@ -185,45 +235,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           nB = marg[j];
           n = S;
         }
-        double r2;
         bool is_nan = false;
-        if constexpr (EST == EST_R2) {
-          const int64_t diff = n * n11 - nA * nB;
-          const double dd = (double)diff;
-          const double num = dd * dd;
-          const double den = (double)(nA * (n - nA)) * (double)(nB * (n - nB));
-          is_nan = !(den > 0.0);
-          r2 = den > 0.0 ? num / den : __builtin_nan("");
-          sval = is_nan ? 0.0 : r2;
-          scnt = !is_nan;
-        } else if constexpr (EST == EST_R2_NAIVE) {
-#pragma clang fp contract(off)
-          // per-site frequencies over every valid sample of the site, joint
-          // frequency over the pair's valid samples (the reference's order of
-          // operations: p_AB - p_i p_j, then D^2 / (p_i q_i p_j q_j))
-          const double vi = MISSING ? (double)marg[ra.n_var + i] : (double)S;
-          const double vj = MISSING ? (double)marg[ra.n_var + j] : (double)S;
-          const double pi = vi > 0.0 ? (double)marg[i] / vi : 0.0;
-          const double pj = vj > 0.0 ? (double)marg[j] / vj : 0.0;
-          const double pab = n > 0 ? (double)n11 / (double)n : 0.0;
-          const double D = pab - pi * pj;
-          const double den = (pi * (1.0 - pi)) * (pj * (1.0 - pj));
-          r2 = den > 0.0 ? (D * D) / den : 0.0;
-          sval = r2;
-          scnt = 1;
-        } else {
-          const double c1 = (double)n11, c2 = (double)(nA - n11), c3 = (double)(nB - n11);
-          const double c4 = (double)(n - nA - nB + n11);
-          const double dd_num = c1 * (c1 - 1.0) * c4 * (c4 - 1.0) + c2 * (c2 - 1.0) * c3 * (c3 - 1.0) -
-                                2.0 * c1 * c2 * c3 * c4;
-          const double pi2_num = (c1 + c2) * (c1 + c3) * (c2 + c4) * (c3 + c4) -
-                                 c1 * c4 * (-1.0 + c1 + 3.0 * c2 + 3.0 * c3 + c4) -
-                                 c2 * c3 * (-1.0 + 3.0 * c1 + c2 + c3 + 3.0 * c4);
-          const bool ok = n >= 4;
-          r2 = (ok && pi2_num != 0.0) ? dd_num / pi2_num : 0.0;
-          sval = r2;
-          scnt = ok;
-        }
+        const double r2 = hap_pair_value<MISSING, EST>(n11, nA, nB, n, marg, ra.n_var, i, j, S, is_nan, sval, scnt);
         r2f = (float)r2;
         if (ra.hist) {
           int key;
@@ -305,6 +318,192 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
       const int v = s_hist[t];
       if (v) atomicAdd(&ra.hist[(int64_t)(tile.key_lo + t / hw) * hw + t % hw], (unsigned long long)v);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wide r^2 kernel: haplotypes without missing data (one int8 plane per side).
+// One workgroup per 128 x 128 super-tile; per 128-byte K chunk 128 A rows and
+// 128 B rows are staged in LDS and wave w accumulates the 64 x 64 plan tile
+// q = w of the super-tile as 2 x 2 MFMA blocks, so each operand fragment read
+// from LDS feeds two MFMAs (half the LDS traffic per MFMA of the 64 x 64
+// kernel).  The per-pair values, keys and outputs are those of k_band_r2 (the
+// same function evaluates r^2); each wave keeps its plan tile's histogram in
+// its own quarter of the LDS histogram.  Per-SNP sums are left to k_band_r2.
+// ---------------------------------------------------------------------------
+constexpr int R2W_ROWS = 2 * BT;
+constexpr int R2W_HIST_PER_WAVE = R2_MAX_HIST_LDS / 4;
+constexpr int R2W_OP_BYTES = 2 * R2W_ROWS * OP_ROW;
+
+template <int EST>
+__global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, const SuperTile* __restrict__ sts,
+                                                                  const int8_t* __restrict__ planes,
+                                                                  const int32_t* __restrict__ marg, int64_t k_pad,
+                                                                  int32_t S) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ double s_pos_i[R2W_ROWS], s_pos_j[R2W_ROWS], s_kc_i[R2W_ROWS], s_kc_j[R2W_ROWS];
+  __shared__ int s_wkey[R2W_ROWS];
+  __shared__ int s_hist[R2_MAX_HIST_LDS];
+
+  const SuperTile st = sts[xcd_tile(blockIdx.x, gridDim.x)];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int hw = ra.n_hist + 1;
+  const int own = w == 0 ? st.t[0] : w == 1 ? st.t[1] : w == 2 ? st.t[2] : st.t[3];
+  const BandTile tile = own >= 0 ? ra.tiles[own] : BandTile{};
+  int* wh = s_hist + w * R2W_HIST_PER_WAVE;
+  const int h_n = (ra.hist && own >= 0) ? tile.n_slots * hw : 0;
+  for (int t = lane; t < h_n; t += 64) wh[t] = 0;
+  if (tid < R2W_ROWS) {
+    const int64_t gi = min((int64_t)st.i0 + tid, ra.n_var - 1), gj = min((int64_t)st.j0 + tid, ra.n_var - 1);
+    s_pos_i[tid] = ra.pos[gi];
+    s_pos_j[tid] = ra.pos[gj];
+    s_kc_i[tid] = ra.kc[gi];
+    s_kc_j[tid] = ra.kc[gj];
+    if (ra.key_mode == PGLD_KEY_WINDOW) {
+      const double wk = floor((ra.kc[gi] - ra.origin) / ra.width);
+      s_wkey[tid] = (wk >= 0.0 && wk < (double)ra.n_keys) ? (int)wk : -1;
+    }
+  }
+  __syncthreads();
+
+  // this wave's plan tile: rows wr .. wr + 63, columns wc .. wc + 63 of the super-tile
+  const int wr = (w >> 1) * BT, wc = (w & 1) * BT;
+  const int64_t ri = (int64_t)st.i0 + wr, rj = (int64_t)st.j0 + wc;
+  const int64_t last_i = min(ri + BT - 1, ra.n_var - 1);
+  const bool live = own >= 0 && rj < ra.n_var && rj + BT - 1 > ri && ri < ra.n_var &&
+                    s_pos_j[wc] - s_pos_i[last_i - st.i0] <= ra.max_dist;
+
+  // ---- K loop: LDS-staged 128 + 128 rows per chunk, two register stages ----
+  constexpr int ITEMS = 2 * R2W_ROWS * 8 / BAND_THREADS;
+  const GLOBAL_AS int8_t* src[ITEMS];
+  int dst[ITEMS];
+#pragma unroll
+  for (int m = 0; m < ITEMS; ++m) {
+    const int e = tid + BAND_THREADS * m;
+    const int q = e & 7, row = (e >> 3) & (R2W_ROWS - 1), side = e >> 10;
+    const int64_t snp = min((int64_t)(side ? st.j0 : st.i0) + row, ra.n_var - 1);
+    src[m] = (const GLOBAL_AS int8_t*)planes + snp * k_pad + 16 * q;  // plane 0 (h)
+    dst[m] = (side * R2W_ROWS + row) * OP_ROW + 16 * q;
+  }
+  v16i acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+  v4i pre0[ITEMS], pre1[ITEMS];
+  auto load = [&](v4i* pre, int64_t k) {  // past the end: the last chunk again (unused)
+    const int64_t kk = k < k_pad ? k : k_pad - KSTEP;
+#pragma unroll
+    for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + kk);
+  };
+  load(pre0, 0);
+  load(pre1, KSTEP);
+  const int kh = 16 * (lane >> 5);
+  auto step = [&](v4i* pre, int64_t k0) {
+    __syncthreads();  // the previous chunk has been read by every wave
+#pragma unroll
+    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(smem + dst[m]) = pre[m];
+    __syncthreads();
+    load(pre, k0 + 2 * KSTEP);
+    if (live) {
+      v4i A[2][2], B[2][2];
+      auto frag = [&](int sub, v4i* a_, v4i* b_) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          a_[h] = *reinterpret_cast<const v4i*>(smem + (wr + 32 * h + (lane & 31)) * OP_ROW + 32 * sub + kh);
+          b_[h] = *reinterpret_cast<const v4i*>(smem + (R2W_ROWS + wc + 32 * h + (lane & 31)) * OP_ROW +
+                                                32 * sub + kh);
+        }
+      };
+      frag(0, A[0], B[0]);
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub) {
+        if (sub < 3) frag(sub + 1, A[(sub + 1) & 1], B[(sub + 1) & 1]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[sub & 1][a], B[sub & 1][b], acc[a][b], 0, 0, 0);
+      }
+    }
+  };
+  {
+    int64_t k0 = 0;
+    for (; k0 + 2 * KSTEP <= k_pad; k0 += 2 * KSTEP) {
+      step(pre0, k0);
+      step(pre1, k0 + KSTEP);
+    }
+    if (k0 < k_pad) step(pre0, k0);
+  }
+
+  // ---- per-pair r^2, histogram, sparse list, per-pair output ----
+  if (live) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int jl = wc + 32 * b + (lane & 31);
+      const int64_t j = (int64_t)st.j0 + jl;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int il = wr + 32 * a + acc_row(r, lane);
+          const int64_t i = (int64_t)st.i0 + il;
+          const bool valid = i < j && j < ra.n_var && s_pos_j[jl] - s_pos_i[il] <= ra.max_dist;
+          bool hit = false;
+          float r2f = 0.0f;
+          if (valid) {
+            bool is_nan = false;
+            double sval = 0.0;
+            int scnt = 0;
+            const double r2 = hap_pair_value<false, EST>(acc[a][b][r], marg[i], marg[j], S, marg, ra.n_var, i, j,
+                                                         S, is_nan, sval, scnt);
+            r2f = (float)r2;
+            if (ra.hist) {
+              int key;
+              if (ra.key_mode == PGLD_KEY_WINDOW) {
+                key = s_wkey[il];
+              } else {
+                double d = s_kc_j[jl] - s_kc_i[il];
+                if (ra.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
+                key = key_bin(d, ra.edges, ra.n_keys);
+              }
+              const int slot = key - tile.key_lo;
+              if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
+                int hb = ra.n_hist;
+                if (!is_nan) hb = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
+                atomicAdd(&wh[slot * hw + hb], 1);
+              }
+            }
+            hit = !is_nan && r2 > ra.thresh;
+            if (ra.r2_out) ra.r2_out[ra.pair_base[i] + (j - i - 1)] = r2;
+          }
+          if (ra.sp_count) {
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+              const int leader = __ffsll((long long)m) - 1;
+              unsigned long long base = 0;
+              if (lane == leader) base = atomicAdd(ra.sp_count, (unsigned long long)__popcll(m));
+              base = __shfl(base, leader);
+              if (hit) {
+                const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
+                if ((int64_t)slot < ra.sp_cap) {
+                  ra.sp_i[slot] = (int32_t)i;
+                  ra.sp_j[slot] = (int32_t)j;
+                  ra.sp_r2[slot] = r2f;
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  // the wave's histogram quarter is written by this wave only
+  for (int t = lane; t < h_n; t += 64) {
+    const int v = wh[t];
+    if (v) atomicAdd(&ra.hist[(int64_t)(tile.key_lo + t / hw) * hw + t % hw], (unsigned long long)v);
   }
 }
 
@@ -416,6 +615,25 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
   const size_t lds = (size_t)2 * npl * BT * OP_ROW;
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[0], s), "pgld_band_r2 event");
+  // haplotypes without missing data and no per-SNP sums: the 128 x 128 kernel
+  const bool wide = !geno && !miss && !popc && !snp_sums && plan->n_super > 0 &&
+                    (!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2W_HIST_PER_WAVE);
+  if (wide) {
+    dim3 gw((unsigned)plan->n_super);
+    const auto* sts = reinterpret_cast<const SuperTile*>(plan->d_super);
+    if (estimator == EST_R2)
+      hipLaunchKernelGGL((k_band_r2_wide<EST_R2>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad, n_samples);
+    else if (estimator == EST_R2_NAIVE)
+      hipLaunchKernelGGL((k_band_r2_wide<EST_R2_NAIVE>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad,
+                         n_samples);
+    else
+      hipLaunchKernelGGL((k_band_r2_wide<EST_SIGMA_D2>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad,
+                         n_samples);
+    rc = check_launch("k_band_r2_wide");
+    if (rc) return rc;
+    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[1], s), "pgld_band_r2 event");
+    return PGLD_OK;
+  }
 #define PGLD_R2(F, M, B, E) \
   hipLaunchKernelGGL((k_band_r2<F, M, B, E>), g, b, lds, s, ra, planes, marg, k_pad, n_samples)
 #define PGLD_R2_EST(M, B)                                                        \
