@@ -2,7 +2,7 @@
 
     python -m pg_gpu_amd.build        # or __graft_entry__.build()
 
-Runs the two code generators under csrc/, compiles every csrc/*.hip to an
+Runs the code generator under csrc/ (gen_epilogue.py), compiles every csrc/*.hip to an
 object with hipcc in parallel, and links pg_gpu_amd/libpgld.so.
 """
 from __future__ import annotations
@@ -36,7 +36,7 @@ def _run(cmd):
 
 
 def build(verbose=False):
-    for gen in ("gen_geno_poly.py", "gen_epilogue.py"):
+    for gen in ("gen_epilogue.py",):
         _run([sys.executable, os.path.join(CSRC, gen)])
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))

@@ -15,7 +15,6 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
-#include "geno_poly.inc"
 
 namespace pgld {
 
@@ -206,6 +205,9 @@ struct GenoP {
   double n;
   double pA, qA, pB, qB;      // allele counts at the two loci
   double xAB, xAb, xaB, xab;  // same-individual joint allele counts
+  // moments of the centred dosages a' = a - 1, b' = b - 1 (in {-1, 0, 1})
+  // over the individuals: Sa = sum a', Sab = sum a' b', Sa2b = sum a'^2 b', ...
+  double Sa, Sb, Sab, Sa2, Sb2, Sa2b, Sab2, Sa2b2;
 };
 
 PGLD_HD GenoP geno_pop(const double* cnt_n00_to_n22) {
@@ -226,18 +228,57 @@ PGLD_HD GenoP geno_pop(const double* cnt_n00_to_n22) {
   p.xAb = 0.5 * g[1] + g[2] + 0.25 * g[4] + 0.5 * g[5];
   p.xaB = 0.5 * g[3] + 0.25 * g[4] + g[6] + 0.5 * g[7];
   p.xab = 0.25 * g[4] + 0.5 * g[5] + 0.5 * g[7] + g[8];
+  // g[3 ia + ib] has a' = 1 - ia, b' = 1 - ib
+  double R[3], B1[3], B2[3];
+#pragma unroll
+  for (int ia = 0; ia < 3; ++ia) {
+    R[ia] = g[3 * ia] + g[3 * ia + 1] + g[3 * ia + 2];
+    B1[ia] = g[3 * ia] - g[3 * ia + 2];
+    B2[ia] = g[3 * ia] + g[3 * ia + 2];
+  }
+  p.Sb = B1[0] + B1[1] + B1[2];
+  p.Sb2 = B2[0] + B2[1] + B2[2];
+  p.Sa = R[0] - R[2];
+  p.Sab = B1[0] - B1[2];
+  p.Sab2 = B2[0] - B2[2];
+  p.Sa2 = R[0] + R[2];
+  p.Sa2b = B1[0] + B1[2];
+  p.Sa2b2 = B2[0] + B2[2];
   return p;
 }
 
-// D from genotype frequencies (_PopDataGeno.D_geno, genotype_kernels.py:441-446)
-PGLD_HD double geno_D(const GenoP& p) { return p.xAB * p.xab - p.xAb * p.xaB; }
+// D from genotype frequencies (_PopDataGeno.D_geno, genotype_kernels.py:441-446):
+// xAB xab - xAb xaB = (n Sab - Sa Sb) / 4.
+PGLD_HD double geno_D(const GenoP& p) { return (p.n * p.Sab - p.Sa * p.Sb) * 0.25; }
 
+// The single-population numerators of genotype_kernels.py (DD :27-71, pi2
+// :279-300 with its correction polynomial, Dz :199-233), polynomials of
+// degree 4 in the nine genotype counts with a few hundred monomials, are
+// written here in the centred moments, where they collapse to a handful of
+// terms (X = Sb^2 - Sb2, Y = Sa^2 - Sa2; the identities are exact and are
+// checked against the reference's tables in the test suite):
+//   DD  = [(n-1)(n-2) Sab^2 - n(n-1) Sa2b2 - 2(n-1) Sa Sb Sab + 2(n-1)(Sb Sa2b + Sa Sab2) + X Y] / 4
+//   Dz  = [(n+1)(Sa Sb Sab - Sb Sa2b - Sa Sab2) - (n-1) Sab^2 + 2n Sa2b2 - X Y] / 4
+//   pi2 = [n(n-1)(n-2)(n-3) - (n-2)(n-3)(X + Y) + X Y - 4 Sa Sb Sab + 4 (Sb Sa2b + Sa Sab2)
+//          + 2 Sab^2 - 6 Sa2b2] / 16
+// Every intermediate value is an integer, so for counts whose products stay
+// below 2^53 they equal the reference's values exactly.
 PGLD_HD double geno_dd_num(const GenoP& p) {
-  double Dg = geno_D(p);
-  return 4.0 * (geno_poly_dd(p.g) * (1.0 / 16.0) + Dg * Dg);
+  const double n = p.n, X = p.Sb * p.Sb - p.Sb2, Y = p.Sa * p.Sa - p.Sa2;
+  const double sab = p.Sa * p.Sb * p.Sab, lin = p.Sb * p.Sa2b + p.Sa * p.Sab2;
+  return 0.25 * ((n - 1.0) * ((n - 2.0) * (p.Sab * p.Sab) - n * p.Sa2b2 - 2.0 * sab + 2.0 * lin) + X * Y);
+}
+PGLD_HD double geno_dz_num(const GenoP& p) {
+  const double n = p.n, X = p.Sb * p.Sb - p.Sb2, Y = p.Sa * p.Sa - p.Sa2;
+  const double sab = p.Sa * p.Sb * p.Sab, lin = p.Sb * p.Sa2b + p.Sa * p.Sab2;
+  return 0.25 * ((n + 1.0) * (sab - lin) - (n - 1.0) * (p.Sab * p.Sab) + 2.0 * n * p.Sa2b2 - X * Y);
 }
 PGLD_HD double geno_pi2_num(const GenoP& p) {
-  return p.pA * p.pB * p.qB * p.qA + geno_poly_pi2corr(p.g) * (1.0 / 16.0);
+  const double n = p.n, X = p.Sb * p.Sb - p.Sb2, Y = p.Sa * p.Sa - p.Sa2;
+  const double sab = p.Sa * p.Sb * p.Sab, lin = p.Sb * p.Sa2b + p.Sa * p.Sab2;
+  const double n23 = (n - 2.0) * (n - 3.0);
+  return 0.0625 * (n * (n - 1.0) * n23 - n23 * (X + Y) + X * Y - 4.0 * sab + 4.0 * lin + 2.0 * (p.Sab * p.Sab) -
+                   6.0 * p.Sa2b2);
 }
 PGLD_HD double geno_dd_single(const GenoP& p) {
   return p.n < 4.0 ? 0.0 : geno_dd_num(p) / f4(p.n);
@@ -245,18 +286,12 @@ PGLD_HD double geno_dd_single(const GenoP& p) {
 PGLD_HD double geno_dd_between(const GenoP& a, const GenoP& b) {
   return safe_div(4.0 * geno_D(a) * geno_D(b), f2(a.n) * f2(b.n));
 }
-PGLD_HD double geno_Acoef(const GenoP& p) {
-  const double* g = p.g;
-  return -g[0] + g[2] - g[3] + g[5] - g[6] + g[8];
-}
-PGLD_HD double geno_Bcoef(const GenoP& p) {
-  const double* g = p.g;
-  return -g[0] - g[1] - g[2] + g[6] + g[7] + g[8];
-}
+// -g1 + g3 - g4 + g6 - g7 + g9 = -Sb and -g1 - g2 - g3 + g7 + g8 + g9 = -Sa
+PGLD_HD double geno_Acoef(const GenoP& p) { return -p.Sb; }
+PGLD_HD double geno_Bcoef(const GenoP& p) { return -p.Sa; }
 PGLD_HD double geno_dz_iii(const GenoP& p) {
   if (p.n < 4.0) return 0.0;
-  double num = 0.25 * geno_poly_dzq(p.g) + geno_Acoef(p) * geno_Bcoef(p) * geno_D(p);
-  return 2.0 * num / f4(p.n);
+  return 2.0 * geno_dz_num(p) / f4(p.n);
 }
 // Dz precompute features (genotype_kernels.py:176-197)
 PGLD_HD double geno_Ucoef(const GenoP& p) {

@@ -307,3 +307,49 @@ def test_dosage_r2_equals_rogers_huff_without_missing():
     np.testing.assert_allclose(R2[ii, jj], np.nan_to_num(r ** 2), rtol=1e-12, atol=1e-15)
     h = np.column_stack([g[:, 0], g[:, 0]])
     assert O.rogers_huff_r(h)[0] == pytest.approx(1.0)
+
+
+def test_genotype_numerators_centred_moment_identity():
+    """The HIP path (ld_math.h geno_dd_num / geno_dz_num / geno_pi2_num)
+    evaluates the single-population genotype numerators in the moments of the
+    centred dosages a' = a - 1, b' = b - 1 instead of the reference's
+    monomial tables (genotype_kernels.py:27-71, :199-233, :279-300; restated
+    in the oracle and pinned there by brute force).  The identities must hold
+    exactly; integer counts, exact rational arithmetic."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(2024)
+    for trial in range(400):
+        hi = (4, 40, 3000)[trial % 3]
+        g = [int(x) for x in rng.integers(0, hi, 9)]
+        col = [np.array([x], dtype=np.int64) for x in g]
+        n = sum(g)
+        # g[3 ia + ib] has a' = 1 - ia, b' = 1 - ib
+        R = [g[3 * a] + g[3 * a + 1] + g[3 * a + 2] for a in range(3)]
+        B1 = [g[3 * a] - g[3 * a + 2] for a in range(3)]
+        B2 = [g[3 * a] + g[3 * a + 2] for a in range(3)]
+        Sb, Sb2 = sum(B1), sum(B2)
+        Sa, Sab, Sab2 = R[0] - R[2], B1[0] - B1[2], B2[0] - B2[2]
+        Sa2, Sa2b, Sa2b2 = R[0] + R[2], B1[0] + B1[2], B2[0] + B2[2]
+        X, Y = Sb * Sb - Sb2, Sa * Sa - Sa2
+        sab, lin = Sa * Sb * Sab, Sb * Sa2b + Sa * Sab2
+        dd = F((n - 1) * ((n - 2) * Sab * Sab - n * Sa2b2 - 2 * sab + 2 * lin) + X * Y, 4)
+        dz = F((n + 1) * (sab - lin) - (n - 1) * Sab * Sab + 2 * n * Sa2b2 - X * Y, 4)
+        pi2 = F(n * (n - 1) * (n - 2) * (n - 3) - (n - 2) * (n - 3) * (X + Y) + X * Y - 4 * sab + 4 * lin
+                + 2 * Sab * Sab - 6 * Sa2b2, 16)
+        # the reference's forms
+        xAB = F(g[0]) + F(g[1], 2) + F(g[3], 2) + F(g[4], 4)
+        xAb = F(g[1], 2) + g[2] + F(g[4], 4) + F(g[5], 2)
+        xaB = F(g[3], 2) + F(g[4], 4) + g[6] + F(g[7], 2)
+        xab = F(g[4], 4) + F(g[5], 2) + F(g[7], 2) + g[8]
+        D = xAB * xab - xAb * xaB
+        assert D == F(n * Sab - Sa * Sb, 4)
+        A = -g[0] + g[2] - g[3] + g[5] - g[6] + g[8]
+        B = -g[0] - g[1] - g[2] + g[6] + g[7] + g[8]
+        assert (A, B) == (-Sb, -Sa)
+        pA = g[0] + g[1] + g[2] + F(g[3] + g[4] + g[5], 2)
+        qA = F(g[3] + g[4] + g[5], 2) + g[6] + g[7] + g[8]
+        pB = g[0] + F(g[1], 2) + g[3] + F(g[4], 2) + g[6] + F(g[7], 2)
+        qB = F(g[1], 2) + g[2] + F(g[4], 2) + g[5] + F(g[7], 2) + g[8]
+        assert dd == 4 * (F(int(O.eval_poly(O._DD_POLY, col)[0]), 16) + D * D), g
+        assert dz == F(int(O.eval_poly(O._DZ_QUARTIC, col)[0]), 4) + A * B * D, g
+        assert pi2 == pA * pB * qB * qA + F(int(O.eval_poly(O._PI2_CORR, col)[0]), 16), g
