@@ -36,6 +36,9 @@ PGLD_HD double safe_div(double num, double den) { return den > 0.0 ? num / den :
 // ---------------------------------------------------------------------------
 struct HapP {
   double c1, c2, c3, c4, n, D, pA, qA, pB, qB;
+  // moments of the centred alleles a' = 2a - 1, b' = 2b - 1 (in {-1, 1})
+  // over the pair-valid haplotypes: Sa = sum a', Sb = sum b', Sab = sum a' b'
+  double Sa, Sb, Sab;
 };
 
 PGLD_HD HapP hap_pop(double c1, double c2, double c3, double c4) {
@@ -45,6 +48,9 @@ PGLD_HD HapP hap_pop(double c1, double c2, double c3, double c4) {
   p.D = c2 * c3 - c1 * c4;
   p.pA = c1 + c2; p.qA = c3 + c4;
   p.pB = c1 + c3; p.qB = c2 + c4;
+  p.Sa = p.pA - p.qA;
+  p.Sb = p.pB - p.qB;
+  p.Sab = (c1 + c4) - (c2 + c3);
   return p;
 }
 
@@ -54,44 +60,47 @@ PGLD_HD double f2(double n) { return n * (n - 1.0); }
 
 // Numerators.  Each statistic below is num_x(...) / den_x(n...) with a
 // denominator that is a product of per-population falling factorials.
+//
+// The reference writes them as polynomials in the four haplotype counts
+// (haplotype_kernels.py:69-289, ld_statistics.py:1116-1434).  Those used by
+// one and two populations are evaluated here in the centred moments (n, Sa,
+// Sb, Sab) with D = c2 c3 - c1 c4 = -(n Sab - Sa Sb) / 4, where they collapse
+// to a few products (the identities are exact and are checked against the
+// reference's forms in the test suite).  Every intermediate value is an
+// integer or a dyadic fraction of one, so for counts whose products stay
+// below 2^53 they equal the reference's values exactly.
 PGLD_HD double num_dd_single(const HapP& a) {
-  return a.c1 * (a.c1 - 1.0) * a.c4 * (a.c4 - 1.0) + a.c2 * (a.c2 - 1.0) * a.c3 * (a.c3 - 1.0) -
-         2.0 * a.c1 * a.c2 * a.c3 * a.c4;
+  const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
+  return 0.0625 * ((n - 1.0) * ((n - 2.0) * (a.Sab * a.Sab) - 2.0 * t) + (n - 2.0) * (x2 + y2 - n * n) + x2 * y2);
 }
 PGLD_HD double num_dd_between(const HapP& a, const HapP& b) { return a.D * b.D; }
 PGLD_HD double num_dz_iii(const HapP& a) {
-  double diff = a.c1 * a.c4 - a.c2 * a.c3;
-  return diff * (a.qA - a.pA) * (a.qB - a.pB) + diff * ((a.c2 + a.c3) - (a.c1 + a.c4)) +
-         2.0 * (a.c2 * a.c3 + a.c1 * a.c4);
+  const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
+  return 0.25 * ((n + 1.0) * t - (n - 1.0) * (a.Sab * a.Sab) + n * n - x2 - y2 - x2 * y2);
 }
 // Dz(i,i,k): a = pop i, b = pop k
-PGLD_HD double num_dz_iij(const HapP& a, const HapP& b) {
-  return (a.qA - a.pA) * (-a.D) * (-b.c1 + b.c2 - b.c3 + b.c4);
-}
+PGLD_HD double num_dz_iij(const HapP& a, const HapP& b) { return -(a.Sa * b.Sb) * a.D; }
 // Dz(i,j,i): a = pop i, b = pop j
-PGLD_HD double num_dz_iji(const HapP& a, const HapP& b) {
-  return (-a.c1 + a.c2 - a.c3 + a.c4) * (-a.D) * (b.qA - b.pA);
-}
+PGLD_HD double num_dz_iji(const HapP& a, const HapP& b) { return -(b.Sa * a.Sb) * a.D; }
 // Dz(i,j,j): a = pop i, b = pop j
-PGLD_HD double num_dz_ijj(const HapP& a, const HapP& b) {
-  return (-a.D) * ((-b.c1 + b.c2 + b.c3 - b.c4) + (-b.c1 + b.c2 - b.c3 + b.c4) * (b.qA - b.pA));
-}
+PGLD_HD double num_dz_ijj(const HapP& a, const HapP& b) { return -a.D * (b.Sa * b.Sb - b.Sab); }
 PGLD_HD double num_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
   return -(a.D * (b.pA - b.qA) * (c.c1 - c.c2 + c.c3 - c.c4));
 }
 PGLD_HD double num_pi2_iiii(const HapP& a) {
-  return a.pA * a.pB * a.qB * a.qA - a.c1 * a.c4 * (-1.0 + a.c1 + 3.0 * a.c2 + 3.0 * a.c3 + a.c4) -
-         a.c2 * a.c3 * (-1.0 + 3.0 * a.c1 + a.c2 + a.c3 + 3.0 * a.c4);
+  const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
+  return 0.0625 * ((n * n - 4.0 * n + 2.0) * (n * n - x2 - y2) + x2 * y2 - 4.0 * t + 2.0 * (a.Sab * a.Sab));
 }
 // three samples of pop t, one of pop s (haplotype_kernels.py:268-289)
 PGLD_HD double num_pi2_triple(const HapP& t, const HapP& s) {
-  return (-(t.pA * t.c4 * s.pB) - (t.c2 * t.qA * s.pB) + (t.pA * (t.c2 + t.c4) * t.qA * s.pB) +
-          (t.pA * t.qA * (-2.0 * s.c2 - 2.0 * s.c4)) + (t.pA * t.c4 * s.qB) + (t.c2 * t.qA * s.qB) +
-          (t.pA * (t.c1 + t.c3) * t.qA * s.qB)) * 0.5;
+  return 0.0625 * ((t.n - 2.0) * (t.n * (t.n * s.n - t.Sb * s.Sb) - s.n * (t.Sa * t.Sa)) +
+                   (t.Sa * s.Sb) * (t.Sa * t.Sb - 2.0 * t.Sab));
 }
 PGLD_HD double num_pi2_iikk(const HapP& a, const HapP& b) { return a.pA * a.qA * (b.pB * b.qB); }
 PGLD_HD double num_pi2_iikk_sym(const HapP& a, const HapP& b) {
-  return 0.5 * (a.pA * a.qA * b.pB * b.qB + b.pA * b.qA * a.pB * a.qB);
+  const double na2 = a.n * a.n, nb2 = b.n * b.n;
+  return 0.03125 * (2.0 * na2 * nb2 - na2 * (b.Sa * b.Sa + b.Sb * b.Sb) - nb2 * (a.Sa * a.Sa + a.Sb * a.Sb) +
+                    (a.Sa * b.Sb) * (a.Sa * b.Sb) + (b.Sa * a.Sb) * (b.Sa * a.Sb));
 }
 // pi2(i,i,k,l), denominator 2 n_i(n_i-1) n_k n_l
 PGLD_HD double num_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
@@ -101,18 +110,13 @@ PGLD_HD double num_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
 PGLD_HD double num_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
   return c.pB * c.qB * (a.pA * b.qA + a.qA * b.pA);
 }
-// pi2(i,j,i,j) (exactly symmetric in the two populations).  The reference's
-// 4 quartic products and 36-monomial cubic correction (haplotype_kernels.py:
-// 243-266) regrouped by marginals: the quartic part factors as
-// (qA_a pA_b + pA_a qA_b)(qB_a pB_b + pB_a qB_b), and the cubic part pairs each
-// haplotype class of one population with the opposite class of the other.
-// Every term is an integer, so the value is exactly the reference's.
+// pi2(i,j,i,j) (exactly symmetric in the two populations; the reference's 4
+// quartic products and 36-monomial cubic correction, haplotype_kernels.py:
+// 243-266, in centred moments)
 PGLD_HD double num_pi2_ijij(const HapP& A, const HapP& B) {
-  const double quart = (A.qA * B.pA + A.pA * B.qA) * (A.qB * B.pB + A.pB * B.qB);
-  const double lin = A.c4 * B.c1 + A.c3 * B.c2 + A.c2 * B.c3 + A.c1 * B.c4;
-  const double qa = B.c1 * (A.qB * A.qA) + B.c2 * (A.pB * A.qA) + B.c3 * (A.pA * A.qB) + B.c4 * (A.pA * A.pB);
-  const double qb = A.c4 * (B.pA * B.pB) + A.c3 * (B.pA * B.qB) + A.c2 * (B.pB * B.qA) + A.c1 * (B.qB * B.qA);
-  return (quart + lin - qa - qb) * 0.25;
+  const double P = A.Sa * B.Sa, Q = A.Sb * B.Sb;
+  return 0.0625 * ((A.n - 1.0) * (B.n - 1.0) * (A.n * B.n - P - Q) + P * Q - (A.Sa * A.Sb) * B.Sab -
+                   (B.Sa * B.Sb) * A.Sab + A.Sab * B.Sab);
 }
 // one population shared between the two locus pairs (haplotype_kernels.py:218-241)
 PGLD_HD double num_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {

@@ -353,3 +353,54 @@ def test_genotype_numerators_centred_moment_identity():
         assert dd == 4 * (F(int(O.eval_poly(O._DD_POLY, col)[0]), 16) + D * D), g
         assert dz == F(int(O.eval_poly(O._DZ_QUARTIC, col)[0]), 4) + A * B * D, g
         assert pi2 == pA * pB * qB * qA + F(int(O.eval_poly(O._PI2_CORR, col)[0]), 16), g
+
+
+def test_haplotype_numerators_centred_moment_identity():
+    """The HIP path (ld_math.h num_dd_single ... num_pi2_ijij) evaluates the one-
+    and two-population haplotype numerators in the centred moments n, Sa, Sb,
+    Sab (a' = 2a - 1, b' = 2b - 1) instead of the reference's count polynomials
+    (restated in the oracle, pinned there by brute force).  For counts this
+    small every numerator and denominator is exact in float64, so the
+    statistics must agree exactly."""
+    rng = np.random.default_rng(77)
+    c = [rng.integers(0, 40, size=(500, 4)) for _ in range(2)]
+    c[0][:7] = [[4, 0, 0, 0], [0, 0, 0, 5], [1, 1, 1, 1], [9, 0, 0, 3], [0, 6, 2, 0], [3, 3, 3, 3], [2, 0, 1, 1]]
+    c = [x[x.sum(1) >= 4] for x in c]
+    m = min(len(c[0]), len(c[1]))
+    c = [x[:m] for x in c]
+    P = [O.HapPop(x) for x in c]
+
+    def mom(p):
+        return p.n, p.pA - p.qA, p.pB - p.qB, (p.c1 + p.c4) - (p.c2 + p.c3), p.D
+
+    def ff(n, k):
+        out = np.ones_like(n)
+        for t in range(k):
+            out = out * (n - t)
+        return out
+
+    (n0, a0, b0, s0, D0), (n1, a1, b1, s1, D1) = mom(P[0]), mom(P[1])
+    t0 = a0 * b0 * s0
+    dd = 0.0625 * ((n0 - 1) * ((n0 - 2) * s0 * s0 - 2 * t0) + (n0 - 2) * (a0 * a0 + b0 * b0 - n0 * n0)
+                   + a0 * a0 * b0 * b0)
+    dz = 0.25 * ((n0 + 1) * t0 - (n0 - 1) * s0 * s0 + n0 * n0 - a0 * a0 - b0 * b0 - a0 * a0 * b0 * b0)
+    pi2 = 0.0625 * ((n0 * n0 - 4 * n0 + 2) * (n0 * n0 - a0 * a0 - b0 * b0) + a0 * a0 * b0 * b0 - 4 * t0
+                    + 2 * s0 * s0)
+    f4 = ff(n0, 4)
+    np.testing.assert_array_equal(dd / f4, O.hap_dd(P, 0, 0))
+    np.testing.assert_array_equal(dz / f4, O.hap_dz(P, 0, 0, 0))
+    np.testing.assert_array_equal(pi2 / f4, O.hap_pi2_kernel(P, 0, 0, 0, 0))
+    np.testing.assert_array_equal(-(a0 * b1) * D0 / (n1 * ff(n0, 3)), O.hap_dz(P, 0, 0, 1))
+    np.testing.assert_array_equal(-(a1 * b0) * D0 / (n1 * ff(n0, 3)), O.hap_dz(P, 0, 1, 0))
+    np.testing.assert_array_equal(-D0 * (a1 * b1 - s1) / (ff(n0, 2) * ff(n1, 2)), O.hap_dz(P, 0, 1, 1))
+    tri = 0.0625 * ((n0 - 2) * (n0 * (n0 * n1 - b0 * b1) - n1 * a0 * a0) + (a0 * b1) * (a0 * b0 - 2 * s0))
+    np.testing.assert_array_equal(tri / (n1 * ff(n0, 3)), O.hap_pi2_kernel(P, 0, 0, 0, 1))
+    sym = 0.03125 * (2 * n0 * n0 * n1 * n1 - n0 * n0 * (a1 * a1 + b1 * b1) - n1 * n1 * (a0 * a0 + b0 * b0)
+                     + (a0 * b1) ** 2 + (a1 * b0) ** 2)
+    np.testing.assert_allclose(sym / (ff(n0, 2) * ff(n1, 2)),
+                               0.5 * (O.hap_pi2_kernel(P, 0, 0, 1, 1) + O.hap_pi2_kernel(P, 1, 1, 0, 0)),
+                               rtol=1e-15, atol=0)
+    Pq, Qq = a0 * a1, b0 * b1
+    ijij = 0.0625 * ((n0 - 1) * (n1 - 1) * (n0 * n1 - Pq - Qq) + Pq * Qq - (a0 * b0) * s1 - (a1 * b1) * s0
+                     + s0 * s1)
+    np.testing.assert_array_equal(ijij / (ff(n0, 2) * ff(n1, 2)), O.hap_pi2_kernel(P, 0, 1, 0, 1))
