@@ -163,8 +163,11 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // statistics of every band pair with eval(r, i, out), keys the pair and keeps
   // per-lane running sums that are folded into this wave's LDS row at key
   // changes: statistic s goes to column col0 + s * cstride, the pair count (if
-  // `count`) to the last column.
-  auto epilogue = [&](auto m_tag, int col0, int cstride, bool count, auto&& eval) {
+  // `count`) to the last column.  Pair keys can be kept between passes over
+  // the same sub-tile: kmode 1 records them in kcode (one byte per row: 0 = no
+  // key, else key - klo + 1), kmode 2 reads them back, kmode 0 neither.
+  v4u kcode = {0u, 0u, 0u, 0u};
+  auto epilogue = [&](auto m_tag, int col0, int cstride, bool count, int kmode, auto&& eval) {
     constexpr int M = decltype(m_tag)::value;
     double lacc[M + 1];
 #pragma unroll
@@ -199,24 +202,31 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     for (int r = 0; r < 16; ++r) {
       const int il = wr + acc_row(r, lane);
       const int64_t i = (int64_t)tile.i0 + il;
-      bool valid = i < j && j < ba.n_var && (s_pos_j[jl] - s_pos_i[il]) <= ba.max_dist;
       int key = -1;
-      if (valid) {
-        if (ba.key_mode == PGLD_KEY_WINDOW) {
-          key = s_wkey[il];
-        } else {
-          double d = s_kc_j[jl] - s_kc_i[il];
-          if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
-          if (!(d >= e_lo) || !(d < e_hi))
-            key = -1;
-          else if (klo == khi)
-            key = klo;
-          else
-            key = edges_in_lds ? key_bin_in(d, s_edges, klo, khi + 1) : key_bin_in(d, ba.edges, klo, khi + 1);
+      if (kmode == 2) {
+        const int code = (int)((kcode[r >> 2] >> (8 * (r & 3))) & 0xffu);
+        if (code) key = key_base + klo + code - 1;
+      } else {
+        const bool valid = i < j && j < ba.n_var && (s_pos_j[jl] - s_pos_i[il]) <= ba.max_dist;
+        if (valid) {
+          if (ba.key_mode == PGLD_KEY_WINDOW) {
+            key = s_wkey[il];
+          } else {
+            double d = s_kc_j[jl] - s_kc_i[il];
+            if (ba.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
+            if (!(d >= e_lo) || !(d < e_hi))
+              key = -1;
+            else if (klo == khi)
+              key = klo;
+            else
+              key = edges_in_lds ? key_bin_in(d, s_edges, klo, khi + 1) : key_bin_in(d, ba.edges, klo, khi + 1);
+          }
+          if (key >= 0) key += key_base;
         }
-        if (key >= 0) key += key_base;
+        if (kmode == 1 && key >= 0)
+          kcode[r >> 2] |= (uint32_t)(key - key_base - klo + 1) << (8 * (r & 3));
       }
-      const bool has = valid && key >= 0;
+      const bool has = key >= 0;
       const bool need = has && ck >= 0 && key != ck;
       if (__ballot(need)) flush(need);
       if (has) {
@@ -242,8 +252,10 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     for (int p = 0; p < P; ++p) {
       block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live,
                                       opbuf, acc);
+      // the first population's pass records the pair keys, the others reuse them
+      const int kmode = tile.n_slots > 255 ? 0 : (p == 0 ? 1 : 2);
       if (wave_live)
-        epilogue(std::integral_constant<int, S1::N>{}, p, P, p == 0, [&](int r, int64_t i, double* out) {
+        epilogue(std::integral_constant<int, S1::N>{}, p, P, p == 0, kmode, [&](int r, int64_t i, double* out) {
           int c[NC];
           elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, i, j, pa.n_samples[p], c);
           typename PopVal<FAMILY>::T V = PopVal<FAMILY>::make(c);
@@ -270,7 +282,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       }
     }
     if (wave_live)
-      epilogue(std::integral_constant<int, SS::N>{}, 0, 1, true, [&](int r, int64_t i, double* out) {
+      epilogue(std::integral_constant<int, SS::N>{}, 0, 1, true, 0, [&](int r, int64_t i, double* out) {
         typename PopVal<FAMILY>::T V[P];
 #pragma unroll
         for (int p = 0; p < P - 1; ++p) {

@@ -60,6 +60,7 @@ namespace pgld {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 constexpr int BT = 64;       // tile edge (SNPs)
 constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
