@@ -36,11 +36,11 @@ sys.path.insert(0, ROOT)
 PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
-# FETCH_SIZE (2,187,585 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
+# FETCH_SIZE (2,176,170 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
 # by the same counters' reading of a 1 GiB device copy (--copy-calibration:
 # FETCH x 2.000, WRITE x 1.000).  Summary and commands:
 # profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
-TRAFFIC_BYTES_PER_LAUNCH = 4.484e9
+TRAFFIC_BYTES_PER_LAUNCH = 4.461e9
 
 
 def parse():
@@ -226,6 +226,21 @@ def run_windowed_hap(args, torch):
                "sample": f"{len(ii)} random pairs of the same band (of {plan.n_pairs}), both populations, "
                          f"counts + 15 statistics + window sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
         log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
+        # the BLAS-tiled exact CPU variant (float32 indicator GEMMs on every
+        # host thread BLAS uses, FP64 statistics) on the first left SNPs
+        i_hi = min(n_var, 20000)
+        per = np.searchsorted(pos, pos[:i_hi] + max_dist, side="right") - np.arange(i_hi) - 1
+        n_tiled = int(np.maximum(per, 0).sum())
+        t2 = time.perf_counter()
+        O.tiled_window_sums_hap(hap, pos, pops, max_dist, window, n_keys, 0, i_hi)
+        tiled_s = time.perf_counter() - t2
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu_tiled = {"value": n_tiled * sum(S) / tiled_s / 1e9, "unit": "Gpair*samples/s", "cores": threads,
+                     "kind": "port",
+                     "sample": f"the {n_tiled} band pairs of the first {i_hi} left SNPs, float32 indicator GEMMs "
+                               f"(BLAS, {threads} threads) + 15 FP64 statistics (NumPy) + window sums, "
+                               f"{tiled_s:.1f} s"}
+        log(f"[bench] cpu tiled {cpu_tiled['value']:.4f} Gpair*samples/s over {tiled_s:.1f} s")
 
     line = base_line(args, value, ms_step,
                      f"batched windowed haplotype LD: {n_var:,} SNPs x {n_hap:,} haplotypes (2 pops x "
@@ -234,7 +249,8 @@ def run_windowed_hap(args, torch):
                      {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples})
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
-                 "fused_vs_plain_max_rel": check, "cpu_baseline": cpu})
+                 "fused_vs_plain_max_rel": check, "cpu_baseline": cpu,
+                 "cpu_baseline_tiled": cpu_tiled if cpu else None})
     return line
 
 

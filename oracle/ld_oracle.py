@@ -21,6 +21,12 @@ Pinning (how each part is checked independently of this transcription):
   dosage kernels (single population: exact; multi-population cases: the
   same construction with distinctness within a population).
 * bin/window sums -- brute-force per-key sums over explicit pair lists.
+* dense r^2 / sigma_D^2 / dosage r^2 matrices, ZnS, Omega -- equal to the
+  pinned count r^2 / numerator ratio / Rogers-Huff r^2 where they coincide;
+  Omega by brute force over partitions.
+* locate_unlinked -- known answers and the reference's property tests
+  (tests/test_ld_r_squared.py:124-146).
+* tiled_window_sums_hap -- the per-pair window sums of this module.
 
 Functions with no independent pin are marked "parity unpinned" in their
 docstring; there are none at present except ``rogers_huff_r``'s parity with
@@ -513,6 +519,43 @@ def two_pop_stats(c0, c1, nv0=None, nv1=None):
             pi(0, 0, 0, 0), pi(0, 0, 0, 1), pi(0, 0, 1, 1), pi(0, 1, 0, 1), pi(0, 1, 1, 1),
             pi(1, 1, 1, 1)]
     return np.stack(cols, axis=1)
+
+
+def tiled_window_sums_hap(hap, pos, pops, max_dist, window, n_keys, i_lo=0, i_hi=None, block=512):
+    """BLAS-tiled exact CPU restatement of the windowed two-population band
+    sums over the left SNPs [i_lo, i_hi): the reference's GEMM form of the
+    counts (_tile_counts, ld_statistics.py:281-298) as float32 products of 0/1
+    indicator planes (exact while counts < 2^24; multi-threaded BLAS), then the
+    FP64 statistics of two_pop_stats and window-of-the-left-SNP sums.  The
+    multi-core CPU baseline of the band pass; pinned against the per-pair
+    oracle in the test suite."""
+    x = np.asarray(hap)
+    pos = np.asarray(pos)
+    i_hi = x.shape[1] if i_hi is None else i_hi
+    planes = []
+    for r in pops:
+        xr = x[np.asarray(r, dtype=np.int64), :]
+        planes.append(((xr == 1).astype(np.float32), (xr >= 0).astype(np.float32), bool(np.any(xr == -1))))
+    acc = np.zeros((n_keys, 15))
+    for b0 in range(i_lo, i_hi, block):
+        b1 = min(b0 + block, i_hi)
+        j1 = int(np.searchsorted(pos, pos[b1 - 1] + max_dist, side="right"))
+        ii, jj = np.meshgrid(np.arange(b0, b1), np.arange(b0, j1), indexing="ij")
+        m = (jj > ii) & (pos[jj] - pos[ii] <= max_dist)
+        if not m.any():
+            continue
+        cl = []
+        for H, V, miss in planes:
+            hh = H[:, b0:b1].T @ H[:, b0:j1]
+            hv = H[:, b0:b1].T @ V[:, b0:j1]
+            vh = V[:, b0:b1].T @ H[:, b0:j1]
+            vv = V[:, b0:b1].T @ V[:, b0:j1]
+            hh, hv, vh, vv = (a[m].astype(np.int64) for a in (hh, hv, vh, vv))
+            c = np.stack([hh, hv - hh, vh - hh, vv - hv - vh + hh], axis=1)
+            cl.append((c, vv if miss else None))
+        st = two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+        acc += keyed_sums(window_key(pos[ii[m]], 0.0, window), st, n_keys)
+    return acc
 
 
 def single_pop_stats_hap(counts, n_valid=None):

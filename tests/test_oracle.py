@@ -404,3 +404,51 @@ def test_haplotype_numerators_centred_moment_identity():
     ijij = 0.0625 * ((n0 - 1) * (n1 - 1) * (n0 * n1 - Pq - Qq) + Pq * Qq - (a0 * b0) * s1 - (a1 * b1) * s0
                      + s0 * s1)
     np.testing.assert_array_equal(ijij / (ff(n0, 2) * ff(n1, 2)), O.hap_pi2_kernel(P, 0, 1, 0, 1))
+
+
+def test_tiled_window_sums_equal_per_pair_oracle():
+    """The BLAS-tiled CPU variant (float32 indicator GEMMs + FP64 statistics)
+    gives the per-pair oracle's window sums."""
+    rng = np.random.default_rng(31)
+    n_hap, m = 120, 700
+    hap = (rng.random((n_hap, m)) < rng.uniform(0.05, 0.6, m)).astype(np.int8)
+    hap[rng.random((n_hap, m)) < 0.03] = -1
+    pos = np.sort(rng.choice(np.arange(1, 70_000), m, replace=False)).astype(np.float64)
+    pops = [list(range(0, 60)), list(range(60, 120))]
+    md, width = 2500.0, 10_000.0
+    n_keys = int(pos[-1] // width) + 1
+    got = O.tiled_window_sums_hap(hap, pos, pops, md, width, n_keys, block=128)
+    ii, jj = O.pairs_within_distance(pos, md)
+    cl = [O.hap_counts(hap, ii, jj, r) for r in pops]
+    st = O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
+    want = O.keyed_sums(O.window_key(pos[ii], 0.0, width), st, n_keys)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    part = O.tiled_window_sums_hap(hap, pos, pops, md, width, n_keys, i_lo=100, i_hi=300, block=64)
+    sel = (ii >= 100) & (ii < 300)
+    np.testing.assert_allclose(part, O.keyed_sums(O.window_key(pos[ii[sel]], 0.0, width), st[sel], n_keys),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_locate_unlinked_known_answers():
+    """Known answers of the greedy window pruning (haplotype_matrix.py:
+    1172-1231): duplicates of an earlier kept variant in the same window are
+    pruned, unlinked ones kept, r^2 equal to the threshold is kept (strict >),
+    a variant pruned in one window is skipped later, and windows only compare
+    variants within `size` of each other.  Also the reference's own property
+    tests (tests/test_ld_r_squared.py:124-146)."""
+    A = [0, 0, 0, 0, 1, 1, 1, 1]
+    B = [0, 1, 0, 1, 0, 1, 0, 1]
+    C = [0, 0, 1, 1, 0, 0, 1, 1]
+    h = np.array([A, A, B, C, B, A], dtype=np.int8).T
+    np.testing.assert_array_equal(O.locate_unlinked(h, size=6, step=6, threshold=0.1),
+                                  [True, False, True, True, False, False])
+    assert O.locate_unlinked(h, size=6, step=6, threshold=1.0).all()
+    h3 = np.array([A, A, A], dtype=np.int8).T
+    np.testing.assert_array_equal(O.locate_unlinked(h3, size=2, step=1, threshold=0.1), [True, False, True])
+    np.testing.assert_array_equal(O.locate_unlinked(h3, size=3, step=1, threshold=0.1), [True, False, False])
+    rng = np.random.RandomState(42)
+    hap = rng.randint(0, 2, (20, 50)).astype(np.int8)
+    loc = O.locate_unlinked(hap, size=10, step=5, threshold=0.1)
+    assert loc.shape == (50,) and loc.dtype == bool and 0 < loc.sum() <= 50
+    hap = rng.randint(0, 2, (20, 20)).astype(np.int8)
+    assert O.locate_unlinked(hap, size=10, step=5, threshold=1.0).all()
