@@ -390,6 +390,32 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopAr
 }
 
 // Per-key reduction over tile partial rows, in ascending tile order.
+// First level of the two-level key reduction: chunk c adds its slots (entries
+// [lohi[2c], lohi[2c + 1]) of the key list) into part2[c][s], in the same fixed
+// order as k_band_reduce.
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ partials,
+                                                       const int64_t* __restrict__ key_list,
+                                                       const int64_t* __restrict__ lohi, int32_t ns_total,
+                                                       double* __restrict__ part2) {
+  __shared__ double buf[256];
+  const int64_t c = blockIdx.x;
+  const int64_t lo = lohi[2 * c], hi = lohi[2 * c + 1];
+  for (int s = 0; s < ns_total; ++s) {
+    double x = 0.0;
+    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) x += partials[key_list[e] * ns_total + s];
+    buf[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part2[c * ns_total + s] = buf[0];
+    __syncthreads();
+  }
+}
+
+// Per-key sums over the key's slots (key_list entries key_off[k] ..
+// key_off[k + 1]; with key_list null the entries are the slots themselves).
 __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ partials,
                                                      const int32_t* __restrict__ key_off,
                                                      const int64_t* __restrict__ key_list, int32_t ns_total,
@@ -400,7 +426,7 @@ __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ 
   const int lo = key_off[k], hi = key_off[k + 1];
   for (int s = 0; s < ns_total; ++s) {
     double x = 0.0;
-    for (int e = lo + threadIdx.x; e < hi; e += 256) x += partials[key_list[e] * ns_total + s];
+    for (int e = lo + threadIdx.x; e < hi; e += 256) x += partials[(key_list ? key_list[e] : e) * ns_total + s];
     buf[threadIdx.x] = x;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -711,6 +737,35 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   if (!rc) rc = up(&plan->d_tiles, tb.tiles.data(), tb.tiles.size() * sizeof(BandTile));
   if (!rc) rc = up((void**)&plan->d_pair_base, base.empty() ? nullptr : base.data(), base.size() * sizeof(int64_t));
   if (!rc) rc = up((void**)&plan->d_key_off, key_off.data(), key_off.size() * sizeof(int32_t));
+  {
+    // keys whose slot lists are long (e.g. few distance bins over many tiles)
+    // are summed in two fixed-order levels: chunks of RED_CHUNK slots, then the
+    // chunks of each key.  A key of at most RED_CHUNK slots sums to the same
+    // value either way, so every plan holding a given key (a region batch or
+    // the region's own band) reduces it identically.
+    int64_t longest = 0;
+    for (int k = 0; k < n_keys; ++k) longest = std::max<int64_t>(longest, key_off[k + 1] - key_off[k]);
+    if (!rc && longest > RED_CHUNK) {
+      std::vector<int64_t> lo;
+      std::vector<int32_t> off(n_keys + 1, 0);
+      for (int k = 0; k < n_keys; ++k) {
+        off[k] = (int32_t)lo.size();
+        for (int64_t e = key_off[k]; e < key_off[k + 1]; e += RED_CHUNK) lo.push_back(e);
+      }
+      off[n_keys] = (int32_t)lo.size();
+      std::vector<int64_t> hi(lo.size());
+      for (int k = 0; k < n_keys; ++k)
+        for (int32_t c = off[k]; c < off[k + 1]; ++c) hi[c] = std::min<int64_t>(lo[c] + RED_CHUNK, key_off[k + 1]);
+      std::vector<int64_t> lohi(2 * lo.size());
+      for (size_t c = 0; c < lo.size(); ++c) {
+        lohi[2 * c] = lo[c];
+        lohi[2 * c + 1] = hi[c];
+      }
+      plan->n_red_chunks = (int64_t)lo.size();
+      rc = up((void**)&plan->d_red_lo, lohi.data(), lohi.size() * sizeof(int64_t));
+      if (!rc) rc = up((void**)&plan->d_red_off, off.data(), off.size() * sizeof(int32_t));
+    }
+  }
   if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
                    key_list.size() * sizeof(int64_t));
   if (!rc && plan->n_regions == 0) {
@@ -988,7 +1043,7 @@ int pgld_band_plan_destroy(pgld_band_plan* plan) {
   void* ptrs[] = {plan->d_pos,      plan->d_kc,       plan->d_edges,     plan->d_tiles,    plan->d_pair_base,
                   plan->d_key_off,  plan->d_key_list, plan->d_partials,  plan->d_regions,  plan->d_order,
                   plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt,
-                  plan->d_super};
+                  plan->d_super,    plan->d_red_lo,   plan->d_red_off};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)
@@ -1060,8 +1115,10 @@ static size_t sums_lds_bytes(const pgld_band_plan* plan, int32_t family, int32_t
   return red_bytes + stash_bytes + op_bytes;
 }
 
+// Workspace: one partial row per (tile, key) slot, then one per reduction
+// chunk when the plan reduces in two levels.
 static int ensure_partials(pgld_band_plan* pl, int64_t ns_total) {
-  const size_t need = (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total;
+  const size_t need = (size_t)(std::max<int64_t>(1, pl->n_slots) + pl->n_red_chunks) * ns_total;
   if (pl->partials_cap >= need) return PGLD_OK;
   if (pl->d_partials) PGLD_TRY_HIP(hipFree(pl->d_partials), "band workspace free");
   pl->d_partials = nullptr;
@@ -1095,7 +1152,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
   if (allow_alloc) {
     int rc = ensure_partials(pl, ns_total);
     if (rc) return rc;
-  } else if (pl->partials_cap < (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total) {
+  } else if (pl->partials_cap < (size_t)(std::max<int64_t>(1, pl->n_slots) + pl->n_red_chunks) * ns_total) {
     set_error("band sums: workspace not bound for this statistic set");
     return PGLD_EINVAL;
   }
@@ -1120,6 +1177,16 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
     int rc = check_launch("k_band_sums");
     if (rc) return rc;
     if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[1], s), "band sums event");
+  }
+  if (pl->n_red_chunks > 0) {
+    double* part2 = pl->d_partials + (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)pl->n_red_chunks), dim3(256), 0, s, pl->d_partials,
+                       pl->d_key_list, pl->d_red_lo, ns_total, part2);
+    int rc = check_launch("k_reduce_chunks");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)pl->n_keys), dim3(256), 0, s, part2, pl->d_red_off,
+                       (const int64_t*)nullptr, ns_total, n_stats, sums, key_pairs);
+    return check_launch("k_band_reduce");
   }
   hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)pl->n_keys), dim3(256), 0, s, pl->d_partials, pl->d_key_off,
                      pl->d_key_list, ns_total, n_stats, sums, key_pairs);

@@ -26,6 +26,12 @@ struct pgld_band_plan {
   int64_t* d_pair_base = nullptr;  // exclusive prefix of per-variant pair counts
   int32_t* d_key_off = nullptr;    // n_keys + 1
   int64_t* d_key_list = nullptr;   // slots grouped by key, ascending tile order
+  // two-level key reduction (keys with many slots): chunk c sums key_list
+  // entries [d_red_lo[c], d_red_lo[c + 1]) of one key; key k owns chunks
+  // [d_red_off[k], d_red_off[k + 1])
+  int64_t n_red_chunks = 0;
+  int64_t* d_red_lo = nullptr;
+  int32_t* d_red_off = nullptr;
   double* d_partials = nullptr;    // workspace
   size_t partials_cap = 0;         // doubles
   bool timing = false;             // record events around each k_band_sums launch
@@ -66,6 +72,7 @@ constexpr int BT = 64;       // tile edge (SNPs)
 constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
 constexpr int BAND_THREADS = 256;
 constexpr int RED_LDS_BYTES = 64 * 1024;
+constexpr int RED_CHUNK = 1024;     // slots per first-level chunk of a long key reduction
 constexpr int MAX_LDS_EDGES = 130;  // bin edges kept in LDS when n_keys < 130
 // static LDS of k_band_sums: positions, key coordinates, window keys, edges
 constexpr int STATIC_LDS_BYTES = 4 * 64 * 8 + 64 * 4 + MAX_LDS_EDGES * 8;
