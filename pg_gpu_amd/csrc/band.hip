@@ -167,6 +167,28 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // the same sub-tile: kmode 1 records them in kcode (one byte per row: 0 = no
   // key, else key - klo + 1), kmode 2 reads them back, kmode 0 neither.
   v4u kcode = {0u, 0u, 0u, 0u};
+  // A sub-tile whose pairs are all valid (j > i everywhere, inside the panel,
+  // the farthest pair within max_dist) and share one key (keys are monotone
+  // in the left SNP's window and in the distance, so the extreme pairs decide)
+  // needs no per-pair key or validity work: fast_key is that key, else -1.
+  int fast_key = -1;
+  if (wave_live && rj > ri + 31 && (int64_t)rj + 31 < ba.n_var && s_pos_j[wc + 31] - s_pos_i[wr] <= ba.max_dist) {
+    if (ba.key_mode == PGLD_KEY_WINDOW) {
+      const int k0 = s_wkey[wr], k1 = s_wkey[wr + 31];
+      if (k0 >= 0 && k0 == k1) fast_key = k0;
+    } else {
+      const double d_lo = s_kc_j[wc] - s_kc_i[wr + 31], d_hi = s_kc_j[wc + 31] - s_kc_i[wr];
+      if (d_lo >= e_lo && d_hi < e_hi) {
+        const int b_lo = klo == khi ? klo
+                                    : (edges_in_lds ? key_bin_in(d_lo, s_edges, klo, khi + 1)
+                                                    : key_bin_in(d_lo, ba.edges, klo, khi + 1));
+        const int b_hi = klo == khi ? klo
+                                    : (edges_in_lds ? key_bin_in(d_hi, s_edges, klo, khi + 1)
+                                                    : key_bin_in(d_hi, ba.edges, klo, khi + 1));
+        if (b_lo == b_hi) fast_key = b_lo;
+      }
+    }
+  }
   auto epilogue = [&](auto m_tag, int col0, int cstride, bool count, int kmode, auto&& eval) {
     constexpr int M = decltype(m_tag)::value;
     double lacc[M + 1];
@@ -198,6 +220,20 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         ck = -1;
       }
     };
+    if (fast_key >= 0) {
+#pragma unroll 1
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = (int64_t)tile.i0 + wr + acc_row(r, lane);
+        double out[M];
+        eval(r, i, out);
+#pragma unroll
+        for (int s = 0; s < M; ++s) lacc[s] += out[s];
+        lacc[M] += 1.0;
+      }
+      ck = fast_key + key_base;
+      flush(true);
+      return;
+    }
 #pragma unroll 1
     for (int r = 0; r < 16; ++r) {
       const int il = wr + acc_row(r, lane);
