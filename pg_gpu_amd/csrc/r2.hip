@@ -439,13 +439,28 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
   }
 
   // ---- per-pair r^2, histogram, sparse list, per-pair output ----
+  // Most pairs land in histogram cell 0 (first key, r^2 below 1 / n_hist):
+  // each lane counts those in a register and the wave adds the total once,
+  // instead of 64 lanes' atomics on one LDS address per row.
+  int cell0 = 0;
   if (live) {
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    // one 32 x 32 block at a time (a runtime loop keeps the unrolled row body
+    // count at 16)
+#pragma unroll 1
+    for (int blk = 0; blk < 4; ++blk) {
+      const int a = blk >> 1, b = blk & 1;
+      v16i cur;
+      if (blk == 0)
+        cur = acc[0][0];
+      else if (blk == 1)
+        cur = acc[0][1];
+      else if (blk == 2)
+        cur = acc[1][0];
+      else
+        cur = acc[1][1];
       const int jl = wc + 32 * b + (lane & 31);
       const int64_t j = (int64_t)st.j0 + jl;
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
+      {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int il = wr + 32 * a + acc_row(r, lane);
@@ -457,8 +472,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
             bool is_nan = false;
             double sval = 0.0;
             int scnt = 0;
-            const double r2 = hap_pair_value<false, EST>(acc[a][b][r], marg[i], marg[j], S, marg, ra.n_var, i, j,
-                                                         S, is_nan, sval, scnt);
+            const double r2 = hap_pair_value<false, EST>(cur[r], marg[i], marg[j], S, marg, ra.n_var, i, j, S,
+                                                         is_nan, sval, scnt);
             r2f = (float)r2;
             if (ra.hist) {
               int key;
@@ -473,7 +488,11 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
               if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
                 int hb = ra.n_hist;
                 if (!is_nan) hb = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
-                atomicAdd(&wh[slot * hw + hb], 1);
+                const int cell = slot * hw + hb;
+                if (cell == 0)
+                  ++cell0;
+                else
+                  atomicAdd(&wh[cell], 1);
               }
             }
             hit = !is_nan && r2 > ra.thresh;
@@ -499,6 +518,11 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
         }
       }
     }
+  }
+  if (h_n > 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cell0 += __shfl_xor(cell0, o);
+    if (lane == 0 && cell0) wh[0] += cell0;
   }
   // the wave's histogram quarter is written by this wave only
   for (int t = lane; t < h_n; t += 64) {
