@@ -147,6 +147,9 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   block_products<FAMILY, MISSING, POPC>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
                                                  acc);
 
+  // pairs of histogram cell 0 (first key, r^2 below 1 / n_hist) are counted
+  // in a register and added once per wave
+  int cell0 = 0;
   if (wave_live) {
     const int jl = wc + (lane & 31);
     const int64_t j = (int64_t)tile.j0 + jl;
@@ -221,7 +224,10 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
             int b = ra.n_hist;
             if (!is_nan) b = max(0, min((int)(val * (double)ra.n_hist), ra.n_hist - 1));
-            atomicAdd(&s_hist[slot * hw + b], 1);
+            if (slot * hw + b == 0)
+              ++cell0;
+            else
+              atomicAdd(&s_hist[slot * hw + b], 1);
           }
         }
       } else if (valid && FAMILY == PGLD_FAMILY_HAP) {
@@ -251,7 +257,10 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
             int b = ra.n_hist;
             if (!is_nan) b = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
-            atomicAdd(&s_hist[slot * hw + b], 1);
+            if (slot * hw + b == 0)
+              ++cell0;
+            else
+              atomicAdd(&s_hist[slot * hw + b], 1);
           }
         }
         hit = !is_nan && r2 > ra.thresh;
@@ -313,6 +322,9 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
     }
   }
   if (ra.hist) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cell0 += __shfl_xor(cell0, o);
+    if (lane == 0 && cell0) atomicAdd(&s_hist[0], cell0);
     __syncthreads();
     for (int t = tid; t < h_n; t += BAND_THREADS) {
       const int v = s_hist[t];
