@@ -485,9 +485,15 @@ __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ 
 // thread takes 16 samples x 4 variants, turns four 4 x 4 byte blocks into
 // variant-major dwords with byte permutes, derives the plane bytes and the
 // domain / missing flags four samples at a time (SWAR on dwords) and writes
-// 16 sample bytes per (plane, variant): 8 lanes fill one 128-byte line.
+// 16 sample bytes per (plane, variant): 8 lanes fill one 128-byte line.  The
+// per-variant plane counts are summed over those 8 lanes with shuffles (four
+// variants' counts packed as bytes) and added to the margins by one lane.
 constexpr int PK_S = 128, PK_V = 128;
-constexpr int PK_ROW = PK_V / 4 + 1;  // LDS row stride in dwords
+constexpr int PK_Q = PK_V / 4;  // dword columns (variant quads) per sample row
+
+// LDS column swizzle: sample row s keeps variant quad q at column q ^ pk_swz(s).
+// Phase 2's lanes read rows 16 apart; the swizzle puts them on distinct banks.
+__device__ __forceinline__ int pk_swz(int s) { return ((s >> 4) & 3) << 3; }
 
 // 4 x 4 byte transpose: r[i] holds bytes (v0..v3) of sample i; on return
 // c[j] holds bytes (s0..s3) of variant j.
@@ -502,19 +508,23 @@ __device__ __forceinline__ void transpose4x4(const uint32_t r[4], uint32_t c[4])
   c[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
 }
 
+// Sets flags[f] unless another wave already has: one relaxed load instead of
+// thousands of same-address atomics serialised at one L2 channel.
+__device__ __forceinline__ void set_flag_once(int32_t* flags, int f) {
+  if (__hip_atomic_load(&flags[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) atomicOr(&flags[f], 1);
+}
+
 template <int FAMILY, bool WORDS>
 __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
                                               const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
                                               int8_t* __restrict__ planes, int32_t* __restrict__ marg,
                                               int32_t* __restrict__ flags) {
   constexpr int NPL = FAMILY == PGLD_FAMILY_HAP ? 2 : 3;
-  __shared__ uint32_t tile[PK_S * PK_ROW];  // [sample][variant quad], padded
-  __shared__ int msum[NPL][PK_V];
+  __shared__ uint32_t tile[PK_S * PK_Q];  // [sample][variant quad ^ pk_swz(sample)]
   __shared__ int64_t srow[PK_S];
   const int tid = threadIdx.x;
   static_assert(PK_V == 128 && PK_S == 128, "index math below assumes 128 x 128 tiles");
   const int64_t v0 = (int64_t)blockIdx.x * PK_V, k0 = (int64_t)blockIdx.y * PK_S;
-  for (int t = tid; t < NPL * PK_V; t += 256) msum[t / PK_V][t % PK_V] = 0;
   if (tid < PK_S) srow[tid] = k0 + tid < n_sel ? (int64_t)(rows ? rows[k0 + tid] : k0 + tid) * n_var : -1;
   __syncthreads();
   // phase 1: 32 threads per row (one dword = 4 variants each), 8 rows per
@@ -541,18 +551,22 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
       words_in[u] = word;
     }
 #pragma unroll
-    for (int u = 0; u < PK_S / 8; ++u) tile[(rr + 8 * u) * PK_ROW + q] = words_in[u];
+    for (int u = 0; u < PK_S / 8; ++u) {
+      const int s = rr + 8 * u;
+      tile[s * PK_Q + (q ^ pk_swz(s))] = words_in[u];
+    }
   }
   __syncthreads();
   // phase 2: thread -> (16-sample group sg, variant quad vq); the 8 lanes of
   // one variant quad cover its 128 samples
   const int sg = tid & 7, vq = tid >> 3;
+  const int qcol = vq ^ ((sg & 3) << 3);  // == vq ^ pk_swz(16 sg + anything below 16)
   uint32_t cols[4][4];  // [variant j][sample dword i]: bytes = samples 16 sg + 4 i .. + 3
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     uint32_t r[4], c[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) r[e] = tile[(16 * sg + 4 * i + e) * PK_ROW + vq];
+    for (int e = 0; e < 4; ++e) r[e] = tile[(16 * sg + 4 * i + e) * PK_Q + qcol];
     transpose4x4(r, c);
 #pragma unroll
     for (int j = 0; j < 4; ++j) cols[j][i] = c[j];
@@ -569,14 +583,14 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   }
   constexpr uint32_t ONES = 0x01010101u;
   int any_neg = 0, any_bad = 0;
+  uint32_t cnt4[NPL];  // byte j = this lane's count of variant 4 vq + j (<= 16)
+#pragma unroll
+  for (int p = 0; p < NPL; ++p) cnt4[p] = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int v = 4 * vq + j;
     if (v0 + v >= n_var) continue;
     uint32_t pw[NPL][4];
-    int cnt[NPL];
-#pragma unroll
-    for (int p = 0; p < NPL; ++p) cnt[p] = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t w = cols[j][i];
@@ -597,7 +611,7 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
       any_neg |= (neg_m & smask[i]) != 0;
       any_bad |= (bad & smask[i]) != 0;
 #pragma unroll
-      for (int p = 0; p < NPL; ++p) cnt[p] += __popc(pw[p][i]);
+      for (int p = 0; p < NPL; ++p) cnt4[p] += (uint32_t)__popc(pw[p][i]) << (8 * j);
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) {
@@ -607,18 +621,28 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
       val4.z = (int)pw[p][2];
       val4.w = (int)pw[p][3];
       *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + 16 * sg) = val4;
-      if (cnt[p]) atomicAdd(&msum[p][v], cnt[p]);
     }
   }
-  __syncthreads();
-  for (int t = tid; t < NPL * PK_V; t += 256) {
-    const int p = t / PK_V, vv = t % PK_V;
-    if (v0 + vv < n_var && msum[p][vv]) atomicAdd(&marg[(int64_t)p * n_var + v0 + vv], msum[p][vv]);
+  // sum the byte-packed counts over the 8 lanes of the quad (<= 128 per byte)
+#pragma unroll
+  for (int p = 0; p < NPL; ++p) {
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) cnt4[p] += (uint32_t)__shfl_xor((int)cnt4[p], o);
+  }
+  if (sg == 0) {
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = (cnt4[p] >> (8 * j)) & 0xff;
+        if (c) atomicAdd(&marg[(int64_t)p * n_var + v0 + 4 * vq + j], c);
+      }
+    }
   }
   unsigned long long mneg = __ballot(any_neg), mbad = __ballot(any_bad);
   if ((tid & 63) == 0) {
-    if (mneg) atomicOr(&flags[0], 1);
-    if (mbad) atomicOr(&flags[1], 1);
+    if (mneg) set_flag_once(flags, 0);
+    if (mbad) set_flag_once(flags, 1);
   }
 }
 
