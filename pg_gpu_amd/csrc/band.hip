@@ -829,12 +829,19 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   if (!rc) rc = up((void**)&plan->d_key_list, key_list.empty() ? nullptr : key_list.data(),
                    key_list.size() * sizeof(int64_t));
   if (!rc && plan->n_regions == 0) {
-    // 2 x 2 groups of tiles in row-major order of the groups
+    // 2 x 2 groups of tiles (super-tiles), ordered by 8 x 8 blocks of super-tiles
+    // and row-major inside a block.  The kernel hands each XCD a contiguous run
+    // of this list, so the 64 super-tiles an XCD holds at once (32 CUs x 2)
+    // share 8 row and 8 column strips of operands in its L2; in plain
+    // row-major order they share one row strip and re-fetch every column strip.
+    constexpr int64_t SG = 8;
     std::vector<std::pair<int64_t, int32_t>> keyed(tb.tiles.size());
     const int64_t n_sc = ceil_div(std::max<int64_t>(n_var, 1), 2 * BT) + 1;
+    const int64_t n_gc = ceil_div(n_sc, SG);
     for (size_t t = 0; t < tb.tiles.size(); ++t) {
-      const int64_t ti = tb.tiles[t].i0 / BT, tj = tb.tiles[t].j0 / BT;
-      keyed[t] = {(ti >> 1) * n_sc + (tj >> 1), (int32_t)t};
+      const int64_t si = (tb.tiles[t].i0 / BT) >> 1, sj = (tb.tiles[t].j0 / BT) >> 1;
+      const int64_t blk = (si / SG) * n_gc + sj / SG;
+      keyed[t] = {(blk * SG + si % SG) * SG + sj % SG, (int32_t)t};
     }
     std::stable_sort(keyed.begin(), keyed.end(),
                      [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
@@ -843,8 +850,9 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
     std::vector<SuperTile> sup;
     for (size_t k = 0; k < keyed.size();) {
       SuperTile st{};
-      st.i0 = (int32_t)((keyed[k].first / n_sc) * 2 * BT);
-      st.j0 = (int32_t)((keyed[k].first % n_sc) * 2 * BT);
+      const BandTile& first = tb.tiles[keyed[k].second];
+      st.i0 = (int32_t)(((first.i0 / BT) >> 1) * 2 * BT);
+      st.j0 = (int32_t)(((first.j0 / BT) >> 1) * 2 * BT);
       for (int q = 0; q < 4; ++q) st.t[q] = -1;
       const int64_t key = keyed[k].first;
       for (; k < keyed.size() && keyed[k].first == key; ++k) {
