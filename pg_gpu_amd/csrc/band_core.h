@@ -321,6 +321,45 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// Wave-wide totals of NV per-lane values, in groups of CH = min(8, NV rounded
+// up to a power of two).  Each group goes through a halving butterfly: at the
+// level of lane bit o (32, 16, ...) a lane keeps one half of its values, adds
+// the partner lane's copy of that half and hands over the other half, so a
+// group costs CH - 1 + (6 - log2 CH) shuffles instead of 6 CH.  Afterwards
+// lane l holds the total of value base + ((l >> (6 - log2 CH)) & (CH - 1)),
+// identical in the 64 / CH lanes that differ in the low bits; the lane whose
+// low bits are zero calls put(index, total).  The summation tree is fixed, so
+// the totals are deterministic.
+template <int NV, typename F>
+__device__ __forceinline__ void wave_sums_scatter(const double* vals, int lane, F&& put) {
+  constexpr int CH = NV > 4 ? 8 : NV > 2 ? 4 : NV > 1 ? 2 : 1;
+  constexpr int LG = CH == 8 ? 3 : CH == 4 ? 2 : CH == 2 ? 1 : 0;
+#pragma unroll
+  for (int base = 0; base < NV; base += CH) {
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) v[k] = base + k < NV ? vals[base + k] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      const int o = 32 >> t;
+      if (t < LG) {
+        const int h = CH >> (t + 1);
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+          const double send = up ? v[k] : v[k + h];
+          const double keep = up ? v[k + h] : v[k];
+          v[k] = keep + __shfl_xor(send, o);
+        }
+      } else {
+        v[0] += __shfl_xor(v[0], o);
+      }
+    }
+    const int idx = base + ((lane >> (6 - LG)) & (CH - 1));
+    if ((lane & ((64 >> LG) - 1)) == 0 && idx < NV) put(idx, v[0]);
+  }
+}
+
 __device__ __forceinline__ int32_t key_bin(double d, const double* edges, int32_t n_keys) {
   if (!(d >= edges[0]) || !(d < edges[n_keys])) return -1;
   int lo = 0, hi = n_keys;
