@@ -103,11 +103,11 @@ constexpr int band_min_waves(int family, int P, int set) {
 
 // One tile of the fused band pass.  key_base moves the tile's local keys
 // (distance bin or window) into the plan's key space: 0 for a single band,
-// region * n_keys_local for the tiles of a region batch.  SCATTER selects the
-// halving-butterfly flush (fewer shuffles, a few more live registers) over one
-// full wave reduction per statistic; the persistent batch kernel, which is at
-// its register limit, keeps the latter.
-template <int FAMILY, bool MISSING, int P, int SET, bool SCATTER = true>
+// region * n_keys_local for the tiles of a region batch.  The pair sums are
+// folded over the wave by the halving butterfly in groups of SCAT values
+// (more values per group: fewer shuffles, more live registers; the persistent
+// batch kernel is at its register limit and reduces one value at a time).
+template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8>
 __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
                                                int32_t key_base, unsigned char* smem) {
   using PL = Planes<FAMILY, MISSING>;
@@ -208,29 +208,16 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         const int slot = kk - tile.key_lo;
         const bool slot_ok = slot >= 0 && slot < tile.n_slots;
         double* row = wred + slot * NS;
-        if constexpr (SCATTER) {
-          if (slot_ok) {
-            double vals[M + 1];
+        if (slot_ok) {
+          double vals[M + 1];
 #pragma unroll
-            for (int s = 0; s <= M; ++s) vals[s] = mine ? lacc[s] : 0.0;
-            if (count)
-              wave_sums_scatter<M + 1>(vals, lane, [&](int s, double x) {
-                row[s < M ? col0 + s * cstride : NS - 1] += x;
-              });
-            else
-              wave_sums_scatter<M>(vals, lane, [&](int s, double x) { row[col0 + s * cstride] += x; });
-          }
-        } else {
-          const bool own = lane == 0 && slot_ok;
-#pragma unroll
-          for (int s = 0; s < M; ++s) {
-            double x = wave_sum(mine ? lacc[s] : 0.0);
-            if (own) row[col0 + s * cstride] += x;
-          }
-          if (count) {
-            double x = wave_sum(mine ? lacc[M] : 0.0);
-            if (own) row[NS - 1] += x;
-          }
+          for (int s = 0; s <= M; ++s) vals[s] = mine ? lacc[s] : 0.0;
+          if (count)
+            wave_sums_scatter<M + 1, SCAT>(vals, lane, [&](int s, double x) {
+              row[s < M ? col0 + s * cstride : NS - 1] += x;
+            });
+          else
+            wave_sums_scatter<M, SCAT>(vals, lane, [&](int s, double x) { row[col0 + s * cstride] += x; });
         }
       }
       if (flag) {
@@ -409,7 +396,7 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     const int32_t key_base = rd.key_base;
     __syncthreads();  // the previous tile's LDS is no longer read
     // rd.pa stays in global memory: the population loop indexes it at run time
-    band_sums_tile<FAMILY, MISSING, P, SET, false>(lb, rd.pa, tile, key_base, smem);
+    band_sums_tile<FAMILY, MISSING, P, SET, 1>(lb, rd.pa, tile, key_base, smem);
   }
 }
 

@@ -315,14 +315,8 @@ __device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING
 
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
-
-// Wave-wide totals of NV per-lane values, in groups of CH = min(8, NV rounded
-// up to a power of two).  Each group goes through a halving butterfly: at the
+// Wave-wide totals of NV per-lane values, in groups of CH = min(CHMAX, NV
+// rounded up to a power of two).  Each group goes through a halving butterfly: at the
 // level of lane bit o (32, 16, ...) a lane keeps one half of its values, adds
 // the partner lane's copy of that half and hands over the other half, so a
 // group costs CH - 1 + (6 - log2 CH) shuffles instead of 6 CH.  Afterwards
@@ -330,9 +324,10 @@ __device__ __forceinline__ double wave_sum(double x) {
 // identical in the 64 / CH lanes that differ in the low bits; the lane whose
 // low bits are zero calls put(index, total).  The summation tree is fixed, so
 // the totals are deterministic.
-template <int NV, typename F>
+template <int NV, int CHMAX = 8, typename F>
 __device__ __forceinline__ void wave_sums_scatter(const double* vals, int lane, F&& put) {
-  constexpr int CH = NV > 4 ? 8 : NV > 2 ? 4 : NV > 1 ? 2 : 1;
+  static_assert(CHMAX == 8 || CHMAX == 4 || CHMAX == 2 || CHMAX == 1, "group size");
+  constexpr int CH = NV > 4 && CHMAX >= 8 ? 8 : NV > 2 && CHMAX >= 4 ? 4 : NV > 1 && CHMAX >= 2 ? 2 : 1;
   constexpr int LG = CH == 8 ? 3 : CH == 4 ? 2 : CH == 2 ? 1 : 0;
 #pragma unroll
   for (int base = 0; base < NV; base += CH) {
