@@ -190,6 +190,10 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   };
   load(pre0, 0);
   if constexpr (PF == 2) load(pre1, KSTEP);
+  // The two workgroups of a CU alternate between this loop (MFMA, LDS) and
+  // the FP64 epilogue; the raised wave priority lets the K loop's
+  // instructions issue first when both contend, the epilogue fills the gaps.
+  __builtin_amdgcn_s_setprio(2);
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
   auto step = [&](v4i* pre, int64_t k0) {
     __syncthreads();  // the previous chunk has been read by every wave
@@ -258,6 +262,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   } else {
     for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) step(pre0, k0);
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // Joint counts of accumulator element r (row i, col j) in the reference's
