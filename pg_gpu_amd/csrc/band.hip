@@ -77,7 +77,7 @@ struct PopVal;
 template <>
 struct PopVal<PGLD_FAMILY_HAP> {
   using T = HapP;
-  __device__ static T make(const int* c) { return hap_pop((double)c[0], (double)c[1], (double)c[2], (double)c[3]); }
+  __device__ static T make(const int* c) { return hap_pop_int(c[0], c[1], c[2], c[3]); }
 };
 template <>
 struct PopVal<PGLD_FAMILY_GENO> {

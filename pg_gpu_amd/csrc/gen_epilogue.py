@@ -92,60 +92,60 @@ def pi2_case(q):
 
 
 def hap_call(t, q, sym):
-    """Numerator and reciprocal-product expressions of one haplotype term."""
+    """(numerator expression, its power-of-two factor, reciprocal-product
+    expression) of one haplotype term.  Numerators with a factor other than 1
+    are the unscaled poly_* forms; the factor is folded into the reciprocal
+    product by emit_set."""
     R = lambda p, k: "R[%d].i%d" % (p, k)
     H = lambda *ps: ", ".join("H[%d]" % x for x in ps)
     if t == "dd":
         a, b = q
         if a == b:
-            return "num_dd_single(%s)" % H(a), R(a, 4)
-        return "num_dd_between(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
+            return "poly_dd_single(%s)" % H(a), 0.0625, R(a, 4)
+        return "poly_dd_between(%s)" % H(a, b), 0.0625, "%s * %s" % (R(a, 2), R(b, 2))
     if t == "dz":
         a, b, c = q
         if a == b == c:
-            return "num_dz_iii(%s)" % H(a), R(a, 4)
+            return "poly_dz_iii(%s)" % H(a), 0.25, R(a, 4)
         if a == b:
-            return "num_dz_iij(%s)" % H(a, c), "%s * %s" % (R(c, 1), R(a, 3))
+            return "poly_dz_iij(%s)" % H(a, c), 0.25, "%s * %s" % (R(c, 1), R(a, 3))
         if a == c:
-            return "num_dz_iji(%s)" % H(a, b), "%s * %s" % (R(b, 1), R(a, 3))
+            return "poly_dz_iji(%s)" % H(a, b), 0.25, "%s * %s" % (R(b, 1), R(a, 3))
         if b == c:
-            return "num_dz_ijj(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
-        return "num_dz_diff(%s)" % H(a, b, c), "%s * %s * %s" % (R(a, 2), R(b, 1), R(c, 1))
+            return "poly_dz_ijj(%s)" % H(a, b), 0.25, "%s * %s" % (R(a, 2), R(b, 2))
+        return "num_dz_diff(%s)" % H(a, b, c), 1.0, "%s * %s * %s" % (R(a, 2), R(b, 1), R(c, 1))
     case, r = pi2_case(q)
     if case == "same":
-        return "num_pi2_iiii(%s)" % H(*r), R(r[0], 4)
+        return "poly_pi2_iiii(%s)" % H(*r), 0.0625, R(r[0], 4)
     if case in ("triple_a", "triple_b"):
         tp, sp = r
-        return "num_pi2_triple(%s)" % H(tp, sp), "%s * %s" % (R(sp, 1), R(tp, 3))
+        return "poly_pi2_triple(%s)" % H(tp, sp), 0.0625, "%s * %s" % (R(sp, 1), R(tp, 3))
     if case == "iikk":
         a, k = r
         if sym:  # symmetric in the two populations
             a, k = min(a, k), max(a, k)
-            return "num_pi2_iikk_sym(%s)" % H(a, k), "%s * %s" % (R(min(a, k), 2), R(max(a, k), 2))
-        return "num_pi2_iikk(%s)" % H(a, k), "%s * %s" % (R(a, 2), R(k, 2))
+            return "poly_pi2_iikk_sym(%s)" % H(a, k), 0.03125, "%s * %s" % (R(a, 2), R(k, 2))
+        return "num_pi2_iikk(%s)" % H(a, k), 1.0, "%s * %s" % (R(a, 2), R(k, 2))
     if case == "iikl":
         a, k, l = r
-        return "num_pi2_iikl(%s)" % H(a, k, l), "0.5 * %s * %s * %s" % (R(a, 2), R(k, 1), R(l, 1))
+        return "num_pi2_iikl(%s)" % H(a, k, l), 0.5, "%s * %s * %s" % (R(a, 2), R(k, 1), R(l, 1))
     if case == "ijkk":
         a, b, k = r
-        return "num_pi2_ijkk(%s)" % H(a, b, k), "0.5 * %s * %s * %s" % (R(k, 2), R(a, 1), R(b, 1))
+        return "num_pi2_ijkk(%s)" % H(a, b, k), 0.5, "%s * %s * %s" % (R(k, 2), R(a, 1), R(b, 1))
     if case == "ijij":
         a, b = min(r), max(r)  # the numerator is exactly symmetric in (i, j)
-        return "num_pi2_ijij(%s)" % H(a, b), "%s * %s" % (R(a, 2), R(b, 2))
+        return "poly_pi2_ijij(%s)" % H(a, b), 0.0625, "%s * %s" % (R(a, 2), R(b, 2))
     if case == "shared":
         s_, x, y = r
-        return "num_pi2_shared(%s)" % H(s_, x, y), "%s * %s * %s" % (R(s_, 2), R(x, 1), R(y, 1))
+        return "num_pi2_shared(%s)" % H(s_, x, y), 1.0, "%s * %s * %s" % (R(s_, 2), R(x, 1), R(y, 1))
     a, b, c, d = r
-    return "num_pi2_alldiff(%s)" % H(a, b, c, d), "0.25 * %s * %s * %s * %s" % (R(a, 1), R(b, 1), R(c, 1),
-                                                                              R(d, 1))
+    return "num_pi2_alldiff(%s)" % H(a, b, c, d), 0.25, "%s * %s * %s * %s" % (R(a, 1), R(b, 1), R(c, 1),
+                                                                            R(d, 1))
 
 
 def call_expr(fam, t, q, sym):
     V = "H" if fam == "hap" else "G"
     pre = "hap" if fam == "hap" else "geno"
-    if fam == "hap":
-        num, den = hap_call(t, q, sym)
-        return "%s * (%s)" % (num, den)
     if t == "dd":
         a, b = q
         if a == b:
@@ -171,9 +171,63 @@ def call_expr(fam, t, q, sym):
     return "%s_pi2_%s(%s)" % (pre, case, args)
 
 
+def emit_hap_set(P, names, sym, fname):
+    """Haplotype statistics: each distinct numerator polynomial is evaluated
+    once (p<k>), each distinct reciprocal product once (r<m>) and each distinct
+    (reciprocal product x constant) once (k<m>).  A statistic whose terms share
+    one reciprocal product and weight adds its (exact, integer-valued)
+    numerators before the single scaling."""
+    polys, dens, coefs, lines, body = {}, {}, {}, [], []
+
+    def poly(e):
+        if e not in polys:
+            polys[e] = "p%d" % len(polys)
+            lines.append("  const double %s = %s;" % (polys[e], e))
+        return polys[e]
+
+    def den_name(den):
+        key = " * ".join(sorted(den.split(" * ")))
+        if key not in dens:
+            if " * " not in key:
+                dens[key] = key
+            else:
+                dens[key] = "r%d" % len(dens)
+                lines.append("  const double %s = %s;" % (dens[key], key))
+        return dens[key]
+
+    def coef(den, c):
+        r = den_name(den)
+        if c == 1.0:
+            return r
+        key = (r, c)
+        if key not in coefs:
+            coefs[key] = "k%d" % len(coefs)
+            lines.append("  const double %s = %r * %s;" % (coefs[key], c, r))
+        return coefs[key]
+
+    for s, (kind, p) in enumerate(names):
+        weights = {}  # (numerator, factor, reciprocal product) -> summed weight
+        for w, t, q in terms_of(kind, p):
+            key = hap_call(t, q, sym)
+            weights[key] = weights.get(key, 0.0) + w
+        groups = {}  # (reciprocal product, weight x factor) -> numerators
+        for (num, fac, den), w in weights.items():
+            groups.setdefault((" * ".join(sorted(den.split(" * "))), w * fac), []).append(poly(num))
+        parts = []
+        for (den, c), nums in groups.items():
+            total = nums[0] if len(nums) == 1 else "(%s)" % " + ".join(nums)
+            parts.append("%s * %s" % (total, coef(den, c)))
+        body.append("  out[%d] = %s;" % (s, " + ".join(parts)))
+    head = "__device__ __forceinline__ void %s(const HapP* H, double* out) {" % fname
+    pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in range(P)]
+    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
+
+
 def emit_set(fam, P, names, sym, fname):
-    T = "HapP" if fam == "hap" else "GenoP"
-    V = "H" if fam == "hap" else "G"
+    if fam == "hap":
+        return emit_hap_set(P, names, sym, fname)
+    T = "GenoP"
+    V = "G"
     calls, lines = {}, []
     body = []
     for s, (kind, p) in enumerate(names):
@@ -194,10 +248,7 @@ def emit_set(fam, P, names, sym, fname):
             expr = " + ".join("%r * %s" % (w, v) for v, w in items)
             body.append("  out[%d] = %s;" % (s, expr))
     head = "__device__ __forceinline__ void %s(const %s* %s, double* out) {" % (fname, T, V)
-    pre = []
-    if fam == "hap":
-        pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in range(P)]
-    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
+    return "\n".join([head] + lines + body + ["}"]), len(names)
 
 
 def self_names(P):

@@ -37,8 +37,9 @@ PGLD_HD double safe_div(double num, double den) { return den > 0.0 ? num / den :
 struct HapP {
   double c1, c2, c3, c4, n, D, pA, qA, pB, qB;
   // moments of the centred alleles a' = 2a - 1, b' = 2b - 1 (in {-1, 1})
-  // over the pair-valid haplotypes: Sa = sum a', Sb = sum b', Sab = sum a' b'
-  double Sa, Sb, Sab;
+  // over the pair-valid haplotypes: Sa = sum a', Sb = sum b', Sab = sum a' b';
+  // Dp = n Sab - Sa Sb = -4 D
+  double Sa, Sb, Sab, Dp;
 };
 
 PGLD_HD HapP hap_pop(double c1, double c2, double c3, double c4) {
@@ -51,6 +52,25 @@ PGLD_HD HapP hap_pop(double c1, double c2, double c3, double c4) {
   p.Sa = p.pA - p.qA;
   p.Sb = p.pB - p.qB;
   p.Sab = (c1 + c4) - (c2 + c3);
+  p.Dp = -4.0 * p.D;
+  return p;
+}
+
+// The same values from integer counts: the sums and differences are formed in
+// int32 (exact) and converted once; D from the moments (-(n Sab - Sa Sb) / 4,
+// exact for counts below 2^26).
+PGLD_HD HapP hap_pop_int(int c1, int c2, int c3, int c4) {
+  HapP p;
+  p.c1 = c1; p.c2 = c2; p.c3 = c3; p.c4 = c4;
+  const int s12 = c1 + c2, s34 = c3 + c4, s13 = c1 + c3, s24 = c2 + c4;
+  p.n = s12 + s34;
+  p.pA = s12; p.qA = s34;
+  p.pB = s13; p.qB = s24;
+  p.Sa = s12 - s34;
+  p.Sb = s13 - s24;
+  p.Sab = (c1 + c4) - (c2 + c3);
+  p.Dp = p.n * p.Sab - p.Sa * p.Sb;
+  p.D = -0.25 * p.Dp;
   return p;
 }
 
@@ -69,39 +89,57 @@ PGLD_HD double f2(double n) { return n * (n - 1.0); }
 // reference's forms in the test suite).  Every intermediate value is an
 // integer or a dyadic fraction of one, so for counts whose products stay
 // below 2^53 they equal the reference's values exactly.
-PGLD_HD double num_dd_single(const HapP& a) {
+// The polynomials below are the numerators without their power-of-two factors
+// (num = scale * poly, the scaling exact); the fused epilogue folds the
+// factors into the per-pair reciprocal products.
+PGLD_HD double poly_dd_single(const HapP& a) {  // x 1/16
   const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
-  return 0.0625 * ((n - 1.0) * ((n - 2.0) * (a.Sab * a.Sab) - 2.0 * t) + (n - 2.0) * (x2 + y2 - n * n) + x2 * y2);
+  return (n - 1.0) * ((n - 2.0) * (a.Sab * a.Sab) - 2.0 * t) + (n - 2.0) * (x2 + y2 - n * n) + x2 * y2;
 }
-PGLD_HD double num_dd_between(const HapP& a, const HapP& b) { return a.D * b.D; }
-PGLD_HD double num_dz_iii(const HapP& a) {
+PGLD_HD double poly_dd_between(const HapP& a, const HapP& b) { return a.Dp * b.Dp; }  // x 1/16
+PGLD_HD double poly_dz_iii(const HapP& a) {                                          // x 1/4
   const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
-  return 0.25 * ((n + 1.0) * t - (n - 1.0) * (a.Sab * a.Sab) + n * n - x2 - y2 - x2 * y2);
+  return (n + 1.0) * t - (n - 1.0) * (a.Sab * a.Sab) + n * n - x2 - y2 - x2 * y2;
 }
+PGLD_HD double poly_dz_iij(const HapP& a, const HapP& b) { return (a.Sa * b.Sb) * a.Dp; }  // x 1/4
+PGLD_HD double poly_dz_iji(const HapP& a, const HapP& b) { return (b.Sa * a.Sb) * a.Dp; }  // x 1/4
+PGLD_HD double poly_dz_ijj(const HapP& a, const HapP& b) { return a.Dp * (b.Sa * b.Sb - b.Sab); }  // x 1/4
+PGLD_HD double poly_pi2_iiii(const HapP& a) {  // x 1/16
+  const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
+  return (n * n - 4.0 * n + 2.0) * (n * n - x2 - y2) + x2 * y2 - 4.0 * t + 2.0 * (a.Sab * a.Sab);
+}
+PGLD_HD double poly_pi2_triple(const HapP& t, const HapP& s) {  // x 1/16
+  return (t.n - 2.0) * (t.n * (t.n * s.n - t.Sb * s.Sb) - s.n * (t.Sa * t.Sa)) +
+         (t.Sa * s.Sb) * (t.Sa * t.Sb - 2.0 * t.Sab);
+}
+PGLD_HD double poly_pi2_iikk_sym(const HapP& a, const HapP& b) {  // x 1/32
+  const double na2 = a.n * a.n, nb2 = b.n * b.n;
+  return 2.0 * na2 * nb2 - na2 * (b.Sa * b.Sa + b.Sb * b.Sb) - nb2 * (a.Sa * a.Sa + a.Sb * a.Sb) +
+         (a.Sa * b.Sb) * (a.Sa * b.Sb) + (b.Sa * a.Sb) * (b.Sa * a.Sb);
+}
+PGLD_HD double poly_pi2_ijij(const HapP& A, const HapP& B) {  // x 1/16
+  const double P = A.Sa * B.Sa, Q = A.Sb * B.Sb;
+  return (A.n - 1.0) * (B.n - 1.0) * (A.n * B.n - P - Q) + P * Q - (A.Sa * A.Sb) * B.Sab - (B.Sa * B.Sb) * A.Sab +
+         A.Sab * B.Sab;
+}
+
+PGLD_HD double num_dd_single(const HapP& a) { return 0.0625 * poly_dd_single(a); }
+PGLD_HD double num_dd_between(const HapP& a, const HapP& b) { return 0.0625 * poly_dd_between(a, b); }
+PGLD_HD double num_dz_iii(const HapP& a) { return 0.25 * poly_dz_iii(a); }
 // Dz(i,i,k): a = pop i, b = pop k
-PGLD_HD double num_dz_iij(const HapP& a, const HapP& b) { return -(a.Sa * b.Sb) * a.D; }
+PGLD_HD double num_dz_iij(const HapP& a, const HapP& b) { return 0.25 * poly_dz_iij(a, b); }
 // Dz(i,j,i): a = pop i, b = pop j
-PGLD_HD double num_dz_iji(const HapP& a, const HapP& b) { return -(b.Sa * a.Sb) * a.D; }
+PGLD_HD double num_dz_iji(const HapP& a, const HapP& b) { return 0.25 * poly_dz_iji(a, b); }
 // Dz(i,j,j): a = pop i, b = pop j
-PGLD_HD double num_dz_ijj(const HapP& a, const HapP& b) { return -a.D * (b.Sa * b.Sb - b.Sab); }
+PGLD_HD double num_dz_ijj(const HapP& a, const HapP& b) { return 0.25 * poly_dz_ijj(a, b); }
 PGLD_HD double num_dz_diff(const HapP& a, const HapP& b, const HapP& c) {
   return -(a.D * (b.pA - b.qA) * (c.c1 - c.c2 + c.c3 - c.c4));
 }
-PGLD_HD double num_pi2_iiii(const HapP& a) {
-  const double n = a.n, x2 = a.Sa * a.Sa, y2 = a.Sb * a.Sb, t = a.Sa * a.Sb * a.Sab;
-  return 0.0625 * ((n * n - 4.0 * n + 2.0) * (n * n - x2 - y2) + x2 * y2 - 4.0 * t + 2.0 * (a.Sab * a.Sab));
-}
+PGLD_HD double num_pi2_iiii(const HapP& a) { return 0.0625 * poly_pi2_iiii(a); }
 // three samples of pop t, one of pop s (haplotype_kernels.py:268-289)
-PGLD_HD double num_pi2_triple(const HapP& t, const HapP& s) {
-  return 0.0625 * ((t.n - 2.0) * (t.n * (t.n * s.n - t.Sb * s.Sb) - s.n * (t.Sa * t.Sa)) +
-                   (t.Sa * s.Sb) * (t.Sa * t.Sb - 2.0 * t.Sab));
-}
+PGLD_HD double num_pi2_triple(const HapP& t, const HapP& s) { return 0.0625 * poly_pi2_triple(t, s); }
 PGLD_HD double num_pi2_iikk(const HapP& a, const HapP& b) { return a.pA * a.qA * (b.pB * b.qB); }
-PGLD_HD double num_pi2_iikk_sym(const HapP& a, const HapP& b) {
-  const double na2 = a.n * a.n, nb2 = b.n * b.n;
-  return 0.03125 * (2.0 * na2 * nb2 - na2 * (b.Sa * b.Sa + b.Sb * b.Sb) - nb2 * (a.Sa * a.Sa + a.Sb * a.Sb) +
-                    (a.Sa * b.Sb) * (a.Sa * b.Sb) + (b.Sa * a.Sb) * (b.Sa * a.Sb));
-}
+PGLD_HD double num_pi2_iikk_sym(const HapP& a, const HapP& b) { return 0.03125 * poly_pi2_iikk_sym(a, b); }
 // pi2(i,i,k,l), denominator 2 n_i(n_i-1) n_k n_l
 PGLD_HD double num_pi2_iikl(const HapP& a, const HapP& b, const HapP& c) {
   return a.pA * a.qA * (b.pB * c.qB + b.qB * c.pB);
@@ -113,11 +151,7 @@ PGLD_HD double num_pi2_ijkk(const HapP& a, const HapP& b, const HapP& c) {
 // pi2(i,j,i,j) (exactly symmetric in the two populations; the reference's 4
 // quartic products and 36-monomial cubic correction, haplotype_kernels.py:
 // 243-266, in centred moments)
-PGLD_HD double num_pi2_ijij(const HapP& A, const HapP& B) {
-  const double P = A.Sa * B.Sa, Q = A.Sb * B.Sb;
-  return 0.0625 * ((A.n - 1.0) * (B.n - 1.0) * (A.n * B.n - P - Q) + P * Q - (A.Sa * A.Sb) * B.Sab -
-                   (B.Sa * B.Sb) * A.Sab + A.Sab * B.Sab);
-}
+PGLD_HD double num_pi2_ijij(const HapP& A, const HapP& B) { return 0.0625 * poly_pi2_ijij(A, B); }
 // one population shared between the two locus pairs (haplotype_kernels.py:218-241)
 PGLD_HD double num_pi2_shared(const HapP& S, const HapP& X, const HapP& Y) {
   const double s1 = S.c1, s2 = S.c2, s3 = S.c3, s4 = S.c4;
