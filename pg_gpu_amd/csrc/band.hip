@@ -82,12 +82,7 @@ struct PopVal<PGLD_FAMILY_HAP> {
 template <>
 struct PopVal<PGLD_FAMILY_GENO> {
   using T = GenoP;
-  __device__ static T make(const int* c) {
-    double d[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) d[t] = (double)c[t];
-    return geno_pop(d);
-  }
+  __device__ static T make(const int* c) { return geno_pop_int(c); }
 };
 
 // ---------------------------------------------------------------------------

@@ -144,17 +144,22 @@ def hap_call(t, q, sym):
 
 
 def call_expr(fam, t, q, sym):
+    """Genotype term expression.  The terms whose division the reference
+    guards (0 for an invalid denominator: DD, Dz(i,i,i), pi2(i,i,i,i)) use the
+    falling-factorial reciprocals GR[p] (hap_recips: one reciprocal per
+    population, 0 where the factorial is 0); the unguarded ones keep their
+    division, so a zero denominator still gives inf / NaN."""
     V = "H" if fam == "hap" else "G"
     pre = "hap" if fam == "hap" else "geno"
     if t == "dd":
         a, b = q
         if a == b:
-            return "%s_dd_single(%s[%d])" % (pre, V, a)
-        return "%s_dd_between(%s[%d], %s[%d])" % (pre, V, a, V, b)
+            return "geno_dd_num(G[%d]) * GR[%d].i4" % (a, a)
+        return "4.0 * geno_D(G[%d]) * geno_D(G[%d]) * (GR[%d].i2 * GR[%d].i2)" % (a, b, a, b)
     if t == "dz":
         a, b, c = q
         if a == b == c:
-            return "%s_dz_iii(%s[%d])" % (pre, V, a)
+            return "2.0 * geno_dz_num(G[%d]) * GR[%d].i4" % (a, a)
         if a == b:
             return "%s_dz_iij(%s[%d], %s[%d])" % (pre, V, a, V, c)
         if a == c:
@@ -165,7 +170,7 @@ def call_expr(fam, t, q, sym):
     case, r = pi2_case(q)
     args = ", ".join("%s[%d]" % (V, x) for x in r)
     if case == "same":
-        return "%s_pi2_iiii(%s)" % (pre, args)
+        return "geno_pi2_num(G[%d]) * GR[%d].i4" % (r[0], r[0])
     if case in ("triple_a", "triple_b"):
         return "geno_pi2_%s(%s)" % ("iiij" if case == "triple_b" else "ijii", args)
     return "%s_pi2_%s(%s)" % (pre, case, args)
@@ -248,7 +253,9 @@ def emit_set(fam, P, names, sym, fname):
             expr = " + ".join("%r * %s" % (w, v) for v, w in items)
             body.append("  out[%d] = %s;" % (s, expr))
     head = "__device__ __forceinline__ void %s(const %s* %s, double* out) {" % (fname, T, V)
-    return "\n".join([head] + lines + body + ["}"]), len(names)
+    used = [p for p in range(P) if any("GR[%d]" % p in ln for ln in lines)]
+    pre = ["  HapR GR[%d];" % P] + ["  GR[%d] = hap_recips(G[%d].n);" % (p, p) for p in used] if used else []
+    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
 
 
 def self_names(P):

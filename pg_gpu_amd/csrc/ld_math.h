@@ -285,6 +285,46 @@ PGLD_HD GenoP geno_pop(const double* cnt_n00_to_n22) {
   return p;
 }
 
+// The same values from integer counts: the sums are formed in int32 (exact,
+// counts below 2^28) and converted once.
+PGLD_HD GenoP geno_pop_int(const int* cnt_n00_to_n22) {
+  GenoP p;
+  int g[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    g[t] = cnt_n00_to_n22[8 - t];
+    p.g[t] = g[t];
+  }
+  int Ri[3], B1[3], B2[3];
+#pragma unroll
+  for (int ia = 0; ia < 3; ++ia) {
+    Ri[ia] = g[3 * ia] + g[3 * ia + 1] + g[3 * ia + 2];
+    B1[ia] = g[3 * ia] - g[3 * ia + 2];
+    B2[ia] = g[3 * ia] + g[3 * ia + 2];
+  }
+  const int n = Ri[0] + Ri[1] + Ri[2];
+  p.n = n;
+  // twice the allele counts, four times the joint ones (then scaled exactly)
+  const int C1 = g[1] + g[4] + g[7];
+  p.pA = 0.5 * (double)(2 * Ri[0] + Ri[1]);
+  p.qA = 0.5 * (double)(Ri[1] + 2 * Ri[2]);
+  p.pB = 0.5 * (double)(2 * (g[0] + g[3] + g[6]) + C1);
+  p.qB = 0.5 * (double)(C1 + 2 * (g[2] + g[5] + g[8]));
+  p.xAB = 0.25 * (double)(4 * g[0] + 2 * g[1] + 2 * g[3] + g[4]);
+  p.xAb = 0.25 * (double)(2 * g[1] + 4 * g[2] + g[4] + 2 * g[5]);
+  p.xaB = 0.25 * (double)(2 * g[3] + g[4] + 4 * g[6] + 2 * g[7]);
+  p.xab = 0.25 * (double)(g[4] + 2 * g[5] + 2 * g[7] + 4 * g[8]);
+  p.Sb = B1[0] + B1[1] + B1[2];
+  p.Sb2 = B2[0] + B2[1] + B2[2];
+  p.Sa = Ri[0] - Ri[2];
+  p.Sab = B1[0] - B1[2];
+  p.Sab2 = B2[0] - B2[2];
+  p.Sa2 = Ri[0] + Ri[2];
+  p.Sa2b = B1[0] + B1[2];
+  p.Sa2b2 = B2[0] + B2[2];
+  return p;
+}
+
 // D from genotype frequencies (_PopDataGeno.D_geno, genotype_kernels.py:441-446):
 // xAB xab - xAb xaB = (n Sab - Sa Sb) / 4.
 PGLD_HD double geno_D(const GenoP& p) { return (p.n * p.Sab - p.Sa * p.Sb) * 0.25; }
