@@ -100,8 +100,7 @@ constexpr int band_min_waves(int family, int P, int set) {
 // (distance bin or window) into the plan's key space: 0 for a single band,
 // region * n_keys_local for the tiles of a region batch.  The pair sums are
 // folded over the wave by the halving butterfly in groups of SCAT values
-// (more values per group: fewer shuffles, more live registers; the persistent
-// batch kernel is at its register limit and reduces one value at a time).
+// (more values per group: fewer shuffles, more live registers).
 template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8>
 __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
                                                int32_t key_base, unsigned char* smem) {
@@ -391,7 +390,7 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     const int32_t key_base = rd.key_base;
     __syncthreads();  // the previous tile's LDS is no longer read
     // rd.pa stays in global memory: the population loop indexes it at run time
-    band_sums_tile<FAMILY, MISSING, P, SET, 1>(lb, rd.pa, tile, key_base, smem);
+    band_sums_tile<FAMILY, MISSING, P, SET>(lb, rd.pa, tile, key_base, smem);
   }
 }
 
