@@ -390,7 +390,8 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     const int32_t key_base = rd.key_base;
     __syncthreads();  // the previous tile's LDS is no longer read
     // rd.pa stays in global memory: the population loop indexes it at run time
-    band_sums_tile<FAMILY, MISSING, P, SET>(lb, rd.pa, tile, key_base, smem);
+    // one value per butterfly for the statistic sets that would otherwise spill
+    band_sums_tile<FAMILY, MISSING, P, SET, (SET == PGLD_SET_SELF ? 8 : 1)>(lb, rd.pa, tile, key_base, smem);
   }
 }
 

@@ -55,17 +55,19 @@ constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
 // r^2-family value of a haplotype pair from its counts (n11 = h_i.h_j, nA, nB
 // the pair-valid allele counts, n the pair-valid sample count); is_nan marks
 // an undefined r^2, sval / scnt the pair's contribution to per-SNP sums.
+// Counts are below 2^15 (at most 32767 samples per population), so the
+// integer products below are exact in 24-bit multiplies and 32-bit results.
 template <bool MISSING, int EST>
-__device__ __forceinline__ double hap_pair_value(int64_t n11, int64_t nA, int64_t nB, int64_t n,
+__device__ __forceinline__ double hap_pair_value(int32_t n11, int32_t nA, int32_t nB, int32_t n,
                                                  const int32_t* __restrict__ marg, int64_t n_var, int64_t i,
                                                  int64_t j, int32_t S, bool& is_nan, double& sval, int& scnt) {
   double r2;
   is_nan = false;
   if constexpr (EST == EST_R2) {
-    const int64_t diff = n * n11 - nA * nB;
+    const int32_t diff = __mul24(n, n11) - __mul24(nA, nB);
     const double dd = (double)diff;
     const double num = dd * dd;
-    const double den = (double)(nA * (n - nA)) * (double)(nB * (n - nB));
+    const double den = (double)__mul24(nA, n - nA) * (double)__mul24(nB, n - nB);
     is_nan = !(den > 0.0);
     r2 = den > 0.0 ? num / den : __builtin_nan("");
     sval = is_nan ? 0.0 : r2;
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           }
         }
       } else if (valid && FAMILY == PGLD_FAMILY_HAP) {
-        int64_t n11 = acc.c[0][0][r], nA, nB, n;
+        int32_t n11 = acc.c[0][0][r], nA, nB, n;
         if constexpr (MISSING) {
           nA = acc.c[0][1][r];
           nB = acc.c[1][0][r];
