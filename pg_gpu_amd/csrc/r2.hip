@@ -474,22 +474,25 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
         cur = acc[1][1];
       const int jl = wc + 32 * b + (lane & 31);
       const int64_t j = (int64_t)st.j0 + jl;
-      {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int il = wr + 32 * a + acc_row(r, lane);
-          const int64_t i = (int64_t)st.i0 + il;
-          const bool valid = i < j && j < ra.n_var && s_pos_j[jl] - s_pos_i[il] <= ra.max_dist;
-          bool hit = false;
-          float r2f = 0.0f;
-          if (valid) {
-            bool is_nan = false;
-            double sval = 0.0;
-            int scnt = 0;
-            const double r2 = hap_pair_value<false, EST>(cur[r], marg[i], marg[j], S, marg, ra.n_var, i, j, S,
-                                                         is_nan, sval, scnt);
-            r2f = (float)r2;
-            if (ra.hist) {
+      // one row of the block; FAST: every pair valid and in key slot
+      // slot_in (no per-pair validity or key work), else both are tested
+      auto row = [&](int r, auto fast_tag, int slot_in) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        const int il = wr + 32 * a + acc_row(r, lane);
+        const int64_t i = (int64_t)st.i0 + il;
+        const bool valid = FAST || (i < j && j < ra.n_var && s_pos_j[jl] - s_pos_i[il] <= ra.max_dist);
+        bool hit = false;
+        float r2f = 0.0f;
+        if (valid) {
+          bool is_nan = false;
+          double sval = 0.0;
+          int scnt = 0;
+          const double r2 = hap_pair_value<false, EST>(cur[r], marg[i], marg[j], S, marg, ra.n_var, i, j, S, is_nan,
+                                                       sval, scnt);
+          r2f = (float)r2;
+          if (ra.hist) {
+            int slot = slot_in;
+            if (!FAST) {
               int key;
               if (ra.key_mode == PGLD_KEY_WINDOW) {
                 key = s_wkey[il];
@@ -498,38 +501,62 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
                 if (ra.key_mode == PGLD_KEY_BIN_ABS) d = fabs(d);
                 key = key_bin(d, ra.edges, ra.n_keys);
               }
-              const int slot = key - tile.key_lo;
-              if (key >= 0 && slot >= 0 && slot < tile.n_slots) {
-                int hb = ra.n_hist;
-                if (!is_nan) hb = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
-                const int cell = slot * hw + hb;
-                if (cell == 0)
-                  ++cell0;
-                else
-                  atomicAdd(&wh[cell], 1);
-              }
+              slot = key >= 0 ? key - tile.key_lo : -1;
+              if (slot >= tile.n_slots) slot = -1;
             }
-            hit = !is_nan && r2 > ra.thresh;
-            if (ra.r2_out) ra.r2_out[ra.pair_base[i] + (j - i - 1)] = r2;
+            if (slot >= 0) {
+              int hb = ra.n_hist;
+              if (!is_nan) hb = max(0, min((int)(r2 * (double)ra.n_hist), ra.n_hist - 1));
+              const int cell = slot * hw + hb;
+              if (cell == 0)
+                ++cell0;
+              else
+                atomicAdd(&wh[cell], 1);
+            }
           }
-          if (ra.sp_count) {
-            const unsigned long long m = __ballot(hit);
-            if (m) {
-              const int leader = __ffsll((long long)m) - 1;
-              unsigned long long base = 0;
-              if (lane == leader) base = atomicAdd(ra.sp_count, (unsigned long long)__popcll(m));
-              base = __shfl(base, leader);
-              if (hit) {
-                const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
-                if ((int64_t)slot < ra.sp_cap) {
-                  ra.sp_i[slot] = (int32_t)i;
-                  ra.sp_j[slot] = (int32_t)j;
-                  ra.sp_r2[slot] = r2f;
-                }
+          hit = !is_nan && r2 > ra.thresh;
+          if (ra.r2_out) ra.r2_out[ra.pair_base[i] + (j - i - 1)] = r2;
+        }
+        if (ra.sp_count) {
+          const unsigned long long m = __ballot(hit);
+          if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(ra.sp_count, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (hit) {
+              const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1ull));
+              if ((int64_t)slot < ra.sp_cap) {
+                ra.sp_i[slot] = (int32_t)i;
+                ra.sp_j[slot] = (int32_t)j;
+                ra.sp_r2[slot] = r2f;
               }
             }
           }
         }
+      };
+      // the block's pairs are all valid when its first column lies right of
+      // its last row, its last column exists and its farthest pair is within
+      // max_dist; window keys are monotone in the row, so equal keys at the
+      // first and last row mean one key for the block (histograms with
+      // distance keys take the per-pair path)
+      const int ri0 = wr + 32 * a, cj0 = wc + 32 * b;
+      int fast_slot = -1;
+      if ((int64_t)st.j0 + cj0 > (int64_t)st.i0 + ri0 + 31 && (int64_t)st.j0 + cj0 + 31 < ra.n_var &&
+          s_pos_j[cj0 + 31] - s_pos_i[ri0] <= ra.max_dist) {
+        if (!ra.hist) {
+          fast_slot = 0;
+        } else if (ra.key_mode == PGLD_KEY_WINDOW) {
+          const int k0 = s_wkey[ri0], k1 = s_wkey[ri0 + 31];
+          if (k0 >= 0 && k0 == k1 && k0 - tile.key_lo < tile.n_slots) fast_slot = k0 - tile.key_lo;
+        }
+      }
+      if (fast_slot >= 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) row(r, std::true_type{}, fast_slot);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) row(r, std::false_type{}, -1);
       }
     }
   }
