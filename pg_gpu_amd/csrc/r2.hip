@@ -348,6 +348,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
 constexpr int R2W_ROWS = 2 * BT;
 constexpr int R2W_HIST_PER_WAVE = R2_MAX_HIST_LDS / 4;
 constexpr int R2W_OP_BYTES = 2 * R2W_ROWS * OP_ROW;
+constexpr int R2W_RESIDENT = 64;  // workgroups resident per XCD (32 CUs x 2)
 
 template <int EST>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, const SuperTile* __restrict__ sts,
@@ -407,8 +408,20 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
   v4i pre0[ITEMS], pre1[ITEMS];
-  auto load = [&](v4i* pre, int64_t k) {  // past the end: the last chunk again (unused)
-    const int64_t kk = k < k_pad ? k : k_pad - KSTEP;
+  // K chunks are visited in a rotated order (integer sums do not depend on
+  // it): the rotation grows with the workgroup's position in its XCD's run,
+  // by one full turn per R2W_RESIDENT workgroups.  Workgroups start about
+  // evenly spaced in time once the XCD is full, so those resident together
+  // work on about the same chunk at the same time and share the operand
+  // strips of their neighbourhood in L2 instead of streaming them at random
+  // offsets.  Loads past the last step read a valid (unused) chunk.
+  const int nch = (int)(k_pad / KSTEP);
+  const int rot = (int)(((int64_t)(blockIdx.x / N_XCD) * nch / R2W_RESIDENT) % nch);
+  auto load = [&](v4i* pre, int64_t k) {
+    int c = (int)(k / KSTEP) + rot;
+    if (c >= nch) c -= nch;
+    if (c >= nch) c -= nch;
+    const int64_t kk = (int64_t)c * KSTEP;
 #pragma unroll
     for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + kk);
   };
