@@ -345,7 +345,8 @@ template <int FAMILY, bool MISSING, int P, int SET>
 __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
                                                                                             PopArgs pa) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const BandTile tile = ba.tiles[ba.order ? ba.order[t] : t];
   if (tile.n_slots <= 0) return;
   band_sums_tile<FAMILY, MISSING, P, SET>(ba, pa, tile, 0, smem);
 }
@@ -866,6 +867,29 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
     rc = up(&plan->d_super, sup.empty() ? nullptr : sup.data(), sup.size() * sizeof(SuperTile));
   }
   if (!rc && plan->n_regions == 0) {
+    // Launch order of the fused sums: tiles by 8 x 8 blocks (block-row major,
+    // row-major inside a block).  Each XCD walks a contiguous run of it, so the
+    // ~64 tiles it holds at once read 8 row and 8 column strips of operands
+    // from its L2 instead of one row strip and ~20 column strips (a row-major
+    // walk along a band, whose strips overflow the L2 once several populations
+    // or planes are read).  Slots, keys and the reduction order belong to the
+    // tiles, so the order changes no result.
+    constexpr int64_t SB = 8;
+    const int64_t n_bc = ceil_div(std::max<int64_t>(n_var, 1), BT * SB) + 1;
+    std::vector<std::pair<int64_t, int32_t>> keyed(tb.tiles.size());
+    for (size_t t = 0; t < tb.tiles.size(); ++t) {
+      const int64_t ti = tb.tiles[t].i0 / BT, tj = tb.tiles[t].j0 / BT;
+      keyed[t] = {(((ti / SB) * n_bc + tj / SB) * SB + ti % SB) * SB + tj % SB, (int32_t)t};
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    std::vector<int32_t> order(keyed.size());
+    for (size_t k = 0; k < keyed.size(); ++k) order[k] = keyed[k].second;
+    rc = up((void**)&plan->d_sums_order, order.empty() ? nullptr : order.data(), order.size() * sizeof(int32_t));
+  }
+  if (!rc && plan->n_regions == 0) {
     // tile lists by tile-row (contiguous in the schedule) and tile-column
     const int64_t n_tr = ceil_div(std::max<int64_t>(n_var, 1), BT);
     std::vector<int64_t> row_off(n_tr + 1, 0), col_off(n_tr + 1, 0);
@@ -1112,7 +1136,7 @@ int pgld_band_plan_destroy(pgld_band_plan* plan) {
   void* ptrs[] = {plan->d_pos,      plan->d_kc,       plan->d_edges,     plan->d_tiles,    plan->d_pair_base,
                   plan->d_key_off,  plan->d_key_list, plan->d_partials,  plan->d_regions,  plan->d_order,
                   plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt,
-                  plan->d_super,    plan->d_red_lo,   plan->d_red_off};
+                  plan->d_super,    plan->d_red_lo,   plan->d_red_off,  plan->d_sums_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)
@@ -1240,6 +1264,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
   ba.key_mode = pl->key_mode;
   ba.n_keys = pl->n_keys_local;
   ba.partials = pl->d_partials;
+  ba.order = pl->n_regions == 0 ? pl->d_sums_order : nullptr;
   if (pl->n_tiles > 0) {
     if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "band sums event");
     L(pl, ba, pa, lds, s);

@@ -53,6 +53,7 @@ struct pgld_band_plan {
   int64_t* d_row_off = nullptr;        // tile-row r -> tiles [row_off[r], row_off[r + 1])
   int64_t* d_col_off = nullptr;        // tile-column c -> col_list[col_off[c] .. col_off[c + 1])
   int32_t* d_col_list = nullptr;
+  int32_t* d_sums_order = nullptr;     // single plans: launch order of the fused sums tiles (8 x 8 blocks)
   void* d_super = nullptr;             // SuperTile[n_super] (single plans): 2 x 2 groups of tiles
   int64_t n_super = 0;
   double* d_snp_part = nullptr;        // workspace: per tile 64 row + 64 column sums
@@ -409,6 +410,7 @@ struct BandArgs {
   double* partials;
   int32_t* counts_out;       // counts mode
   const int64_t* pair_base;  // counts mode
+  const int32_t* order;      // fused sums of a single band: tile launch order (null: list order)
 };
 
 }  // namespace pgld
