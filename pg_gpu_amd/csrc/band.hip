@@ -52,18 +52,22 @@ struct StatSet;
   template <> struct StatSet<PGLD_FAMILY_HAP, P, PGLD_SET_MOMENTS> {                               \
     static constexpr int N = EPI_HAP_MOMENTS_##P##_N;                                              \
     __device__ static void eval(const HapP* v, double* out) { epi_hap_moments_##P(v, out); }       \
+    __device__ static double scale(int c) { return epi_hap_moments_##P##_scale(c); }               \
   };                                                                                               \
   template <> struct StatSet<PGLD_FAMILY_HAP, P, PGLD_SET_SELF> {                                  \
     static constexpr int N = EPI_HAP_SELF_##P##_N;                                                 \
     __device__ static void eval(const HapP* v, double* out) { epi_hap_self_##P(v, out); }          \
+    __device__ static double scale(int c) { return epi_hap_self_##P##_scale(c); }                  \
   };                                                                                               \
   template <> struct StatSet<PGLD_FAMILY_GENO, P, PGLD_SET_MOMENTS> {                              \
     static constexpr int N = EPI_GENO_MOMENTS_##P##_N;                                             \
     __device__ static void eval(const GenoP* v, double* out) { epi_geno_moments_##P(v, out); }     \
+    __device__ static double scale(int c) { return epi_geno_moments_##P##_scale(c); }              \
   };                                                                                               \
   template <> struct StatSet<PGLD_FAMILY_GENO, P, PGLD_SET_SELF> {                                 \
     static constexpr int N = EPI_GENO_SELF_##P##_N;                                                \
     __device__ static void eval(const GenoP* v, double* out) { epi_geno_self_##P(v, out); }        \
+    __device__ static double scale(int c) { return epi_geno_self_##P##_scale(c); }                 \
   };
 PGLD_STATSETS(1)
 PGLD_STATSETS(2)
@@ -334,11 +338,21 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       });
   }
   __syncthreads();
-  // fixed-order sum over the four wave rows into this tile's partial row
+  // fixed-order sum over the four wave rows into this tile's partial row,
+  // times the statistic's power-of-two factor (left out of the per-pair values)
   const int row_n = tile.n_slots * NS;
   double* dst = ba.partials + tile.slot_base * NS;
-  for (int t = tid; t < row_n; t += BAND_THREADS)
-    dst[t] = ((red[t] + red[row_n + t]) + red[2 * row_n + t]) + red[3 * row_n + t];
+  for (int t = tid; t < row_n; t += BAND_THREADS) {
+    const int c = t % NS;
+    double f = 1.0;
+    if (c < NS - 1) {
+      if constexpr (SET == PGLD_SET_SELF && P > 1)
+        f = StatSet<FAMILY, 1, PGLD_SET_SELF>::scale(c / P);  // column s * P + p
+      else
+        f = SS::scale(c);
+    }
+    dst[t] = (((red[t] + red[row_n + t]) + red[2 * row_n + t]) + red[3 * row_n + t]) * f;
+  }
 }
 
 template <int FAMILY, bool MISSING, int P, int SET>
@@ -428,32 +442,52 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopAr
 }
 
 // Per-key reduction over tile partial rows, in ascending tile order.
+// Fixed-order sums of the partial rows listed in key_list[lo, hi) (the rows
+// lo .. hi - 1 themselves when key_list is null), every column of a row at
+// once: thread t takes column t % cols of the slot group t / cols (groups
+// stride over the rows, so the cols threads of a group read one row
+// contiguously), and the group totals are added in group order.  out(s, v)
+// receives column s.  The order depends only on the row list and ns_total.
+template <class F>
+__device__ __forceinline__ void fixed_order_rows(const double* __restrict__ partials,
+                                                 const int64_t* __restrict__ key_list, int64_t lo, int64_t hi,
+                                                 int32_t ns_total, double* buf, F&& out) {
+  const int t = threadIdx.x;
+  for (int c0 = 0; c0 < ns_total; c0 += 256) {
+    const int cols = min(ns_total - c0, 256), groups = 256 / cols;
+    const int g = t / cols, s = t - g * cols;
+    double x = 0.0;
+    if (g < groups) {
+#pragma unroll 4
+      for (int64_t e = lo + g; e < hi; e += groups) x += partials[(key_list ? key_list[e] : e) * ns_total + c0 + s];
+    }
+    __syncthreads();  // the previous column block has read buf
+    if (g < groups) buf[g * cols + s] = x;
+    __syncthreads();
+    if (t < cols) {
+      double v = 0.0;
+      for (int q = 0; q < groups; ++q) v += buf[q * cols + t];
+      out(c0 + t, v);
+    }
+  }
+}
+
 // First level of the two-level key reduction: chunk c adds its slots (entries
-// [lohi[2c], lohi[2c + 1]) of the key list) into part2[c][s], in the same fixed
-// order as k_band_reduce.
+// [lohi[2c], lohi[2c + 1]) of the key list) into part2[c][s].
 __global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ partials,
                                                        const int64_t* __restrict__ key_list,
                                                        const int64_t* __restrict__ lohi, int32_t ns_total,
                                                        double* __restrict__ part2) {
   __shared__ double buf[256];
   const int64_t c = blockIdx.x;
-  const int64_t lo = lohi[2 * c], hi = lohi[2 * c + 1];
-  for (int s = 0; s < ns_total; ++s) {
-    double x = 0.0;
-    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) x += partials[key_list[e] * ns_total + s];
-    buf[threadIdx.x] = x;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part2[c * ns_total + s] = buf[0];
-    __syncthreads();
-  }
+  fixed_order_rows(partials, key_list, lohi[2 * c], lohi[2 * c + 1], ns_total, buf,
+                   [&](int s, double v) { part2[c * ns_total + s] = v; });
 }
 
 // Per-key sums over the key's slots (key_list entries key_off[k] ..
 // key_off[k + 1]; with key_list null the entries are the slots themselves).
+// A key of one first-level chunk sums to the same value either way: its
+// chunk total plus the zero totals of the empty groups.
 __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ partials,
                                                      const int32_t* __restrict__ key_off,
                                                      const int64_t* __restrict__ key_list, int32_t ns_total,
@@ -461,22 +495,10 @@ __global__ __launch_bounds__(256) void k_band_reduce(const double* __restrict__ 
                                                      double* __restrict__ key_pairs) {
   __shared__ double buf[256];
   const int k = blockIdx.x;
-  const int lo = key_off[k], hi = key_off[k + 1];
-  for (int s = 0; s < ns_total; ++s) {
-    double x = 0.0;
-    for (int e = lo + threadIdx.x; e < hi; e += 256) x += partials[(key_list ? key_list[e] : e) * ns_total + s];
-    buf[threadIdx.x] = x;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      if (s < n_stats) sums[(int64_t)k * n_stats + s] = buf[0];
-      else if (key_pairs) key_pairs[k] = buf[0];
-    }
-    __syncthreads();
-  }
+  fixed_order_rows(partials, key_list, key_off[k], key_off[k + 1], ns_total, buf, [&](int s, double v) {
+    if (s < n_stats) sums[(int64_t)k * n_stats + s] = v;
+    else if (key_pairs) key_pairs[k] = v;
+  });
 }
 
 // ---------------------------------------------------------------------------

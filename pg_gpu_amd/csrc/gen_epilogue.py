@@ -182,7 +182,7 @@ def emit_hap_set(P, names, sym, fname):
     (reciprocal product x constant) once (k<m>).  A statistic whose terms share
     one reciprocal product and weight adds its (exact, integer-valued)
     numerators before the single scaling."""
-    polys, dens, coefs, lines, body = {}, {}, {}, [], []
+    polys, dens, coefs, lines, body, scales = {}, {}, {}, [], [], []
 
     def poly(e):
         if e not in polys:
@@ -218,14 +218,36 @@ def emit_hap_set(P, names, sym, fname):
         groups = {}  # (reciprocal product, weight x factor) -> numerators
         for (num, fac, den), w in weights.items():
             groups.setdefault((" * ".join(sorted(den.split(" * "))), w * fac), []).append(poly(num))
+        # the smallest power-of-two factor of the statistic is left out of the
+        # per-pair value and applied to the tile's partial sums (scales[s]);
+        # scaling by a power of two commutes with FP64 rounding and addition,
+        # so the sums are the same to the bit
+        m = min(c for (_, c) in groups)
+        scales.append(m)
         parts = []
         for (den, c), nums in groups.items():
             total = nums[0] if len(nums) == 1 else "(%s)" % " + ".join(nums)
-            parts.append("%s * %s" % (total, coef(den, c)))
+            parts.append("%s * %s" % (total, coef(den, c / m)))
         body.append("  out[%d] = %s;" % (s, " + ".join(parts)))
     head = "__device__ __forceinline__ void %s(const HapP* H, double* out) {" % fname
     pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in range(P)]
-    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
+    return "\n".join([head] + pre + lines + body + ["}", scale_fn(fname, scales)]), len(names)
+
+
+def scale_fn(fname, scales):
+    """Device function: the factor applied to partial-sum column s of the set."""
+    out = ["__device__ __forceinline__ double %s_scale(int s) {" % fname]
+    if all(c == 1.0 for c in scales):
+        out.append("  return 1.0;")
+    else:
+        out.append("  switch (s) {")
+        for k, c in enumerate(scales):
+            if c != 1.0:
+                out.append("    case %d: return %r;" % (k, c))
+        out.append("    default: return 1.0;")
+        out.append("  }")
+    out.append("}")
+    return "\n".join(out)
 
 
 def emit_set(fam, P, names, sym, fname):
@@ -255,7 +277,7 @@ def emit_set(fam, P, names, sym, fname):
     head = "__device__ __forceinline__ void %s(const %s* %s, double* out) {" % (fname, T, V)
     used = [p for p in range(P) if any("GR[%d]" % p in ln for ln in lines)]
     pre = ["  HapR GR[%d];" % P] + ["  GR[%d] = hap_recips(G[%d].n);" % (p, p) for p in used] if used else []
-    return "\n".join([head] + pre + lines + body + ["}"]), len(names)
+    return "\n".join([head] + pre + lines + body + ["}", scale_fn(fname, [1.0] * len(names))]), len(names)
 
 
 def self_names(P):
