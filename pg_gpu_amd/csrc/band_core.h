@@ -73,7 +73,7 @@ constexpr int BT = 64;       // tile edge (SNPs)
 constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
 constexpr int BAND_THREADS = 256;
 constexpr int RED_LDS_BYTES = 64 * 1024;
-constexpr int RED_CHUNK = 1024;     // slots per first-level chunk of a long key reduction
+constexpr int RED_CHUNK = 256;      // slots per first-level chunk of a long key reduction
 constexpr int MAX_LDS_EDGES = 130;  // bin edges kept in LDS when n_keys < 130
 // static LDS of k_band_sums: positions, key coordinates, window keys, edges
 constexpr int STATIC_LDS_BYTES = 4 * 64 * 8 + 64 * 4 + MAX_LDS_EDGES * 8;
