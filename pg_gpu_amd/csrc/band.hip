@@ -548,14 +548,14 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   __shared__ int64_t srow[PK_S];
   const int tid = threadIdx.x;
   static_assert(PK_V == 128 && PK_S == 128, "index math below assumes 128 x 128 tiles");
-  // Block (x, y) of the launch grid packs tile t = xcd_tile(linear id): each
-  // XCD takes a contiguous run of variant blocks (all sample blocks of one
-  // before the next), so the 128-byte lines that straddle two variant blocks
-  // (rows of n_var bytes are rarely line-aligned) are fetched into one L2,
-  // and a variant's plane rows are written by one XCD.
-  const int64_t ny = gridDim.y;
-  const int64_t t = xcd_tile((int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x, (int64_t)gridDim.x * ny);
-  const int64_t v0 = (t / ny) * PK_V, k0 = (t % ny) * PK_S;
+  // Block (x, y) of the launch grid packs tile t = xcd_tile(linear id),
+  // variant blocks fastest: each XCD takes a contiguous run of the tiles, so
+  // neighbouring variant blocks of the same sample rows run on one XCD — the
+  // 128-byte lines that straddle two of them (rows of n_var bytes are rarely
+  // line-aligned) are fetched into one L2, and reads stay in the same DRAM rows.
+  const int64_t nx = gridDim.x;
+  const int64_t t = xcd_tile((int64_t)blockIdx.x + (int64_t)blockIdx.y * nx, nx * gridDim.y);
+  const int64_t v0 = (t % nx) * PK_V, k0 = (t / nx) * PK_S;
   if (tid < PK_S) srow[tid] = k0 + tid < n_sel ? (int64_t)(rows ? rows[k0 + tid] : k0 + tid) * n_var : -1;
   __syncthreads();
   // phase 1: 32 threads per row (one dword = 4 variants each), 8 rows per
