@@ -36,11 +36,11 @@ sys.path.insert(0, ROOT)
 PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
-# FETCH_SIZE (2,201,660 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
+# FETCH_SIZE (769,840 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
 # by the same counters' reading of a 1 GiB device copy (--copy-calibration:
 # FETCH x 2.000, WRITE x 1.000).  Summary and commands:
 # profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
-TRAFFIC_BYTES_PER_LAUNCH = 4.513e9
+TRAFFIC_BYTES_PER_LAUNCH = 1.581e9
 
 
 def parse():
