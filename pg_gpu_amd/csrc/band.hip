@@ -39,15 +39,6 @@ namespace pgld {
 
 template <int FAMILY, int P, int SET>
 struct StatSet;
-#define PGLD_STATSET(FAM, FAMC, P, SET, SETC, NAME)                                     \
-  template <>                                                                           \
-  struct StatSet<FAMC, P, SETC> {                                                       \
-    static constexpr int N = EPI_##FAM##_##NAME##_##P##_N;                              \
-    template <class T>                                                                  \
-    __device__ static void eval(const T* v, double* out) {                              \
-      epi_##SET##_##P(v, out);                                                          \
-    }                                                                                   \
-  };
 #define PGLD_STATSETS(P)                                                                           \
   template <> struct StatSet<PGLD_FAMILY_HAP, P, PGLD_SET_MOMENTS> {                               \
     static constexpr int N = EPI_HAP_MOMENTS_##P##_N;                                              \
@@ -74,7 +65,6 @@ PGLD_STATSETS(2)
 PGLD_STATSETS(3)
 PGLD_STATSETS(4)
 #undef PGLD_STATSETS
-#undef PGLD_STATSET
 
 template <int FAMILY>
 struct PopVal;
