@@ -26,6 +26,9 @@ Pinning (how each part is checked independently of this transcription):
   Omega by brute force over partitions.
 * locate_unlinked -- known answers and the reference's property tests
   (tests/test_ld_r_squared.py:124-146).
+* per-window ZnS / Omega -- the pinned sigma_D^2 matrix, ZnS and Omega
+  applied to each window's columns; the window ranges and the NaN / 0.0
+  cases by known answers (bin edges, empty and short windows).
 * tiled_window_sums_hap -- the per-pair window sums of this module.
 
 Functions with no independent pin are marked "parity unpinned" in their
@@ -1100,22 +1103,25 @@ def zns_from_matrix(M, valid=None):
 
 def omega_from_matrix(M):
     """Kim and Nielsen's Omega over the upper triangle (ld_statistics.py:
-    761-862): max over split points l = 3 .. m-2 of mean within / mean across,
-    by explicit block sums (no prefix sums)."""
+    761-862): the largest, over split points l = 3 .. m-2, of mean within /
+    mean across where the split is valid (cross sum > 0) and 0.0 where it is
+    not -- the maximum of those values, so a negative Omega (sigma_D^2 sums
+    can be negative) when every split is valid and negative; by explicit
+    block sums (no prefix sums)."""
     M = np.triu(np.asarray(M, dtype=np.float64), k=1)
     m = M.shape[0]
     if m < 5:
         return 0.0
-    best = 0.0
+    vals = []
     for l in range(3, m - 1):
         left = M[:l, :l].sum()
         right = M[l:, l:].sum()
         cross = M[:l, l:].sum()
         n_w = l * (l - 1) // 2 + (m - l) * (m - l - 1) // 2
         n_c = l * (m - l)
-        if n_w > 0 and n_c > 0 and cross > 0:
-            best = max(best, ((left + right) / n_w) / (cross / n_c))
-    return float(best)
+        ok = n_w > 0 and n_c > 0 and cross > 0
+        vals.append(((left + right) / n_w) / (cross / n_c) if ok else 0.0)
+    return float(max(vals))
 
 
 def dosage_r2_matrix(geno):
@@ -1157,3 +1163,48 @@ def locate_unlinked(hap, size=100, step=20, threshold=0.1):
                 if loc[active[j]] and r2[i, j] > threshold:
                     loc[active[j]] = False
     return loc
+
+
+# ---------------------------------------------------------------------------
+# Per-window ZnS and Omega  (windowed_analysis.py:1658-1667, :2013-2058;
+# ld_statistics.py:595-676)
+# ---------------------------------------------------------------------------
+
+
+def window_ranges(positions, starts_bp, stops_bp):
+    """Variant index range [start, stop) of each window: searchsorted 'left'
+    of the window's start and end in bp (windowed_analysis.py:1658-1667,
+    :1740-1749), i.e. the variants with start <= pos < end."""
+    pos = np.asarray(positions, dtype=np.float64)
+    return (np.searchsorted(pos, np.asarray(starts_bp, dtype=np.float64), side="left"),
+            np.searchsorted(pos, np.asarray(stops_bp, dtype=np.float64), side="left"))
+
+
+def windowed_zns_omega(hap, positions, starts_bp, stops_bp, missing_data="include"):
+    """The 'zns' and 'omega' columns of windowed_statistics_fused
+    (windowed_analysis.py:2013-2058): NaN for windows of fewer than 4
+    variants; otherwise ZnS = _zns_from_precomputed(..., use_projection=True)
+    over the window's columns (segregating filter, mean sigma_D^2 over the
+    valid pairs, 0.0 below 2 sites or without valid pairs), and Omega =
+    ld_statistics.omega(window matrix, missing_data) with its default
+    sigma_D^2 estimator (0.0 below 5 sites)."""
+    h = np.asarray(hap)
+    s, e = window_ranges(positions, starts_bp, stops_bp)
+    zns = np.full(len(s), np.nan)
+    om = np.full(len(s), np.nan)
+    for w in range(len(s)):
+        if e[w] - s[w] < 4:
+            continue
+        win = h[:, s[w]:e[w]]
+        seg = segregating_sites(win)
+        if len(seg) < 2:
+            zns[w] = 0.0
+        else:
+            sig, valid = sigma_d2_matrix(win[:, seg])
+            zns[w] = zns_from_matrix(sig, valid)
+        wo = win
+        if missing_data == "exclude":
+            wo = win[:, np.nonzero((win >= 0).all(axis=0))[0]]
+        seg = segregating_sites(wo)
+        om[w] = omega_from_matrix(sigma_d2_matrix(wo[:, seg])[0]) if len(seg) >= 5 else 0.0
+    return zns, om

@@ -295,6 +295,49 @@ def test_zns_omega_pairwise_r2(missing):
         assert LS.zns(ex, missing_data="exclude") == pytest.approx(O.zns_from_matrix(sig, valid), rel=1e-10)
 
 
+
+def _close_or_nan(got, want, rtol):
+    got, want = np.asarray(got), np.asarray(want)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_allclose(got[ok], want[ok], rtol=rtol, atol=1e-12)
+
+
+@pytest.mark.parametrize("missing", [0.0, 0.03])
+def test_windowed_zns_omega(missing):
+    """Per-window ZnS / Omega of windowed_statistics_fused against the
+    oracle's window loop: bins (short and empty windows included), a named
+    population, overlapping explicit windows, missing_data='exclude'."""
+    from pg_gpu_amd import windowed_analysis as WA
+    rng = np.random.default_rng(64)
+    h = _ld_panel(64, n_hap=80, n_var=400, missing=missing)
+    pos = np.sort(rng.choice(np.arange(1, 200_000), size=h.shape[1], replace=False))
+    pos[50:53] = pos[50]  # duplicate positions stay in one window
+    h[:, 60:64] = 0       # a window stretch without segregating sites
+    bins = np.array([0, 3_000, 20_000, 20_500, 60_000, 61_000, 120_000, 150_000, 200_000, 250_000])
+    hm = HaplotypeMatrix(h, pos, sample_sets={"a": list(range(0, 50)), "b": list(range(50, 80))})
+    got = WA.windowed_statistics_fused(hm, bins, statistics=("zns", "omega"))
+    want = O.windowed_zns_omega(h, pos, bins[:-1], bins[1:])
+    assert list(got)[:6] == ["chrom", "start", "end", "center", "n_variants", "window_id"]
+    np.testing.assert_array_equal(got["n_variants"], np.diff(np.searchsorted(pos, bins)))
+    _close_or_nan(got["zns"], want[0], 1e-10)
+    _close_or_nan(got["omega"], want[1], 1e-10)
+    sub = h[hm.sample_sets["b"]]
+    got = WA.windowed_statistics_fused(hm, bins, statistics=("zns",), population="b")
+    _close_or_nan(got["zns"], O.windowed_zns_omega(sub, pos, bins[:-1], bins[1:])[0], 1e-10)
+    assert "omega" not in got
+    starts = np.arange(0, 180_000, 10_000)
+    stops = starts + 25_000  # windows overlap by 15 kb: three layers
+    got = WA.windowed_statistics_fused(hm, None, statistics=("omega", "zns"), _win_starts=starts,
+                                       _win_stops=stops, missing_data="exclude")
+    want = O.windowed_zns_omega(h, pos, starts, stops, missing_data="exclude")
+    _close_or_nan(got["zns"], want[0], 1e-10)
+    _close_or_nan(got["omega"], want[1], 1e-10)
+    with pytest.raises(NotImplementedError):
+        WA.windowed_statistics_fused(hm, bins, statistics=("pi", "zns"))
+    with pytest.raises(ValueError):
+        WA.windowed_statistics_fused(hm, bins, statistics=("zns",), population="c")
+
 def test_windowed_r_squared():
     h = _ld_panel(63, n_hap=80, n_var=300, missing=0.02)
     rng = np.random.default_rng(4)

@@ -188,3 +188,20 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=300, cwd=ROOT)
     assert r.returncode == 2 and "one GPU only" in (r.stdout + r.stderr)
+
+
+def test_window_layers_are_disjoint_and_cover():
+    """Overlapping windows are dealt into layers of disjoint index ranges,
+    each window exactly once, every layer in increasing order."""
+    from pg_gpu_amd.windowed_analysis import _layers
+    rng = np.random.default_rng(3)
+    start = np.sort(rng.integers(0, 500, size=60))
+    stop = start + rng.integers(1, 80, size=60)
+    ws = list(range(60))
+    layers = _layers(start, stop, ws)
+    assert sorted(w for layer in layers for w in layer) == ws
+    for layer in layers:
+        for a, b in zip(layer, layer[1:]):
+            assert stop[a] <= start[b]
+    assert _layers(start, stop, []) == []
+

@@ -452,3 +452,49 @@ def test_locate_unlinked_known_answers():
     assert loc.shape == (50,) and loc.dtype == bool and 0 < loc.sum() <= 50
     hap = rng.randint(0, 2, (20, 20)).astype(np.int8)
     assert O.locate_unlinked(hap, size=10, step=5, threshold=1.0).all()
+
+
+def test_window_ranges_and_short_windows():
+    """Known answers for the per-window LD columns (windowed_analysis.py:
+    1658-1667, :2013-2058): a variant at a bin edge belongs to the window on
+    its right, variants at or past the last edge to none; windows of fewer
+    than 4 variants are NaN; windows of 4+ variants with fewer than 2
+    segregating sites give ZnS 0.0, fewer than 5 give Omega 0.0."""
+    pos = np.array([10, 20, 30, 40, 50, 60, 70, 80, 90, 100])
+    s, e = O.window_ranges(pos, [10, 30, 100], [30, 100, 200])
+    np.testing.assert_array_equal(s, [0, 2, 9])
+    np.testing.assert_array_equal(e, [2, 9, 10])
+    rng = np.random.default_rng(7)
+    hap = rng.integers(0, 2, size=(12, 10)).astype(np.int8)
+    hap[:, 2:6] = 0                      # window [30, 70): variants 2..5, of which
+    hap[0, 2] = 1                        # only variant 2 segregates
+    zns, om = O.windowed_zns_omega(hap, pos, [0, 30, 100, 0], [30, 70, 200, 1])
+    assert np.isnan(zns[0]) and np.isnan(om[0])      # 2 variants
+    assert np.isnan(zns[2]) and np.isnan(zns[3])     # 1 and 0 variants
+    assert zns[1] == 0.0 and om[1] == 0.0           # 4 variants, 1 segregating
+    zns, om = O.windowed_zns_omega(hap, pos, [30], [200])
+    seg = O.segregating_sites(hap[:, 2:10])
+    assert len(seg) == 5
+    sig, valid = O.sigma_d2_matrix(hap[:, 2:10][:, seg])
+    assert zns[0] == O.zns_from_matrix(sig, valid) and om[0] == O.omega_from_matrix(sig)
+
+
+def test_omega_known_answers():
+    """Omega's split rule (ld_statistics.py:848-862): the maximum over
+    l = 3 .. m-2 of the within / across mean ratio where the cross sum is
+    positive, 0.0 for other splits -- negative when every split has a
+    positive cross sum and a negative within sum."""
+    m = 5
+    M = np.zeros((m, m))
+    iu = np.triu_indices(m, k=1)
+    M[iu] = -1.0
+    M[:3, 3:] = 2.0          # cross block of the only split (l = 3): mean 2
+    # within: 3 pairs left + 1 pair right, all -1 -> mean -1; omega = -1 / 2
+    assert O.omega_from_matrix(M) == -0.5
+    M[:3, 3:] = -2.0         # no valid split -> 0.0
+    assert O.omega_from_matrix(M) == 0.0
+    M = np.zeros((6, 6))
+    M[np.triu_indices(6, k=1)] = 1.0
+    assert O.omega_from_matrix(M) == 1.0  # uniform LD: every ratio 1
+    assert O.omega_from_matrix(np.ones((4, 4))) == 0.0  # fewer than 5 sites
+
