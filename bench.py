@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
+PEAK_FP4_OPS = 10.1e15   # MI355X dense FP4 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
 # FETCH_SIZE (769,840 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
@@ -56,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--planes", default="int8", choices=["int8", "fp4"],
+                    help="operand planes of the fused sums (windowed and genotype workloads): int8 MFMA, or FP4 "
+                         "(e2m1 0 / 1) on the FP4 MFMA")
     ap.add_argument("--method", default="mfma", choices=["mfma", "popcount"],
                     help="tile products of the dense r^2 workload: int8 MFMA or AND + popcount on bit planes")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -132,11 +136,18 @@ def base_line(args, value, ms_step, workload, config, dtype="int8"):
     }
 
 
-def mfma_roofline(ops, kernel_ms, kernel, macs, traffic=None):
+def mfma_roofline(ops, kernel_ms, kernel, macs, traffic=None, fp4=False):
+    """MFMA roofline of a kernel: algorithmic ops over its time against the
+    dense peak of the MFMA it runs on (int8, or FP4 for FP4 planes; the
+    int8-peak fraction is kept beside the FP4 one)."""
     achieved = ops / (kernel_ms * 1e-3)
-    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_INT8_OPS, "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_ms,
-            "ops": f"int8 MFMA ops (1 MAC = 2 ops) of {macs} MACs per pair*sample"}
+    peak = PEAK_FP4_OPS if fp4 else PEAK_INT8_OPS
+    out = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+           "frac": achieved / peak, "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_ms,
+           "ops": f"{'FP4' if fp4 else 'int8'} MFMA ops (1 MAC = 2 ops) of {macs} MACs per pair*sample"}
+    if fp4:
+        out["frac_of_int8_peak"] = achieved / PEAK_INT8_OPS
+    return out
 
 
 def _plain_compare(torch, plain_step, fused_out, steps):
@@ -175,12 +186,13 @@ def run_windowed_hap(args, torch):
 
     # the input is validated once (domain, missing data); a timed step then
     # repacks both populations into the same buffers and runs the fused pass
-    packed = [band.pack_population(hap_d, r, "hap") for r in rows_d]
+    fp4 = args.planes == "fp4"
+    packed = [band.pack_population(hap_d, r, "hap", fp4=fp4) for r in rows_d]
     with_missing = any(p.has_missing for p in packed)
 
     def step():
         for k, r in enumerate(rows_d):
-            band.pack_population(hap_d, r, "hap", validate=False, out=packed[k])
+            band.pack_population(hap_d, r, "hap", validate=False, out=packed[k], fp4=fp4)
         return plan.sums(packed, with_missing=with_missing)
 
     ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
@@ -188,7 +200,7 @@ def run_windowed_hap(args, torch):
     log(f"[bench] fused step {ms_step:.3f} ms")
     band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
     roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs,
-                         TRAFFIC_BYTES_PER_LAUNCH if (n_var, n_hap) == (100_000, 2000) else None)
+                         TRAFFIC_BYTES_PER_LAUNCH if (n_var, n_hap) == (100_000, 2000) and not fp4 else None, fp4)
     log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
 
     # plain per-pair path on identical inputs and outputs (speedup vs plain)
@@ -246,7 +258,8 @@ def run_windowed_hap(args, torch):
                      f"batched windowed haplotype LD: {n_var:,} SNPs x {n_hap:,} haplotypes (2 pops x "
                      f"{n_hap // 2:,}), 2% MCAR missing, gamma(0.5) heterogeneous positions over 10 Mb, pairs "
                      "within 100 kb, 15 two-pop stats summed per 100 kb window (FP64)",
-                     {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples})
+                     {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
+                      "planes": args.planes}, dtype="fp4" if fp4 else "int8")
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "fused_vs_plain_max_rel": check, "cpu_baseline": cpu,
@@ -271,19 +284,20 @@ def run_genotype_bins(args, torch):
     plan = band.BandPlan(pos, max_dist, edges=edges)
     pair_samples = plan.n_pairs * n_ind
     log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
-    packed = [band.pack_population(g_d, None, "geno")]
+    fp4 = args.planes == "fp4"
+    packed = [band.pack_population(g_d, None, "geno", fp4=fp4)]
     with_missing = bool(packed[0].has_missing)
 
     def step():
-        band.pack_population(g_d, None, "geno", validate=False, out=packed[0])
+        band.pack_population(g_d, None, "geno", validate=False, out=packed[0], fp4=fp4)
         return plan.sums(packed, with_missing=with_missing)
 
     ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
     value = pair_samples / (ms_step * 1e-3) / 1e9
     log(f"[bench] fused step {ms_step:.3f} ms")
     band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
-    roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs)
-    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
+    roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs, fp4=fp4)
+    log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} TOP/s")
 
     plain_ms = check = None
     if not args.no_plain:
@@ -326,7 +340,8 @@ def run_genotype_bins(args, torch):
                      f"genotype LD: {n_var:,} SNPs x {n_ind:,} diploids, 5% missing, HWE Beta(0.5,0.5) dosages, "
                      "all pairs in a 1 Mb span, 16 log-spaced distance bins (1 bp - 1 Mb), single-population "
                      "DD/Dz/pi2 FP64 bin sums (no biallelic filter)",
-                     {"n_var": n_var, "n_ind": n_ind, "pairs": plan.n_pairs, "pair_samples": pair_samples})
+                     {"n_var": n_var, "n_ind": n_ind, "pairs": plan.n_pairs, "pair_samples": pair_samples,
+                      "planes": args.planes}, dtype="fp4" if fp4 else "int8")
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "fused_vs_plain_max_rel": check, "cpu_baseline": cpu})

@@ -144,6 +144,14 @@ int pgld_keyed_sums(const double* values, const int32_t* keys, int64_t n_pairs, 
 int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var,
                      const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
                      int32_t* flags, void* stream);
+/* The same planes in FP4 (no counterpart in the reference: an operand layout
+ * for the gfx950 FP4 MFMA): each 0/1 indicator as an e2m1 code (0x0 or
+ * 0x2 = 1.0), two samples per byte, the even sample in the low nibble; k_pad
+ * must be a multiple of 256 and planes has n_planes * n_var * k_pad / 2
+ * bytes.  marg and flags as above.  Consumed by pgld_band_sums_fp4. */
+int pgld_pack_planes_fp4(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var,
+                         const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
+                         int32_t* flags, void* stream);
 
 typedef struct pgld_band_plan pgld_band_plan;
 
@@ -184,6 +192,15 @@ int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
  * HaplotypeMatrix.compute_ld_statistics_gpu_* (haplotype_matrix.py:
  * 1592-1866). */
 int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
+                   const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                   const int32_t* n_samples, int32_t with_missing, int32_t stat_set, double* sums,
+                   double* key_pairs, void* stream);
+/* pgld_band_sums over FP4 planes (pgld_pack_planes_fp4; no counterpart in the
+ * reference): the plane products run on v_mfma_f32_32x32x64_f8f6f4, 64
+ * samples per instruction, with float accumulators that hold the counts
+ * exactly; the sums equal pgld_band_sums' bit for bit.  Single bands only;
+ * k_pad of every population a multiple of 256. */
+int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
                    const int32_t* n_samples, int32_t with_missing, int32_t stat_set, double* sums,
                    double* key_pairs, void* stream);

@@ -40,6 +40,7 @@ _SIGNATURES = {
     "pgld_pair_keys": ([_vp, _vp, _vp, _i64, _i32, _vp, _i32, _f64, _f64, _vp, _vp], C.c_int),
     "pgld_keyed_sums": ([_vp, _vp, _i64, _i32, _i32, _vp, _vp], C.c_int),
     "pgld_pack_planes": ([_i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp], C.c_int),
+    "pgld_pack_planes_fp4": ([_i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp], C.c_int),
     "pgld_band_plan_create": ([_vp, _i64, _f64, _i32, _vp, _vp, _i32, _f64, _f64, C.POINTER(_vp)], C.c_int),
     "pgld_band_plan_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], C.c_int),
     "pgld_band_plan_destroy": ([_vp], C.c_int),
@@ -65,6 +66,8 @@ _SIGNATURES = {
     "pgld_band_plan_kernel_ms": ([_vp, C.POINTER(C.c_float)], C.c_int),
     "pgld_band_sums": ([_vp, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32), _i32, _i32, _vp, _vp, _vp],
                        C.c_int),
+    "pgld_band_sums_fp4": ([_vp, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32), _i32, _i32, _vp, _vp,
+                            _vp], C.c_int),
     "pgld_band_counts": ([_vp, _i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp], C.c_int),
 }
 

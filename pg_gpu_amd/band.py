@@ -18,16 +18,18 @@ import numpy as np
 from . import _lib as L
 
 
-def k_pad_for(n_samples):
-    """K padding of a population: a multiple of 128 bytes, at least 128."""
-    return max(128, ((int(n_samples) + 127) // 128) * 128)
+def k_pad_for(n_samples, fp4=False):
+    """K padding of a population: a multiple of 128 samples (256 for FP4
+    planes, whose rows hold two samples per byte), at least one multiple."""
+    q = 256 if fp4 else 128
+    return max(q, ((int(n_samples) + q - 1) // q) * q)
 
 
 @dataclass
 class PackedPop:
     """Indicator planes of one population (SNP-major, K padded)."""
     family: int
-    planes: object       # int8 tensor (n_planes, n_var, k_pad)
+    planes: object       # int8 tensor (n_planes, n_var, k_pad), or (n_planes, n_var, k_pad // 2) for FP4
     marg: object         # int32 tensor (n_planes, n_var)
     k_pad: int
     n_samples: int
@@ -35,10 +37,13 @@ class PackedPop:
     flags: object = None  # int32 tensor [any missing, any out of domain]
     bits: object = None   # uint32-as-int32 tensor of bit planes (popcount variant), made on demand
     bits_fresh: bool = False
+    fp4: bool = False     # planes are FP4 (pgld_pack_planes_fp4): fused band sums only
 
     def bit_planes(self):
         """Bit planes of this population (32 samples per word), rebuilt from
         the int8 planes when they were repacked since."""
+        if self.fp4:
+            raise ValueError("bit planes are made from int8 planes, not FP4 ones")
         n_planes, n_var, kp = self.planes.shape
         if self.bits is None:
             self.bits = L.empty((n_planes, n_var, 32 * ((kp + 1023) // 1024)), np.int32)
@@ -49,8 +54,12 @@ class PackedPop:
         return self.bits
 
 
-def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, out=None):
+def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, out=None, fp4=False):
     """Pack one population's rows of a (rows, n_var) int8 matrix.
+
+    fp4=True packs the indicators as FP4 (two samples per byte) for the
+    FP4-MFMA fused band sums (BandPlan.sums); the other band passes take the
+    int8 planes.
 
     validate=True reads the kernel's value flags back (one small sync) to
     reject out-of-domain values and report missing data; validate=False skips
@@ -71,24 +80,25 @@ def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, ou
                 raise IndexError("population row index out of range")
             rows_t = L.to_device(r, np.int32)
             n_sel = int(r.size)
-    kp = k_pad_for(n_sel) if k_pad is None else int(k_pad)
+    kp = k_pad_for(n_sel, fp4) if k_pad is None else int(k_pad)
     npl = 2 if fam == L.FAMILY_HAP else 3
-    if out is not None and tuple(out.planes.shape) == (npl, n_var, kp):
+    row = kp // 2 if fp4 else kp
+    if out is not None and tuple(out.planes.shape) == (npl, n_var, row) and out.fp4 == fp4:
         planes, marg, flags = out.planes, out.marg, out.flags
     else:
-        planes = L.empty((npl, n_var, kp), np.int8)
+        planes = L.empty((npl, n_var, row), np.int8)
         marg = L.empty((npl, n_var), np.int32)
         flags = L.empty((2,), np.int32)
-    L.call("pgld_pack_planes", fam, L.ptr(g), n_rows, n_var, L.ptr(rows_t), n_sel, kp, L.ptr(planes),
-           L.ptr(marg), L.ptr(flags), L.stream_handle())
-    bits = out.bits if (out is not None and out.planes is planes) else None
+    L.call("pgld_pack_planes_fp4" if fp4 else "pgld_pack_planes", fam, L.ptr(g), n_rows, n_var, L.ptr(rows_t),
+           n_sel, kp, L.ptr(planes), L.ptr(marg), L.ptr(flags), L.stream_handle())
+    bits = out.bits if (out is not None and out.planes is planes and not fp4) else None
     if not validate:
-        return PackedPop(fam, planes, marg, kp, n_sel, None, flags, bits)
+        return PackedPop(fam, planes, marg, kp, n_sel, None, flags, bits, fp4=fp4)
     f = flags.cpu().numpy()
     if f[1]:
         raise ValueError("input values outside {-1, 0, 1}" if fam == L.FAMILY_HAP
                          else "genotype values outside {-1, 0, 1, 2}")
-    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]), flags, bits)
+    return PackedPop(fam, planes, marg, kp, n_sel, bool(f[0]), flags, bits, fp4=fp4)
 
 
 class BandPlan:
@@ -159,6 +169,9 @@ class BandPlan:
         fam = pops[0].family
         if any(p.family != fam for p in pops):
             raise ValueError("all populations must be of one family")
+        fp4 = pops[0].fp4
+        if any(p.fp4 != fp4 for p in pops):
+            raise ValueError("all populations must be packed in one layout (int8 or FP4)")
         if with_missing is None:
             if any(p.has_missing is None for p in pops):
                 raise ValueError("with_missing must be given for populations packed with validate=False")
@@ -174,8 +187,8 @@ class BandPlan:
         margs, _k2 = L.ptr_array([p.marg for p in pops])
         kpad = (C.c_int64 * P)(*[p.k_pad for p in pops])
         ns = (C.c_int32 * P)(*[p.n_samples for p in pops])
-        L.call("pgld_band_sums", self._h, fam, P, planes, margs, kpad, ns, int(bool(with_missing)), sset,
-               L.ptr(sums), L.ptr(kp), L.stream_handle())
+        L.call("pgld_band_sums_fp4" if fp4 else "pgld_band_sums", self._h, fam, P, planes, margs, kpad, ns,
+               int(bool(with_missing)), sset, L.ptr(sums), L.ptr(kp), L.stream_handle())
         return (sums, kp) if want_pairs else sums
 
     def r2(self, pop, n_hist=100, thresh=0.2, sparse_cap=1 << 20, per_pair=False, histogram=True,
@@ -196,6 +209,8 @@ class BandPlan:
           r2        FP64 per band pair in (i, j) order if per_pair,
           row_sums, col_sums, row_counts, col_counts  per-SNP sums over the
                     pairs (v, j > v) and (i < v, v) if snp_sums."""
+        if pop.fp4:
+            raise ValueError("the r^2 passes take int8 planes (pack without fp4)")
         geno_est = estimator in ("rogers_huff_r", "dosage_r2")
         if (pop.family == L.FAMILY_GENO) != geno_est:
             raise ValueError("estimators 'r2', 'naive', 'sigma_d2' take haplotype planes; 'rogers_huff_r' and "
@@ -251,6 +266,8 @@ class BandPlan:
         """Per-pair counts of one population through the tiled path (int8
         MFMA, or AND + popcount on bit planes), in the band's (i, j) pair
         order: int32 (n_pairs, 4) or (n_pairs, 9)."""
+        if pop.fp4:
+            raise ValueError("the counts pass takes int8 planes (pack without fp4)")
         if with_missing is None:
             with_missing = pop.has_missing
         if method not in ("mfma", "popcount"):

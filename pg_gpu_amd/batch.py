@@ -71,6 +71,8 @@ class RegionBatch:
             if len(pl) != P or any(p.family != fam for p in pl):
                 raise ValueError(f"region {r}: every region needs {P} populations of one family")
             for p in pl:
+                if p.fp4:
+                    raise ValueError(f"region {r}: region batches take int8 planes (pack without fp4)")
                 if p.planes.shape[1] != self.n_var[r]:
                     raise ValueError(f"region {r}: planes have {p.planes.shape[1]} SNPs, plan has {self.n_var[r]}")
         flat = [p for pl in pops for p in pl]

@@ -95,7 +95,7 @@ constexpr int band_min_waves(int family, int P, int set) {
 // region * n_keys_local for the tiles of a region batch.  The pair sums are
 // folded over the wave by the halving butterfly in groups of SCAT values
 // (more values per group: fewer shuffles, more live registers).
-template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8>
+template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8, bool NIB = false>
 __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
                                                int32_t key_base, unsigned char* smem) {
   using PL = Planes<FAMILY, MISSING>;
@@ -271,7 +271,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     flush(ck >= 0);
   };
 
-  WaveAcc<PL::N> acc;
+  WaveAcc<PL::N, NIB> acc;
   if constexpr (SET == PGLD_SET_SELF && P > 1) {
     // Independent per-population statistics (column s * P + p): each
     // population's products are followed at once by its own epilogue, so no
@@ -280,8 +280,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live,
-                                      opbuf, acc);
+      block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
+                                                  wave_live, opbuf, acc);
       // the first population's pass records the pair keys, the others reuse them
       const int kmode = tile.n_slots > 255 ? 0 : (p == 0 ? 1 : 2);
       if (wave_live)
@@ -296,8 +296,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      block_products<FAMILY, MISSING>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc, wave_live,
-                                      opbuf, acc);
+      block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
+                                                  wave_live, opbuf, acc);
       if (p == P - 1 || !wave_live) continue;
       // all populations but the last park their counts in LDS as int16
       const int64_t jj = rj + (lane & 31);
@@ -345,14 +345,14 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   }
 }
 
-template <int FAMILY, bool MISSING, int P, int SET>
+template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
 __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
                                                                                             PopArgs pa) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
   const BandTile tile = ba.tiles[ba.order ? ba.order[t] : t];
   if (tile.n_slots <= 0) return;
-  band_sums_tile<FAMILY, MISSING, P, SET>(ba, pa, tile, 0, smem);
+  band_sums_tile<FAMILY, MISSING, P, SET, 8, NIB>(ba, pa, tile, 0, smem);
 }
 
 // Region batch, persistent: workgroup b takes tiles order[b], order[b + grid],
@@ -528,7 +528,9 @@ __device__ __forceinline__ void set_flag_once(int32_t* flags, int f) {
   if (__hip_atomic_load(&flags[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) atomicOr(&flags[f], 1);
 }
 
-template <int FAMILY, bool WORDS>
+// NIB: FP4 planes (two samples per byte as e2m1 0 / 1.0 codes, rows of
+// k_pad / 2 bytes) instead of int8 0/1 bytes.
+template <int FAMILY, bool WORDS, bool NIB = false>
 __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
                                               const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
                                               int8_t* __restrict__ planes, int32_t* __restrict__ marg,
@@ -636,12 +638,26 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) {
-      int4 val4;
-      val4.x = (int)pw[p][0];
-      val4.y = (int)pw[p][1];
-      val4.z = (int)pw[p][2];
-      val4.w = (int)pw[p][3];
-      *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + 16 * sg) = val4;
+      if constexpr (NIB) {
+        // four 0/1 bytes of a dword -> four nibbles (byte b -> bits 4b), times the e2m1 code of 1.0
+        uint32_t h[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          uint32_t t = (pw[p][i] | (pw[p][i] >> 4)) & 0x00ff00ffu;
+          h[i] = ((t | (t >> 8)) & 0xffffu) * (uint32_t)FP4_ONE;
+        }
+        uint2 val2;
+        val2.x = h[0] | (h[1] << 16);
+        val2.y = h[2] | (h[3] << 16);
+        *reinterpret_cast<uint2*>(planes + ((int64_t)p * n_var + v0 + v) * (k_pad / 2) + (k0 + 16 * sg) / 2) = val2;
+      } else {
+        int4 val4;
+        val4.x = (int)pw[p][0];
+        val4.y = (int)pw[p][1];
+        val4.z = (int)pw[p][2];
+        val4.w = (int)pw[p][3];
+        *reinterpret_cast<int4*>(planes + ((int64_t)p * n_var + v0 + v) * k_pad + k0 + 16 * sg) = val4;
+      }
     }
   }
   // sum the byte-packed counts over the 8 lanes of the quad (<= 128 per byte)
@@ -932,10 +948,10 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   return rc;
 }
 
-template <int FAMILY, bool MISSING, int P, int SET>
+template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
 static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
                         hipStream_t s) {
-  if (plan->n_regions > 0) {
+  if (!NIB && plan->n_regions > 0) {  // FP4 planes are taken by single bands only (checked by the caller)
     const int m = plan->order_mode;
     const int64_t len = plan->order_len[m];
     const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
@@ -943,31 +959,31 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const Po
                        ba, reinterpret_cast<const RegionDesc*>(plan->d_regions), plan->d_order + plan->order_off[m],
                        len, (int32_t)plan->xcd_split);
   } else {
-    hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)plan->n_tiles), dim3(BAND_THREADS),
-                       lds, s, ba, pa);
+    hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB>), dim3((unsigned)plan->n_tiles),
+                       dim3(BAND_THREADS), lds, s, ba, pa);
   }
 }
 
 typedef void (*SumsLauncher)(const pgld_band_plan*, const BandArgs&, const PopArgs&, size_t, hipStream_t);
 
-template <int FAMILY, bool MISSING>
+template <int FAMILY, bool MISSING, bool NIB>
 static SumsLauncher pick_launcher(int P, int set) {
   if constexpr (FAMILY == PGLD_FAMILY_HAP) {
     if (set == PGLD_SET_MOMENTS) {
-      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
-      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS>;
-      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_MOMENTS>;
+      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS, NIB>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_MOMENTS, NIB>;
     } else {
-      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
-      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF>;
-      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_SELF>;
-      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_SELF>;
+      if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF, NIB>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_SELF, NIB>;
+      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_SELF, NIB>;
     }
   } else {
-    if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS>;
+    if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
     if (P == 2) {
-      if (set == PGLD_SET_MOMENTS) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS>;
-      return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF>;
+      if (set == PGLD_SET_MOMENTS) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS, NIB>;
+      return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF, NIB>;
     }
   }
   return nullptr;
@@ -1034,12 +1050,14 @@ static std::vector<int32_t> blocked_order(const std::vector<BandTileHost>& h, co
 
 extern "C" {
 
-int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
-                     int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream) {
+static int pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                       int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream,
+                       bool nib) {
   PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_pack_planes: bad family");
   if (!rows) n_sel = n_rows;
   PGLD_REQUIRE(n_var >= 0 && n_sel >= 0, "pgld_pack_planes: negative size");
-  PGLD_REQUIRE(k_pad % 128 == 0 && k_pad >= n_sel, "pgld_pack_planes: k_pad must be a multiple of 128 >= n_sel");
+  PGLD_REQUIRE(k_pad % (nib ? 256 : 128) == 0 && k_pad >= n_sel,
+               "pgld_pack_planes: k_pad must be a multiple of %d >= n_sel", nib ? 256 : 128);
   PGLD_REQUIRE(n_sel <= 32767, "pgld_pack_planes: at most 32767 samples per population");
   PGLD_REQUIRE(data && planes && marg && flags, "pgld_pack_planes: null pointer");
   const int npl = family == PGLD_FAMILY_HAP ? 2 : 3;
@@ -1052,15 +1070,28 @@ int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t
   if (k_pad == 0) return PGLD_OK;
   dim3 g((unsigned)ceil_div(n_var, PK_V), (unsigned)(k_pad / PK_S));
   const bool words = (n_var % 4 == 0) && (reinterpret_cast<uintptr_t>(data) % 4 == 0);
-#define PGLD_PACK(F, W) \
-  hipLaunchKernelGGL((k_pack<F, W>), g, dim3(256), 0, s, data, n_var, rows, n_sel, k_pad, planes, marg, flags)
+#define PGLD_PACK(F, W, N) \
+  hipLaunchKernelGGL((k_pack<F, W, N>), g, dim3(256), 0, s, data, n_var, rows, n_sel, k_pad, planes, marg, flags)
+#define PGLD_PACK_W(F, N) \
+  if (words) PGLD_PACK(F, true, N); else PGLD_PACK(F, false, N)
   if (family == PGLD_FAMILY_HAP) {
-    if (words) PGLD_PACK(PGLD_FAMILY_HAP, true); else PGLD_PACK(PGLD_FAMILY_HAP, false);
+    if (nib) { PGLD_PACK_W(PGLD_FAMILY_HAP, true); } else { PGLD_PACK_W(PGLD_FAMILY_HAP, false); }
   } else {
-    if (words) PGLD_PACK(PGLD_FAMILY_GENO, true); else PGLD_PACK(PGLD_FAMILY_GENO, false);
+    if (nib) { PGLD_PACK_W(PGLD_FAMILY_GENO, true); } else { PGLD_PACK_W(PGLD_FAMILY_GENO, false); }
   }
+#undef PGLD_PACK_W
 #undef PGLD_PACK
   return check_launch("k_pack");
+}
+
+int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                     int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream) {
+  return pack_planes(family, data, n_rows, n_var, rows, n_sel, k_pad, planes, marg, flags, stream, false);
+}
+
+int pgld_pack_planes_fp4(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
+                         int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream) {
+  return pack_planes(family, data, n_rows, n_var, rows, n_sel, k_pad, planes, marg, flags, stream, true);
 }
 
 int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int32_t key_mode,
@@ -1202,12 +1233,19 @@ static int fill_pop_args(int32_t n_pops, const int8_t* const* planes, const int3
 
 namespace pgld {
 
-static SumsLauncher launcher_for(int32_t family, int32_t n_pops, bool miss, int32_t stat_set) {
+static SumsLauncher launcher_for(int32_t family, int32_t n_pops, bool miss, int32_t stat_set, bool fp4 = false) {
+  if (fp4) {
+    if (family == PGLD_FAMILY_HAP)
+      return miss ? pick_launcher<PGLD_FAMILY_HAP, true, true>(n_pops, stat_set)
+                  : pick_launcher<PGLD_FAMILY_HAP, false, true>(n_pops, stat_set);
+    return miss ? pick_launcher<PGLD_FAMILY_GENO, true, true>(n_pops, stat_set)
+                : pick_launcher<PGLD_FAMILY_GENO, false, true>(n_pops, stat_set);
+  }
   if (family == PGLD_FAMILY_HAP)
-    return miss ? pick_launcher<PGLD_FAMILY_HAP, true>(n_pops, stat_set)
-                : pick_launcher<PGLD_FAMILY_HAP, false>(n_pops, stat_set);
-  return miss ? pick_launcher<PGLD_FAMILY_GENO, true>(n_pops, stat_set)
-              : pick_launcher<PGLD_FAMILY_GENO, false>(n_pops, stat_set);
+    return miss ? pick_launcher<PGLD_FAMILY_HAP, true, false>(n_pops, stat_set)
+                : pick_launcher<PGLD_FAMILY_HAP, false, false>(n_pops, stat_set);
+  return miss ? pick_launcher<PGLD_FAMILY_GENO, true, false>(n_pops, stat_set)
+              : pick_launcher<PGLD_FAMILY_GENO, false, false>(n_pops, stat_set);
 }
 
 // LDS bytes of one fused workgroup, or 0 when the tile's keys x statistics do
@@ -1245,9 +1283,10 @@ static int ensure_partials(pgld_band_plan* pl, int64_t ns_total) {
 // allow_alloc false (region batches, which may be captured into a graph) the
 // workspace must already be large enough.
 static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const PopArgs& pa, int32_t with_missing,
-                    int32_t stat_set, double* sums, double* key_pairs, hipStream_t s, bool allow_alloc) {
+                    int32_t stat_set, double* sums, double* key_pairs, hipStream_t s, bool allow_alloc,
+                    bool fp4 = false) {
   const bool miss = with_missing != 0;
-  SumsLauncher L = launcher_for(family, n_pops, miss, stat_set);
+  SumsLauncher L = launcher_for(family, n_pops, miss, stat_set, fp4);
   if (!L) {
     set_error("band sums: no fused kernel for family %d with %d populations and stat set %d", family, n_pops,
               stat_set);
@@ -1323,6 +1362,23 @@ int pgld_band_sums(const pgld_band_plan* plan, int32_t family, int32_t n_pops, c
   if (rc) return rc;
   return run_sums(const_cast<pgld_band_plan*>(plan), family, n_pops, pa, with_missing, stat_set, sums, key_pairs,
                   as_stream(stream), true);
+}
+
+int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pops, const int8_t* const* planes,
+                       const int32_t* const* marg, const int64_t* k_pad, const int32_t* n_samples,
+                       int32_t with_missing, int32_t stat_set, double* sums, double* key_pairs, void* stream) {
+  PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples && sums, "pgld_band_sums_fp4: null pointer");
+  PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_sums_fp4: region batches take int8 planes");
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_sums_fp4: bad family");
+  PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_band_sums_fp4: n_pops must be 1..4");
+  PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_band_sums_fp4: bad stat_set");
+  for (int p = 0; p < n_pops; ++p)
+    PGLD_REQUIRE(k_pad[p] % 256 == 0, "pgld_band_sums_fp4: k_pad must be a multiple of 256 (FP4 planes)");
+  PopArgs pa;
+  int rc = fill_pop_args(n_pops, planes, marg, k_pad, n_samples, pa);
+  if (rc) return rc;
+  return run_sums(const_cast<pgld_band_plan*>(plan), family, n_pops, pa, with_missing, stat_set, sums, key_pairs,
+                  as_stream(stream), true, true);
 }
 
 int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -67,6 +68,8 @@ namespace pgld {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 constexpr int BT = 64;       // tile edge (SNPs)
@@ -122,10 +125,23 @@ struct Planes<PGLD_FAMILY_GENO, false> {  // e1, e2 only
   static constexpr int N = 2, FIRST = 1, NPL = 3, NC = 9;
 };
 
-template <int N>
+// Accumulators of the N x N plane products: int32 for the int8 MFMA, float
+// for the FP4 MFMA (exact: counts stay far below 2^24).
+template <int N, bool F = false>
 struct WaveAcc {
-  v16i c[N][N];
+  using T = typename std::conditional<F, v16f, v16i>::type;
+  T c[N][N];
 };
+
+// FP4 planes: the 0/1 indicators as e2m1 codes (0 -> 0x0, 1 -> 0x2 = 1.0),
+// two samples per byte, the even sample in the low nibble; a row holds
+// k_pad / 2 bytes, k_pad a multiple of 256.  v_mfma_f32_32x32x64_f8f6f4 with
+// e2m1 operands (cbsz = blgp = 4) and unit scales (E8M0 127) takes the 16
+// bytes a lane reads for one int8 MFMA as 32 samples of row (lane mod 32),
+// K half (lane div 32), nibble k low first -- the same lane layout on both
+// operands, so one FP4 MFMA covers 64 samples where the int8 one covers 32.
+constexpr int FP4_ONE = 0x2;
+constexpr int FP4_UNIT_SCALE = 0x7f;
 
 // LDS-staged products for the whole workgroup (all four waves must call it:
 // it contains barriers).  Per 128-byte K chunk the 64 A rows and 64 B rows of
@@ -146,12 +162,14 @@ __host__ __device__ inline int64_t bit_row_bytes(int64_t k_pad) { return 128 * (
 // as AND + popcount on the VALU, accumulated into the same accumulator
 // layout as the MFMA (lane l -> column l % 32, rows acc_row(r, l)), so every
 // epilogue works with either.
-template <int FAMILY, bool MISSING, bool POPC = false>
+template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false>
 __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
                                                int i0, int j0, int wr, int wc, bool wave_live,
                                                unsigned char* __restrict__ opbuf,
-                                               WaveAcc<Planes<FAMILY, MISSING>::N>& acc) {
+                                               WaveAcc<Planes<FAMILY, MISSING>::N, NIB>& acc) {
+  static_assert(!(POPC && NIB), "bit planes and FP4 planes are separate layouts");
   if constexpr (POPC) k_pad = bit_row_bytes(k_pad);  // from here on: bytes per operand row
+  if constexpr (NIB) k_pad /= 2;
   using PL = Planes<FAMILY, MISSING>;
   constexpr int N = PL::N;
   constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
@@ -248,8 +266,16 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
 #pragma unroll
         for (int a = 0; a < N; ++a)
 #pragma unroll
-          for (int b = 0; b < N; ++b)
-            acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
+          for (int b = 0; b < N; ++b) {
+            if constexpr (NIB) {
+              const v4i x = A[st & 1][a], y = B[st & 1][b];
+              const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
+              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc.c[a][b], 4, 4, 0,
+                                                                            FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
+            } else {
+              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
+            }
+          }
       }
     }
   };
@@ -268,18 +294,18 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
 
 // Joint counts of accumulator element r (row i, col j) in the reference's
 // column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
-template <int FAMILY, bool MISSING>
-__device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING>::N>& acc, int r,
+template <int FAMILY, bool MISSING, class ACC>
+__device__ __forceinline__ void elem_counts(const ACC& acc, int r,
                                             const int32_t* __restrict__ marg_, int64_t n_var, int64_t i, int64_t j,
                                             int32_t S, int* out) {
   const GLOBAL_AS int32_t* marg = (const GLOBAL_AS int32_t*)marg_;
   if constexpr (FAMILY == PGLD_FAMILY_HAP) {
     int hh, hv, vh, vv;
-    hh = acc.c[0][0][r];
+    hh = (int)acc.c[0][0][r];  // float accumulators (FP4) hold exact integers
     if constexpr (MISSING) {
-      hv = acc.c[0][1][r];
-      vh = acc.c[1][0][r];
-      vv = acc.c[1][1][r];
+      hv = (int)acc.c[0][1][r];
+      vh = (int)acc.c[1][0][r];
+      vv = (int)acc.c[1][1][r];
     } else {
       hv = marg[i];  // plane 0 (h) sums
       vh = marg[j];
@@ -295,12 +321,12 @@ __device__ __forceinline__ void elem_counts(const WaveAcc<Planes<FAMILY, MISSING
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) N[3 * a + b] = acc.c[a][b][r];
+        for (int b = 0; b < 3; ++b) N[3 * a + b] = (int)acc.c[a][b][r];
     } else {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) N[3 * (a + 1) + (b + 1)] = acc.c[a][b][r];
+        for (int b = 0; b < 2; ++b) N[3 * (a + 1) + (b + 1)] = (int)acc.c[a][b][r];
       N[0] = S;
       N[1] = marg[n_var + j];      // e1 sums
       N[2] = marg[2 * n_var + j];  // e2 sums

@@ -401,3 +401,64 @@ def test_pack_planes_domain_error(family, bad):
         band.pack_population(d, None, family)
     # the same value in a row outside the selected population is not an error
     band.pack_population(d, [r for r in range(70) if r != 13], family)
+
+
+# ---------------------------------------------------------------------------
+# FP4 planes and the FP4-MFMA fused sums
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("family,n_var,n_rows,missing", [("hap", 301, 150, 0.05), ("hap", 256, 600, 0.0),
+                                                          ("geno", 259, 131, 0.05), ("geno", 130, 300, 0.0)])
+def test_pack_planes_fp4_definition(family, n_var, n_rows, missing):
+    """Nibble n of an FP4 row byte b holds sample 2b + n as 0x0 / 0x2, rows of
+    k_pad / 2 bytes with k_pad a multiple of 256; marginals as the int8 planes'."""
+    if family == "hap":
+        d, _ = hap_data(103, n_rows, n_var, missing)
+    else:
+        d, _ = geno_data(104, n_rows, n_var, missing)
+    rows = np.sort(np.random.default_rng(5).choice(n_rows, size=n_rows - 3, replace=False))
+    p8 = band.pack_population(d, rows, family)
+    p4 = band.pack_population(d, rows, family, fp4=True)
+    kp = band.k_pad_for(len(rows), fp4=True)
+    assert p4.fp4 and p4.k_pad == kp and kp % 256 == 0
+    b = np_(p4.planes).view(np.uint8)
+    assert b.shape == (np_(p8.planes).shape[0], n_var, kp // 2)
+    nib = np.stack([b & 0xF, b >> 4], axis=-1).reshape(b.shape[0], n_var, kp)
+    assert set(np.unique(nib)) <= {0, 2}
+    want = np.zeros_like(nib)
+    want[:, :, :p8.k_pad] = np_(p8.planes) * 2
+    np.testing.assert_array_equal(nib, want)
+    np.testing.assert_array_equal(np_(p4.marg), np_(p8.marg))
+    assert p4.has_missing == p8.has_missing
+
+
+@pytest.mark.parametrize("family,P,missing,stat_set,key", [
+    ("hap", 1, True, "moments", "window"), ("hap", 2, True, "moments", "window"),
+    ("hap", 2, False, "moments", "bins"), ("hap", 3, True, "moments", "bins"),
+    ("hap", 4, True, "self", "window"), ("hap", 2, True, "self", "bins"),
+    ("geno", 1, True, "moments", "bins"), ("geno", 1, False, "moments", "bins"),
+    ("geno", 2, True, "moments", "window"), ("geno", 2, True, "self", "bins")])
+def test_band_sums_fp4_equal_int8(family, P, missing, stat_set, key):
+    """The FP4-MFMA fused pass forms the same exact counts as the int8 one,
+    so every sum agrees bit for bit (population sizes that are not multiples
+    of 256 pad differently in the two layouts)."""
+    if family == "hap":
+        d, pos = hap_data(110 + P, 70 * P + 13, 1200, 0.02 if missing else 0.0)
+    else:
+        d, pos = geno_data(120 + P, 90 * P + 7, 900, 0.04 if missing else 0.0)
+    n = d.shape[0]
+    pops = [list(range(p, n, P)) for p in range(P)]
+    if key == "window":
+        plan = band.BandPlan(pos, 6000.0, window=15000.0)
+    else:
+        edges = np.logspace(1, 4.5, 11)
+        plan = band.BandPlan(pos, float(edges[-1]), edges=edges)
+    p8 = [band.pack_population(d, r, family) for r in pops]
+    p4 = [band.pack_population(d, r, family, fp4=True) for r in pops]
+    s8, k8 = plan.sums(p8, stat_set=stat_set, want_pairs=True)
+    s4, k4 = plan.sums(p4, stat_set=stat_set, want_pairs=True)
+    np.testing.assert_array_equal(np_(s4), np_(s8))
+    np.testing.assert_array_equal(np_(k4), np_(k8))
+    with pytest.raises(ValueError):
+        plan.sums([p8[0]] + p4[1:] if P > 1 else [p4[0], p8[0]], stat_set=stat_set)
+    with pytest.raises(ValueError):
+        plan.counts(p4[0])
