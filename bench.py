@@ -37,11 +37,11 @@ PEAK_INT8_OPS = 5.0e15   # MI355X dense int8 MFMA, ops/s
 PEAK_FP4_OPS = 10.1e15   # MI355X dense FP4 MFMA, ops/s
 PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
-# FETCH_SIZE (769,840 KiB) and WRITE_SIZE (3,908 KiB) per dispatch, scaled
-# by the same counters' reading of a 1 GiB device copy (--copy-calibration:
-# FETCH x 2.000, WRITE x 1.000).  Summary and commands:
-# profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
-TRAFFIC_BYTES_PER_LAUNCH = 1.581e9
+# FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters'
+# reading of a 1 GiB device copy (--copy-calibration: FETCH x 2.000, WRITE x
+# 1.000): FP4 planes 158,867 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
+# Summary and commands: profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
+TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3294e9, "int8": 1.581e9}
 
 
 def parse():
@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--planes", default="int8", choices=["int8", "fp4"],
+    ap.add_argument("--planes", default="fp4", choices=["int8", "fp4"],
                     help="operand planes of the fused sums (windowed and genotype workloads): int8 MFMA, or FP4 "
                          "(e2m1 0 / 1) on the FP4 MFMA")
     ap.add_argument("--method", default="mfma", choices=["mfma", "popcount"],
@@ -200,7 +200,7 @@ def run_windowed_hap(args, torch):
     log(f"[bench] fused step {ms_step:.3f} ms")
     band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
     roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs,
-                         TRAFFIC_BYTES_PER_LAUNCH if (n_var, n_hap) == (100_000, 2000) and not fp4 else None, fp4)
+                         TRAFFIC_BYTES_PER_LAUNCH[args.planes] if (n_var, n_hap) == (100_000, 2000) else None, fp4)
     log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
 
     # plain per-pair path on identical inputs and outputs (speedup vs plain)

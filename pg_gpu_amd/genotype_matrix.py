@@ -207,6 +207,6 @@ class GenotypeMatrix:
         rows = [self.sample_sets[p] if isinstance(p, str) else p for p in pops]
         bins = np.asarray(bp_bins, dtype=np.float64)
         plan = B.BandPlan(self._pos_np(), float(bins[-1]), edges=bins)
-        packed = [B.pack_population(self._genotypes, r, "geno") for r in rows]
+        packed = [B.pack_population(self._genotypes, r, "geno", fp4=True) for r in rows]
         sums, kp = plan.sums(packed, want_pairs=True)
         return sums, kp, LP.ld_names(len(rows))

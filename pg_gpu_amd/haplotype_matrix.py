@@ -312,7 +312,7 @@ class HaplotypeMatrix:
         self.transfer_to_gpu()
         bins = np.asarray(bp_bins, dtype=np.float64)
         plan = B.BandPlan(self._pos_np(), float(bins[-1]), edges=bins)
-        packed = [B.pack_population(self._haplotypes, r, "hap") for r in pops]
+        packed = [B.pack_population(self._haplotypes, r, "hap", fp4=True) for r in pops]
         sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
         return sums.cpu().numpy(), kp.cpu().numpy()
 
@@ -476,7 +476,7 @@ class HaplotypeMatrix:
         rows = [self.sample_sets[p] if isinstance(p, str) else p for p in pops]
         md = float(window if max_dist is None else max_dist)
         plan = B.BandPlan(self._pos_np(), md, window=float(window), n_keys=n_windows, origin=origin)
-        packed = [B.pack_population(self._haplotypes, r, "hap") for r in rows]
+        packed = [B.pack_population(self._haplotypes, r, "hap", fp4=True) for r in rows]
         sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
         P = len(rows)
         if stat_set == "moments" or P == 1:

@@ -154,7 +154,7 @@ def _compute_ld_sums(mat, pops, bins, gen, max_bp, use_genotypes, method="auto")
     fam = "geno" if use_genotypes else "hap"
     if method == "tiled" or (method == "auto" and _fused_supported(use_genotypes, P)):
         plan = B.BandPlan(pos, max_bp, edges=bins, key_coord=gen, abs_dist=gen is not None)
-        packed = [B.pack_population(data, r, fam) for r in rows]
+        packed = [B.pack_population(data, r, fam, fp4=True) for r in rows]
         return plan.sums(packed).cpu().numpy()
     return _plain_ld_sums(data, pos, rows, bins, gen, max_bp, use_genotypes)
 
