@@ -426,12 +426,13 @@ def run_dense_r2(args, torch):
     plan = band.BandPlan(pos, np.inf, window=1e15, n_keys=1)
     pair_samples = plan.n_pairs * n_hap
     log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
-    pk = band.pack_population(hap_d, None, "hap")
+    fp4 = args.planes == "fp4" and args.method == "mfma"
+    pk = band.pack_population(hap_d, None, "hap", fp4=fp4)
     assert not pk.has_missing
     cap = 1 << 22
 
     def step():
-        p2 = band.pack_population(hap_d, None, "hap", validate=False, out=pk)
+        p2 = band.pack_population(hap_d, None, "hap", validate=False, out=pk, fp4=fp4)
         return plan.r2(p2, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=False, method=args.method)
 
     ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
@@ -445,7 +446,7 @@ def run_dense_r2(args, torch):
         kms.append(plan.kernel_ms())
     plan.set_timing(False)
     k_ms = float(np.mean(kms))
-    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_band_r2", macs)
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_band_r2", macs, fp4=fp4)
     if args.method == "popcount":  # VALU variant: report against the int8 peak all the same, say so
         roof["ops"] = "pair*sample MACs x 2 as AND + popcount on the VALU (bit planes), against the int8 MFMA peak"
         roof["kernel"] = "k_band_r2<popcount>"
@@ -468,10 +469,11 @@ def run_dense_r2(args, torch):
         log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
     line = base_line(args, value, ms_step,
                      f"dense all-pairs r^2: {n_var:,} SNPs x {n_hap:,} haplotypes, neutral SFS, no missing data; "
-                     "r^2 of every pair from exact int32 counts, 100-bin histogram on [0, 1] (NaN pairs counted "
+                     "r^2 of every pair from exact integer counts, 100-bin histogram on [0, 1] (NaN pairs counted "
                      "apart) + sparse (i, j, r^2) list for r^2 > 0.2",
                      {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
-                      "n_sparse": out["n_sparse"], "method": args.method})
+                      "n_sparse": out["n_sparse"], "method": args.method, "planes": "fp4" if fp4 else "int8"},
+                     dtype="fp4" if fp4 else "int8")
     line.update({"roofline": roof, "cpu_baseline": cpu})
     return line
 

@@ -47,6 +47,8 @@ _SIGNATURES = {
     "pgld_band_plan_set_timing": ([_vp, _i32], C.c_int),
     "pgld_band_ld_pairs": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                             _vp, _vp], C.c_int),
+    "pgld_band_ld_pairs_fp4": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                            _vp, _vp], C.c_int),
     "pgld_pack_bits": ([_vp, _i64, _i64, _i64, _vp, _vp], C.c_int),
     "pgld_band_counts_bits": ([_vp, _i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "pgld_band_r2_bits": ([_vp, _vp, _vp, _i64, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],

@@ -209,8 +209,8 @@ class BandPlan:
           r2        FP64 per band pair in (i, j) order if per_pair,
           row_sums, col_sums, row_counts, col_counts  per-SNP sums over the
                     pairs (v, j > v) and (i < v, v) if snp_sums."""
-        if pop.fp4:
-            raise ValueError("the r^2 passes take int8 planes (pack without fp4)")
+        if pop.fp4 and method != "mfma":
+            raise ValueError("the popcount variant takes int8 planes (pack without fp4)")
         geno_est = estimator in ("rogers_huff_r", "dosage_r2")
         if (pop.family == L.FAMILY_GENO) != geno_est:
             raise ValueError("estimators 'r2', 'naive', 'sigma_d2' take haplotype planes; 'rogers_huff_r' and "
@@ -238,7 +238,10 @@ class BandPlan:
         sums = L.empty((2 * self.n_var,), np.float64) if snp_sums else None
         cnts = L.empty((2 * self.n_var,), np.int64) if snp_sums else None
         if method == "mfma":
-            L.call("pgld_band_ld_pairs", self._h, L.ptr(pop.planes), L.ptr(pop.marg), pop.k_pad, pop.n_samples,
+            # FP4 planes: haplotype estimators without missing data or per-SNP
+            # sums (the library raises otherwise)
+            L.call("pgld_band_ld_pairs_fp4" if pop.fp4 else "pgld_band_ld_pairs", self._h, L.ptr(pop.planes),
+                   L.ptr(pop.marg), pop.k_pad, pop.n_samples,
                    int(bool(with_missing)), est, int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),
                    L.ptr(sp_r), cap, L.ptr(n_sp), L.ptr(r2), L.ptr(sums), L.ptr(cnts), L.stream_handle())
         else:

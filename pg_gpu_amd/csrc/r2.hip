@@ -350,7 +350,9 @@ constexpr int R2W_HIST_PER_WAVE = R2_MAX_HIST_LDS / 4;
 constexpr int R2W_OP_BYTES = 2 * R2W_ROWS * OP_ROW;
 constexpr int R2W_RESIDENT = 64;  // workgroups resident per XCD (32 CUs x 2)
 
-template <int EST>
+// NIB: FP4 planes (rows of k_pad / 2 bytes, 32 samples per 16-byte
+// fragment) on v_mfma_f32_32x32x64_f8f6f4 with float accumulators.
+template <int EST, bool NIB = false>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, const SuperTile* __restrict__ sts,
                                                                   const int8_t* __restrict__ planes,
                                                                   const int32_t* __restrict__ marg, int64_t k_pad,
@@ -397,10 +399,11 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
     const int e = tid + BAND_THREADS * m;
     const int q = e & 7, row = (e >> 3) & (R2W_ROWS - 1), side = e >> 10;
     const int64_t snp = min((int64_t)(side ? st.j0 : st.i0) + row, ra.n_var - 1);
-    src[m] = (const GLOBAL_AS int8_t*)planes + snp * k_pad + 16 * q;  // plane 0 (h)
+    src[m] = (const GLOBAL_AS int8_t*)planes + snp * (NIB ? k_pad / 2 : k_pad) + 16 * q;  // plane 0 (h)
     dst[m] = (side * R2W_ROWS + row) * OP_ROW + 16 * q;
   }
-  v16i acc[2][2];
+  if constexpr (NIB) k_pad /= 2;  // from here on: bytes per operand row
+  typename std::conditional<NIB, v16f, v16i>::type acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -451,8 +454,16 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[sub & 1][a], B[sub & 1][b], acc[a][b], 0, 0, 0);
+          for (int b = 0; b < 2; ++b) {
+            if constexpr (NIB) {
+              const v4i x = A[sub & 1][a], y = B[sub & 1][b];
+              const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
+              acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc[a][b], 4, 4, 0, FP4_UNIT_SCALE,
+                                                                          0, FP4_UNIT_SCALE);
+            } else {
+              acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[sub & 1][a], B[sub & 1][b], acc[a][b], 0, 0, 0);
+            }
+          }
       }
     }
   };
@@ -477,14 +488,22 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
     for (int blk = 0; blk < 4; ++blk) {
       const int a = blk >> 1, b = blk & 1;
       v16i cur;
+      auto take = [&](const auto& x) {  // float accumulators (FP4) hold exact integer counts
+        if constexpr (NIB) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) cur[r] = (int)x[r];
+        } else {
+          cur = x;
+        }
+      };
       if (blk == 0)
-        cur = acc[0][0];
+        take(acc[0][0]);
       else if (blk == 1)
-        cur = acc[0][1];
+        take(acc[0][1]);
       else if (blk == 2)
-        cur = acc[1][0];
+        take(acc[1][0]);
       else
-        cur = acc[1][1];
+        take(acc[1][1]);
       const int jl = wc + 32 * b + (lane & 31);
       const int64_t j = (int64_t)st.j0 + jl;
       // one row of the block; FAST: every pair valid and in key slot
@@ -628,7 +647,21 @@ extern "C" {
 static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, const int32_t* marg, int64_t k_pad,
                    int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
                    uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
-                   uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream) {
+                   uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream,
+                   bool fp4 = false) {
+  if (fp4) {
+    // FP4 planes run on the 128 x 128 super-tile kernel only
+    PGLD_REQUIRE(plan, "pgld_band_ld_pairs_fp4: null plan");
+    PGLD_REQUIRE(k_pad % 256 == 0, "pgld_band_ld_pairs_fp4: k_pad must be a multiple of 256 (FP4 planes)");
+    PGLD_REQUIRE(estimator == EST_R2 || estimator == EST_R2_NAIVE || estimator == EST_SIGMA_D2,
+                 "pgld_band_ld_pairs_fp4: haplotype estimators only (r2, naive, sigma_d2)");
+    if (with_missing || snp_sums || snp_counts || plan->n_super == 0 ||
+        (hist && (int64_t)plan->max_tile_slots * (n_hist + 1) > R2W_HIST_PER_WAVE)) {
+      set_error("pgld_band_ld_pairs_fp4: FP4 planes are taken for haplotypes without missing data, without "
+                "per-SNP sums, on single-band plans whose per-tile histogram fits the super-tile kernel");
+      return PGLD_EUNSUPPORTED;
+    }
+  }
   PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_DIP_R2, "pgld_band_ld_pairs: unknown estimator %d",
                estimator);
   PGLD_REQUIRE(!(estimator == EST_RH_R && with_missing),
@@ -696,6 +729,23 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
   // haplotypes without missing data and no per-SNP sums: the 128 x 128 kernel
   const bool wide = !geno && !miss && !popc && !snp_sums && plan->n_super > 0 &&
                     (!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2W_HIST_PER_WAVE);
+  if (wide && fp4) {
+    dim3 gw((unsigned)plan->n_super);
+    const auto* sts = reinterpret_cast<const SuperTile*>(plan->d_super);
+    if (estimator == EST_R2)
+      hipLaunchKernelGGL((k_band_r2_wide<EST_R2, true>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad,
+                         n_samples);
+    else if (estimator == EST_R2_NAIVE)
+      hipLaunchKernelGGL((k_band_r2_wide<EST_R2_NAIVE, true>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad,
+                         n_samples);
+    else
+      hipLaunchKernelGGL((k_band_r2_wide<EST_SIGMA_D2, true>), gw, b, R2W_OP_BYTES, s, ra, sts, planes, marg, k_pad,
+                         n_samples);
+    rc = check_launch("k_band_r2_wide");
+    if (rc) return rc;
+    if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[1], s), "pgld_band_r2 event");
+    return PGLD_OK;
+  }
   if (wide) {
     dim3 gw((unsigned)plan->n_super);
     const auto* sts = reinterpret_cast<const SuperTile*>(plan->d_super);
@@ -758,6 +808,14 @@ int pgld_band_ld_pairs(const pgld_band_plan* plan, const int8_t* planes, const i
                        uint64_t* n_sparse, double* values, double* snp_sums, int64_t* snp_counts, void* stream) {
   return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, estimator, n_hist, thresh, hist,
                  sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, values, snp_sums, snp_counts, stream);
+}
+
+int pgld_band_ld_pairs_fp4(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
+                           int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
+                           uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
+                           uint64_t* n_sparse, double* values, double* snp_sums, int64_t* snp_counts, void* stream) {
+  return band_r2(plan, planes, false, marg, k_pad, n_samples, with_missing, estimator, n_hist, thresh, hist,
+                 sparse_i, sparse_j, sparse_r2, sparse_cap, n_sparse, values, snp_sums, snp_counts, stream, true);
 }
 
 int pgld_band_r2_bits(const pgld_band_plan* plan, const uint32_t* bits, const int32_t* marg, int64_t k_pad,

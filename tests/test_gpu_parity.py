@@ -462,3 +462,25 @@ def test_band_sums_fp4_equal_int8(family, P, missing, stat_set, key):
         plan.sums([p8[0]] + p4[1:] if P > 1 else [p4[0], p8[0]], stat_set=stat_set)
     with pytest.raises(ValueError):
         plan.counts(p4[0])
+
+
+@pytest.mark.parametrize("estimator", ["r2", "naive", "sigma_d2"])
+def test_band_r2_fp4_equal_int8(estimator):
+    """Dense / banded r^2 on the FP4 super-tile kernel: histograms, sparse
+    lists and per-pair values bit-identical to the int8 kernel; FP4 planes
+    with missing data or per-SNP sums are refused."""
+    d, pos = hap_data(130, 300, 700, 0.0)
+    plan = band.BandPlan(pos, np.inf, window=1e300, n_keys=1)
+    p8 = band.pack_population(d, None, "hap")
+    p4 = band.pack_population(d, None, "hap", fp4=True)
+    a = plan.r2(p8, n_hist=50, thresh=0.2, per_pair=True, estimator=estimator)
+    b = plan.r2(p4, n_hist=50, thresh=0.2, per_pair=True, estimator=estimator)
+    for k in ("hist", "nan_pairs", "sparse_i", "sparse_j", "sparse_r2", "r2"):
+        np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
+    assert b["n_sparse"] == a["n_sparse"]
+    dm, _ = hap_data(131, 300, 400, 0.02)
+    with pytest.raises(RuntimeError):
+        plan2 = band.BandPlan(np.arange(400, dtype=np.float64), np.inf, window=1e300, n_keys=1)
+        plan2.r2(band.pack_population(dm, None, "hap", fp4=True), estimator=estimator)
+    with pytest.raises(RuntimeError):
+        plan.r2(p4, histogram=False, sparse_cap=0, estimator=estimator, snp_sums=True)
