@@ -490,7 +490,8 @@ def run_ragged_regions(args, torch):
     datas, pos, rows = synthetic.ragged_regions(n_regions=n_regions, device=dev)
     rows_d = [[torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in rr] for rr in rows]
     S = np.array([d.shape[0] for d in datas], dtype=np.int64)
-    packed = batch.pack_regions(datas, rows_d)  # validates once
+    fp4 = args.planes == "fp4"
+    packed = batch.pack_regions(datas, rows_d, fp4=fp4)  # validates once
     with_missing = any(p.has_missing for pl in packed for p in pl)
     t0 = time.perf_counter()
     b = batch.RegionBatch(pos, 1e6, edges).bind(packed, stat_set="self")
@@ -506,7 +507,7 @@ def run_ragged_regions(args, torch):
     kp = torch.empty((n_regions, 16), dtype=torch.float64, device=dev)
 
     def step_body():
-        batch.pack_regions(datas, rows_d, out=packed, validate=False)
+        batch.pack_regions(datas, rows_d, out=packed, validate=False, fp4=fp4)
         b.sums(with_missing, out=out, key_pairs=kp)
 
     graph = batch.Graph().capture(step_body)
@@ -526,7 +527,7 @@ def run_ragged_regions(args, torch):
         kms.append(b.kernel_ms())
     b.set_timing(False)
     k_ms = float(np.mean(kms))
-    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_batch_sums", macs)
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_batch_sums", macs, fp4=fp4)
     log(f"[bench] k_batch_sums {k_ms:.3f} ms -> {roof['achieved']:.1f} int8 TOP/s")
 
     # the same work region by region: one band plan and one fused launch each
@@ -575,7 +576,7 @@ def run_ragged_regions(args, torch):
                      "pairs; per-region 16 log-bin DD/Dz/pi2 sums per population (FP64) from one HIP graph "
                      "(packing + one autotuned persistent launch)",
                      {"regions": n_regions, "pairs": b.n_pairs, "pair_samples": pair_samples, "tiles": b.n_tiles,
-                      "schedule": choice})
+                      "schedule": choice, "planes": args.planes}, dtype="fp4" if fp4 else "int8")
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "plain": "one band plan and one fused launch per region", "batch_vs_plain_max_abs": check,

@@ -311,6 +311,12 @@ int pgld_batch_plan_create(int32_t n_regions, const int64_t* n_var, const double
 int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
                     const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
                     const int32_t* n_samples);
+/* pgld_batch_bind for FP4 planes (pgld_pack_planes_fp4, every k_pad a
+ * multiple of 256; no counterpart in the reference): the bound batch runs its
+ * products on the FP4 MFMA, sums bit-identical to the int8 binding. */
+int pgld_batch_bind_fp4(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                    const int32_t* n_samples);
 /* Fused counts -> statistics -> per-key sums for every region: sums is
  * (n_regions * n_keys, n_stats) FP64, key_pairs (optional) the pair count
  * per key.  Allocates nothing, records no memset: may be captured. */

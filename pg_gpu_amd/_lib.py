@@ -55,6 +55,7 @@ _SIGNATURES = {
                           C.c_int),
     "pgld_batch_plan_create": ([_i32, _vp, _vp, _f64, _i32, _vp, _vp, _i32, _f64, _f64, C.POINTER(_vp)], C.c_int),
     "pgld_batch_bind": ([_vp, _i32, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32)], C.c_int),
+    "pgld_batch_bind_fp4": ([_vp, _i32, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32)], C.c_int),
     "pgld_batch_sums": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
     "pgld_batch_schedule": ([_vp, _i32, _i32, _i32], C.c_int),
     "pgld_batch_autotune": ([_vp, _i32, _vp, _vp, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32),

@@ -67,12 +67,13 @@ class RegionBatch:
             raise ValueError("one population list per region")
         P = len(pops[0])
         fam = pops[0][0].family
+        fp4 = pops[0][0].fp4
         for r, pl in enumerate(pops):
             if len(pl) != P or any(p.family != fam for p in pl):
                 raise ValueError(f"region {r}: every region needs {P} populations of one family")
             for p in pl:
-                if p.fp4:
-                    raise ValueError(f"region {r}: region batches take int8 planes (pack without fp4)")
+                if p.fp4 != fp4:
+                    raise ValueError(f"region {r}: every population must be packed in one layout (int8 or FP4)")
                 if p.planes.shape[1] != self.n_var[r]:
                     raise ValueError(f"region {r}: planes have {p.planes.shape[1]} SNPs, plan has {self.n_var[r]}")
         flat = [p for pl in pops for p in pl]
@@ -81,7 +82,7 @@ class RegionBatch:
         margs, k2 = L.ptr_array([p.marg for p in flat])
         kpad = (C.c_int64 * len(flat))(*[p.k_pad for p in flat])
         ns = (C.c_int32 * len(flat))(*[p.n_samples for p in flat])
-        L.call("pgld_batch_bind", self._h, fam, P, sset, planes, margs, kpad, ns)
+        L.call("pgld_batch_bind_fp4" if fp4 else "pgld_batch_bind", self._h, fam, P, sset, planes, margs, kpad, ns)
         self.n_stats = (3 * P) if (sset == L.SET_SELF and P > 1) else {1: 3, 2: 15, 3: 45, 4: 105}[P]
         self._bound = (flat, k1, k2)  # keep the planes alive while bound
         return self
@@ -191,14 +192,14 @@ class Graph:
             pass
 
 
-def pack_regions(datas, rows_per_region, family="hap", out=None, validate=True):
+def pack_regions(datas, rows_per_region, family="hap", out=None, validate=True, fp4=False):
     """PackedPop lists for every region: datas[r] is region r's int8 matrix,
-    rows_per_region[r] its population row lists."""
+    rows_per_region[r] its population row lists (fp4: FP4 planes)."""
     from .band import pack_population
     res = []
     for r, (d, rows) in enumerate(zip(datas, rows_per_region)):
         prev = out[r] if out is not None else [None] * len(rows)
-        res.append([pack_population(d, rw, family, validate=validate, out=o) for rw, o in zip(rows, prev)])
+        res.append([pack_population(d, rw, family, validate=validate, out=o, fp4=fp4) for rw, o in zip(rows, prev)])
     return res
 
 

@@ -367,7 +367,7 @@ constexpr int batch_min_waves(int family, int P, int set) {
   return (family == PGLD_FAMILY_HAP && (P <= 3 || set == PGLD_SET_SELF)) ? band_min_waves(family, P, set) : 1;
 }
 
-template <int FAMILY, bool MISSING, int P, int SET>
+template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
 __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void k_batch_sums(
     BandArgs ba, const RegionDesc* __restrict__ regions, const int32_t* __restrict__ order, int64_t n_tiles,
     int32_t xcd_split) {
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     __syncthreads();  // the previous tile's LDS is no longer read
     // rd.pa stays in global memory: the population loop indexes it at run time
     // one value per butterfly for the statistic sets that would otherwise spill
-    band_sums_tile<FAMILY, MISSING, P, SET, (SET == PGLD_SET_SELF ? 8 : 1)>(lb, rd.pa, tile, key_base, smem);
+    band_sums_tile<FAMILY, MISSING, P, SET, (SET == PGLD_SET_SELF ? 8 : 1), NIB>(lb, rd.pa, tile, key_base, smem);
   }
 }
 
@@ -951,11 +951,11 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
 template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
 static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
                         hipStream_t s) {
-  if (!NIB && plan->n_regions > 0) {  // FP4 planes are taken by single bands only (checked by the caller)
+  if (plan->n_regions > 0) {
     const int m = plan->order_mode;
     const int64_t len = plan->order_len[m];
     const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
-    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET>), dim3((unsigned)grid), dim3(BAND_THREADS), lds, s,
+    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB>), dim3((unsigned)grid), dim3(BAND_THREADS), lds, s,
                        ba, reinterpret_cast<const RegionDesc*>(plan->d_regions), plan->d_order + plan->order_off[m],
                        len, (int32_t)plan->xcd_split);
   } else {
@@ -1381,15 +1381,18 @@ int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pop
                   as_stream(stream), true, true);
 }
 
-int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
-                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
-                    const int32_t* n_samples) {
+static int batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                      const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                      const int32_t* n_samples, bool fp4) {
   PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples, "pgld_batch_bind: null pointer");
   PGLD_REQUIRE(plan->n_regions > 0, "pgld_batch_bind: not a region batch plan");
   PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_batch_bind: bad family");
   PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_batch_bind: n_pops must be 1..4");
   PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_batch_bind: bad stat_set");
   PGLD_REQUIRE(launcher_for(family, n_pops, true, stat_set), "pgld_batch_bind: no fused kernel for this set");
+  if (fp4)
+    for (int64_t e = 0; e < (int64_t)plan->n_regions * n_pops; ++e)
+      PGLD_REQUIRE(k_pad[e] % 256 == 0, "pgld_batch_bind_fp4: k_pad must be a multiple of 256 (FP4 planes)");
   const int R = plan->n_regions;
   std::vector<RegionDesc> rd(R);
   std::vector<double> wsum(R, 0.0);  // MFMA work per tile pair of the region (sum of k_pad)
@@ -1437,9 +1440,22 @@ int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_
   int rc = ensure_partials(plan, n_stats_of(family, n_pops, stat_set) + 1);
   if (rc) return rc;
   plan->bound_family = family;
+  plan->bound_fp4 = fp4;
   plan->bound_pops = n_pops;
   plan->bound_set = stat_set;
   return PGLD_OK;
+}
+
+int pgld_batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                    const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                    const int32_t* n_samples) {
+  return batch_bind(plan, family, n_pops, stat_set, planes, marg, k_pad, n_samples, false);
+}
+
+int pgld_batch_bind_fp4(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
+                        const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                        const int32_t* n_samples) {
+  return batch_bind(plan, family, n_pops, stat_set, planes, marg, k_pad, n_samples, true);
 }
 
 int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream) {
@@ -1447,7 +1463,7 @@ int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, do
   PGLD_REQUIRE(plan->n_regions > 0 && plan->bound_family >= 0, "pgld_batch_sums: plan is not a bound region batch");
   PopArgs none{};
   return run_sums(plan, plan->bound_family, plan->bound_pops, none, with_missing, plan->bound_set, sums, key_pairs,
-                  as_stream(stream), false);
+                  as_stream(stream), false, plan->bound_fp4);
 }
 
 int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split) {
@@ -1474,8 +1490,8 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
   double mean_k = 0.0;
   for (int64_t r = 0; r < plan->n_regions; ++r) mean_k += (double)plan->h_region_nvar[r];
   char key[160];
-  snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%d", plan->bound_family, plan->bound_pops, plan->bound_set,
-           with_missing != 0, (int)std::log2((double)std::max<int64_t>(1, plan->n_tiles)),
+  snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%d/%d", plan->bound_family, (int)plan->bound_fp4, plan->bound_pops,
+           plan->bound_set, with_missing != 0, (int)std::log2((double)std::max<int64_t>(1, plan->n_tiles)),
            (int)std::log2(1.0 + mean_k / plan->n_regions), cus);
   static std::mutex mu;
   static std::map<std::string, std::array<int32_t, 3>> cache;

@@ -44,6 +44,7 @@ struct pgld_band_plan {
   void* d_regions = nullptr;           // RegionDesc[n_regions], written by bind
   int32_t* d_order = nullptr;          // tile orders, see order_mode (-1 entries are skipped)
   int32_t bound_family = -1, bound_pops = 0, bound_set = -1;
+  bool bound_fp4 = false;              // the bound planes are FP4 (pgld_batch_bind_fp4)
   int32_t grid = 0;                    // persistent grid (0 = one workgroup per tile)
   int32_t order_mode = 0;              // 0 cost-descending, 1 natural, 2 L2-blocked per XCD
   int64_t order_off[3] = {0, 0, 0};    // start of each order in d_order

@@ -81,6 +81,25 @@ def test_batch_schedules_and_autotune_are_bit_identical(small_batch):
     assert again.autotune(True)["ms"] is None  # answered from the cache
 
 
+
+def test_batch_fp4_equals_int8(small_batch):
+    """A region batch bound to FP4 planes gives the int8 binding's sums bit
+    for bit, under every schedule; layouts cannot be mixed."""
+    _, pos, _, d_dev, rows_dev = small_batch
+    ref = np_(batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(d_dev, rows_dev), stat_set="self")
+              .sums(True))
+    p4 = batch.pack_regions(d_dev, rows_dev, fp4=True)
+    b = batch.RegionBatch(pos, 1e6, EDGES).bind(p4, stat_set="self")
+    for grid, order in ((0, "blocked"), (256, "cost"), (1024, "natural")):
+        b.schedule(grid, order, False)
+        np.testing.assert_array_equal(np_(b.sums(True)), ref)
+    b.autotune(True)
+    np.testing.assert_array_equal(np_(b.sums(True)), ref)
+    mixed = batch.pack_regions(d_dev, rows_dev)
+    mixed[0][0] = p4[0][0]
+    with pytest.raises(ValueError):
+        batch.RegionBatch(pos, 1e6, EDGES).bind(mixed, stat_set="self")
+
 def test_batch_moments_two_pops_and_genotypes():
     datas, pos, rows = SY.ragged_regions(seed=32, n_regions=5, m_lo=60, m_hi=300, s_lo=40, s_hi=120, n_pops=2,
                                          span=30_000)
