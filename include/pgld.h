@@ -262,12 +262,11 @@ int pgld_band_ld_pairs(const pgld_band_plan* plan, const int8_t* planes, const i
                        int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
                        uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
                        uint64_t* n_sparse, double* values, double* snp_sums, int64_t* snp_counts, void* stream);
-/* pgld_band_ld_pairs over FP4 planes (pgld_pack_planes_fp4; no counterpart in
- * the reference): the 128 x 128 super-tile kernel with its products on
- * v_mfma_f32_32x32x64_f8f6f4, 64 samples per instruction and exact float
- * counts; outputs equal pgld_band_ld_pairs' bit for bit.  Haplotype estimators
- * (r2, naive, sigma_d2) without missing data and without per-SNP sums on a
- * single-band plan; other cases return PGLD_EUNSUPPORTED. */
+/* pgld_band_ld_pairs over FP4 planes (pgld_pack_planes_fp4, k_pad a multiple
+ * of 256; no counterpart in the reference): every estimator, with or without
+ * missing data and per-SNP sums, with its products on
+ * v_mfma_f32_32x32x64_f8f6f4 (64 samples per instruction, exact float
+ * counts); outputs equal pgld_band_ld_pairs' bit for bit. */
 int pgld_band_ld_pairs_fp4(const pgld_band_plan* plan, const int8_t* planes, const int32_t* marg, int64_t k_pad,
                        int32_t n_samples, int32_t with_missing, int32_t estimator, int32_t n_hist, double thresh,
                        uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,

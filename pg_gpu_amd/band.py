@@ -238,8 +238,6 @@ class BandPlan:
         sums = L.empty((2 * self.n_var,), np.float64) if snp_sums else None
         cnts = L.empty((2 * self.n_var,), np.int64) if snp_sums else None
         if method == "mfma":
-            # FP4 planes: haplotype estimators without missing data or per-SNP
-            # sums (the library raises otherwise)
             L.call("pgld_band_ld_pairs_fp4" if pop.fp4 else "pgld_band_ld_pairs", self._h, L.ptr(pop.planes),
                    L.ptr(pop.marg), pop.k_pad, pop.n_samples,
                    int(bool(with_missing)), est, int(n_hist), float(thresh), L.ptr(hist), L.ptr(sp_i), L.ptr(sp_j),

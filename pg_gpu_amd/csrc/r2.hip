@@ -104,7 +104,7 @@ __device__ __forceinline__ double hap_pair_value(int32_t n11, int32_t nA, int32_
 }
 
 
-template <int FAMILY, bool MISSING, bool POPC, int EST>
+template <int FAMILY, bool MISSING, bool POPC, int EST, bool NIB = false>
 __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const int8_t* __restrict__ planes,
                                                              const int32_t* __restrict__ marg, int64_t k_pad,
                                                              int32_t S) {
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   const int64_t last_i = min((int64_t)ri + 31, ra.n_var - 1);
   const bool wave_live = (int64_t)rj < ra.n_var && rj + 31 > ri && (int64_t)ri < ra.n_var &&
                          s_pos_j[wc] - s_pos_i[last_i - tile.i0] <= ra.max_dist;
-  WaveAcc<PL::N> acc;
-  block_products<FAMILY, MISSING, POPC>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
+  WaveAcc<PL::N, NIB> acc;  // float accumulators (FP4 planes) hold exact integer counts
+  block_products<FAMILY, MISSING, POPC, NIB>(planes, ra.n_var, k_pad, tile.i0, tile.j0, wr, wc, wave_live, smem,
                                                  acc);
 
   // pairs of histogram cell 0 (first key, r^2 below 1 / n_hist) are counted
@@ -649,19 +649,7 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
                    uint64_t* hist, int32_t* sparse_i, int32_t* sparse_j, float* sparse_r2, int64_t sparse_cap,
                    uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream,
                    bool fp4 = false) {
-  if (fp4) {
-    // FP4 planes run on the 128 x 128 super-tile kernel only
-    PGLD_REQUIRE(plan, "pgld_band_ld_pairs_fp4: null plan");
-    PGLD_REQUIRE(k_pad % 256 == 0, "pgld_band_ld_pairs_fp4: k_pad must be a multiple of 256 (FP4 planes)");
-    PGLD_REQUIRE(estimator == EST_R2 || estimator == EST_R2_NAIVE || estimator == EST_SIGMA_D2,
-                 "pgld_band_ld_pairs_fp4: haplotype estimators only (r2, naive, sigma_d2)");
-    if (with_missing || snp_sums || snp_counts || plan->n_super == 0 ||
-        (hist && (int64_t)plan->max_tile_slots * (n_hist + 1) > R2W_HIST_PER_WAVE)) {
-      set_error("pgld_band_ld_pairs_fp4: FP4 planes are taken for haplotypes without missing data, without "
-                "per-SNP sums, on single-band plans whose per-tile histogram fits the super-tile kernel");
-      return PGLD_EUNSUPPORTED;
-    }
-  }
+  if (fp4) PGLD_REQUIRE(k_pad % 256 == 0, "pgld_band_ld_pairs_fp4: k_pad must be a multiple of 256 (FP4 planes)");
   PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_DIP_R2, "pgld_band_ld_pairs: unknown estimator %d",
                estimator);
   PGLD_REQUIRE(!(estimator == EST_RH_R && with_missing),
@@ -762,8 +750,14 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
     if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[1], s), "pgld_band_r2 event");
     return PGLD_OK;
   }
-#define PGLD_R2(F, M, B, E) \
-  hipLaunchKernelGGL((k_band_r2<F, M, B, E>), g, b, lds, s, ra, planes, marg, k_pad, n_samples)
+  // FP4 planes take the NIB instances (never with bit planes: popc and fp4 come from separate entry points)
+#define PGLD_R2(F, M, B, E)                                                                                  \
+  do {                                                                                                       \
+    if (fp4)                                                                                                 \
+      hipLaunchKernelGGL((k_band_r2<F, M, B, E, !B>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);     \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_band_r2<F, M, B, E, false>), g, b, lds, s, ra, planes, marg, k_pad, n_samples);  \
+  } while (0)
 #define PGLD_R2_EST(M, B)                                                        \
   do {                                                                           \
     if (estimator == EST_R2) PGLD_R2(PGLD_FAMILY_HAP, M, B, EST_R2);             \

@@ -466,9 +466,8 @@ def test_band_sums_fp4_equal_int8(family, P, missing, stat_set, key):
 
 @pytest.mark.parametrize("estimator", ["r2", "naive", "sigma_d2"])
 def test_band_r2_fp4_equal_int8(estimator):
-    """Dense / banded r^2 on the FP4 super-tile kernel: histograms, sparse
-    lists and per-pair values bit-identical to the int8 kernel; FP4 planes
-    with missing data or per-SNP sums are refused."""
+    """Dense r^2 on the FP4 super-tile kernel: histograms, sparse lists and
+    per-pair values bit-identical to the int8 kernel."""
     d, pos = hap_data(130, 300, 700, 0.0)
     plan = band.BandPlan(pos, np.inf, window=1e300, n_keys=1)
     p8 = band.pack_population(d, None, "hap")
@@ -478,9 +477,26 @@ def test_band_r2_fp4_equal_int8(estimator):
     for k in ("hist", "nan_pairs", "sparse_i", "sparse_j", "sparse_r2", "r2"):
         np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
     assert b["n_sparse"] == a["n_sparse"]
-    dm, _ = hap_data(131, 300, 400, 0.02)
-    with pytest.raises(RuntimeError):
-        plan2 = band.BandPlan(np.arange(400, dtype=np.float64), np.inf, window=1e300, n_keys=1)
-        plan2.r2(band.pack_population(dm, None, "hap", fp4=True), estimator=estimator)
-    with pytest.raises(RuntimeError):
-        plan.r2(p4, histogram=False, sparse_cap=0, estimator=estimator, snp_sums=True)
+
+
+@pytest.mark.parametrize("family,estimator,missing,snp_sums", [
+    ("hap", "r2", 0.02, False), ("hap", "naive", 0.02, True), ("hap", "sigma_d2", 0.0, True),
+    ("hap", "sigma_d2", 0.03, True), ("geno", "rogers_huff_r", 0.0, False),
+    ("geno", "rogers_huff_r", 0.0, True), ("geno", "dosage_r2", 0.03, False)])
+def test_band_r2_fp4_64x64_equal_int8(family, estimator, missing, snp_sums):
+    """FP4 planes on the 64 x 64 r^2 kernel (missing data, genotype
+    estimators, per-SNP sums, banded plans): every output bit-identical to
+    the int8 planes."""
+    d, pos = (hap_data(132, 270, 900, missing) if family == "hap" else geno_data(133, 150, 900, missing))
+    plan = band.BandPlan(pos, 15_000.0, window=1e300, n_keys=1)
+    p8 = band.pack_population(d, None, family)
+    p4 = band.pack_population(d, None, family, fp4=True)
+    kw = dict(n_hist=40, thresh=0.1, per_pair=not snp_sums, estimator=estimator, snp_sums=snp_sums)
+    a = plan.r2(p8, **kw)
+    b = plan.r2(p4, **kw)
+    assert set(a) == set(b)
+    for k in a:
+        if isinstance(a[k], (int, float)):
+            assert a[k] == b[k], k
+        else:
+            np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
