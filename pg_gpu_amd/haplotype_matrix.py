@@ -377,7 +377,7 @@ class HaplotypeMatrix:
         if m < 2:
             return out
         plan = B.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
-        pk = B.pack_population(self._haplotypes, None, "hap")
+        pk = B.pack_population(self._haplotypes, None, "hap", fp4=True)
         vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="naive")["r2"]
         iu = T.triu_indices(m, m, offset=1, device=out.device)
         out[iu[0], iu[1]] = vals
@@ -432,11 +432,11 @@ class HaplotypeMatrix:
         plan = B.BandPlan(pos, float(bins[-1]), edges=bins)
         if estimator == "rogers_huff":
             from .ld_statistics import _dosage_matrix
-            pk = B.pack_population(_dosage_matrix(self).genotypes, None, "geno")
+            pk = B.pack_population(_dosage_matrix(self).genotypes, None, "geno", fp4=True)
             r = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="rogers_huff_r")["r2"]
             vals = r * r
         else:
-            pk = B.pack_population(self._haplotypes, rows, "hap")
+            pk = B.pack_population(self._haplotypes, rows, "hap", fp4=True)
             vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="r2")["r2"]
         keys = L.empty((plan.n_pairs,), np.int32)
         pos_d = L.to_device(pos, np.float64)

@@ -198,7 +198,7 @@ def _snp_sums(mat, estimator):
     from . import band as B
     pos = np.arange(mat.num_variants, dtype=np.float64)  # all pairs: positions play no part
     plan = B.BandPlan(pos, np.inf, window=1e300, n_keys=1)
-    pk = B.pack_population(mat.haplotypes, None, "hap")
+    pk = B.pack_population(mat.haplotypes, None, "hap", fp4=True)
     out = plan.r2(pk, histogram=False, sparse_cap=0, estimator=estimator, snp_sums=True)
     return {k: out[k].cpu().numpy() for k in ("row_sums", "col_sums", "row_counts", "col_counts")}
 
@@ -333,7 +333,7 @@ def _all_pairs_values(gm, estimator, per_pair=True, snp_sums=False):
     from . import band as B
     m = gm.num_variants
     plan = B.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
-    pk = B.pack_population(gm.genotypes, None, "geno")
+    pk = B.pack_population(gm.genotypes, None, "geno", fp4=True)
     return plan.r2(pk, histogram=False, sparse_cap=0, per_pair=per_pair, estimator=estimator, snp_sums=snp_sums)
 
 

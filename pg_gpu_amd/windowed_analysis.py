@@ -96,7 +96,7 @@ def _layer_sums(hap_rows, pos, keep, start, stop, layer):
         return out
     sel = np.concatenate(cols)
     sub = hap_rows[:, L.to_device(sel, np.int64)]
-    pk = B.pack_population(sub, None, "hap")
+    pk = B.pack_population(sub, None, "hap", fp4=True)
     # same-window pairs differ by at most span < max_dist; pairs of different
     # windows by at least big - span = 3 span + 4 > max_dist
     plan = B.BandPlan(np.concatenate(coord), 2.0 * (span + 1.0), window=1e300, n_keys=1)
