@@ -13,13 +13,22 @@ Pinning (how each part is checked independently of this transcription):
 * 4-way / 9-way counts -- definition-level brute force, the reference KATs
   (tests/test_haplotype_counting.py:43-130), marginal/symmetry invariants.
 * r, r^2, D' -- closed-form KATs (tests/test_ld_r_squared.py:15-57,171-230).
-* haplotype DD/Dz/pi2, every population case -- brute-force U-statistics
-  over ordered tuples of haplotypes (distinct within a population) in
+* haplotype DD/Dz/pi2, every population case for 1-4 populations (the
+  four-distinct-population pi2 included) -- brute-force U-statistics over
+  ordered tuples of haplotypes (distinct within a population) in
   tests/ustat_bruteforce.py, and the frozen moments output on the IM example
   (tests/cache/moments_ld_test_im_4bins_deme0-deme1.pkl, decoded statically).
-* genotype DD/Dz/pi2 -- brute-force U-statistics over individuals with
-  dosage kernels (single population: exact; multi-population cases: the
-  same construction with distinctness within a population).
+* genotype DD/Dz/pi2, 1-4 populations -- brute-force U-statistics over
+  individuals with dosage kernels (distinctness within a population).
+* ld_names / het_names / stat_specs -- a literal three-population name list
+  written from moments' naming rule, the orbit form of that rule for 1-4
+  populations, and each spec as the uniform average over its statistic's
+  symmetry group.
+* union_biallelic_keep_geno -- known answers (all-reference, all-alternative,
+  one valid individual, duplicate rows across populations, rows in no
+  population).
+* heterozygosity, haplotype and genotype modes -- enumeration of allele pairs
+  per site (missing haplotypes counted as reference over the constant |pop|).
 * bin/window sums -- brute-force per-key sums over explicit pair lists.
 * dense r^2 / sigma_D^2 / dosage r^2 matrices, ZnS, Omega -- equal to the
   pinned count r^2 / numerator ratio / Rogers-Huff r^2 where they coincide;

@@ -121,10 +121,12 @@ def _hap_triple_normalised(p):
     return p
 
 
-@pytest.mark.parametrize("seed", range(5))
-def test_hap_stats_bruteforce(seed):
-    P = 3
-    rng = np.random.default_rng(seed)
+@pytest.mark.parametrize("P,seed", [(3, s) for s in range(5)] + [(4, s) for s in range(3)])
+def test_hap_stats_bruteforce(P, seed):
+    """Every (statistic, populations) call the moments specs make, up to four
+    populations: P = 4 reaches the pi2 case with four distinct populations
+    (haplotype_kernels.py:167-304, ld_statistics.py:1299-1617)."""
+    rng = np.random.default_rng(seed + 1000 * (P - 3))
     pops = [rng.integers(0, 2, size=(rng.integers(4, 6), 2)) for _ in range(P)]
     H = [O.HapPop(U.hap_counts_of(p)[None, :]) for p in pops]
     for t, p in _calls(P):
@@ -139,10 +141,11 @@ def test_hap_stats_bruteforce(seed):
         assert abs(float(got[0]) - exp) <= 1e-12 * max(1.0, abs(exp)), (t, p)
 
 
-@pytest.mark.parametrize("seed", range(4))
-def test_geno_stats_bruteforce(seed):
-    P = 3
-    rng = np.random.default_rng(100 + seed)
+@pytest.mark.parametrize("P,seed", [(3, s) for s in range(4)] + [(4, s) for s in range(3)])
+def test_geno_stats_bruteforce(P, seed):
+    """As above for genotypes; P = 4 reaches the all-distinct pi2 case of
+    genotype_kernels.py:279-405."""
+    rng = np.random.default_rng(100 + seed + 1000 * (P - 3))
     pops = [rng.integers(0, 3, size=(rng.integers(4, 6), 2)) for _ in range(P)]
     G = [O.GenoPop(U.geno_counts_of(p)[None, :]) for p in pops]
     fn = {"dd": O.geno_dd, "dz": O.geno_dz, "pi2": O.geno_pi2}
@@ -191,6 +194,152 @@ def test_names_counts_and_spec_weights():
     for P in (1, 2, 3, 4):
         for _, terms in O.stat_specs(P):
             assert abs(sum(w for w, _, _ in terms) - 1.0) < 1e-15
+
+
+# ld_names(3) written out by hand from moments' naming rule: DD over unordered
+# population pairs, Dz_i over unordered (j, k), pi2 over unordered pairs of
+# unordered pairs (locus-A pair, locus-B pair), each family in lexicographic
+# order of its smallest representative.
+_LD_NAMES_3 = """
+DD_0_0 DD_0_1 DD_0_2 DD_1_1 DD_1_2 DD_2_2
+Dz_0_0_0 Dz_0_0_1 Dz_0_0_2 Dz_0_1_1 Dz_0_1_2 Dz_0_2_2
+Dz_1_0_0 Dz_1_0_1 Dz_1_0_2 Dz_1_1_1 Dz_1_1_2 Dz_1_2_2
+Dz_2_0_0 Dz_2_0_1 Dz_2_0_2 Dz_2_1_1 Dz_2_1_2 Dz_2_2_2
+pi2_0_0_0_0 pi2_0_0_0_1 pi2_0_0_0_2 pi2_0_0_1_1 pi2_0_0_1_2 pi2_0_0_2_2
+pi2_0_1_0_1 pi2_0_1_0_2 pi2_0_1_1_1 pi2_0_1_1_2 pi2_0_1_2_2
+pi2_0_2_0_2 pi2_0_2_1_1 pi2_0_2_1_2 pi2_0_2_2_2
+pi2_1_1_1_1 pi2_1_1_1_2 pi2_1_1_2_2 pi2_1_2_1_2 pi2_1_2_2_2 pi2_2_2_2_2
+""".split()
+
+
+def _orbit_rule_names(P):
+    """The naming rule as orbits: a statistic is one name per orbit of its
+    symmetry group, named by the orbit's lexicographically smallest tuple."""
+    pairs = [(i, j) for i in range(P) for j in range(i, P)]
+    dd = sorted(pairs)
+    dz = sorted((i,) + jk for i in range(P) for jk in pairs)
+    pi2 = sorted({min(a, b) + max(a, b) for a in pairs for b in pairs})
+    return (["DD_%d_%d" % t for t in dd] + ["Dz_%d_%d_%d" % t for t in dz]
+            + ["pi2_%d_%d_%d_%d" % t for t in pi2])
+
+
+def _group_images(kind, t):
+    """Images of a population tuple under the statistic's symmetry group:
+    Dz_i_j_k is symmetric in (j, k); pi2_i_j_k_l in i <-> j, k <-> l and in
+    exchanging the (i, j) and (k, l) pairs (moments averages over the group;
+    DD_i_j is taken as one call because DD(i, j) = DD(j, i) itself)."""
+    if kind == "DD":
+        return [t]
+    if kind == "Dz":
+        i, j, k = t
+        return [(i, j, k), (i, k, j)]
+    i, j, k, l = t
+    out = []
+    for a in ((i, j), (j, i)):
+        for b in ((k, l), (l, k)):
+            out += [a + b, b + a]
+    return out
+
+
+def test_ld_names_literal_and_orbit_rule():
+    assert O.ld_names(3) == _LD_NAMES_3
+    for P in (1, 2, 3, 4):
+        assert O.ld_names(P) == _orbit_rule_names(P)
+        assert O.het_names(P) == ["H_%d_%d" % (i, j) for i in range(P) for j in range(i, P)]
+    # 4 populations: 10 DD + 40 Dz + 55 pi2 (55 = unordered pairs of the 10 unordered pairs)
+    n4 = O.ld_names(4)
+    assert [sum(x.startswith(k) for x in n4) for k in ("DD", "Dz", "pi2")] == [10, 40, 55]
+    assert len(set(n4)) == 105
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_stat_specs_are_group_averages(P):
+    """Each spec is the uniform average of its statistic over the symmetry
+    group images of the name's tuple (ld_pipeline.py:388-457 restates moments'
+    averaging); weights are accumulated per distinct call so that repeated
+    images (e.g. pi2_0_1_0_1) compare as their total weight."""
+    kinds = {"DD": "dd", "Dz": "dz", "pi2": "pi2"}
+    for name, terms in O.stat_specs(P):
+        kind, *rest = name.split("_")
+        t = tuple(int(x) for x in rest)
+        imgs = _group_images(kind, t)
+        want = {}
+        for im in imgs:
+            want[(kinds[kind], im)] = want.get((kinds[kind], im), 0.0) + 1.0 / len(imgs)
+        got = {}
+        for w, k, p in terms:
+            got[(k, p)] = got.get((k, p), 0.0) + w
+        assert got.keys() == want.keys(), name
+        for key in got:
+            assert got[key] == pytest.approx(want[key], abs=1e-15), (name, key)
+    # DD between populations is symmetric, so one call stands for the orbit
+    rng = np.random.default_rng(3)
+    H = [O.HapPop(rng.integers(0, 9, size=(50, 4))) for _ in range(2)]
+    np.testing.assert_array_equal(O.hap_dd(H, 0, 1), O.hap_dd(H, 1, 0))
+
+
+def test_union_biallelic_keep_geno_known_answers():
+    """moments_ld.py:256-285: over the union of the populations' individuals
+    (each counted once), keep a site when 0 < alt < 2 n_valid and
+    n_valid >= 2."""
+    g = np.array([
+        # s0 all ref, s1 all alt, s2 one valid, s3 poly, s4 poly only via pop B,
+        # s5 alt only in a duplicated row, s6 all missing, s7 het everywhere
+        [0, 2, -1, 0, 0, 0, -1, 1],   # row 0: pop A
+        [0, 2, -1, 1, 0, 0, -1, 1],   # row 1: pop A and pop B (duplicate)
+        [0, 2, 1, 2, 0, 0, -1, 1],    # row 2: pop A
+        [0, 2, -1, 0, 1, 0, -1, 1],   # row 3: pop B
+        [0, 2, -1, 0, 0, 2, -1, 1],   # row 4: in no population
+    ], dtype=np.int8)
+    keep = O.union_biallelic_keep_geno(g, [[0, 1, 2], [1, 3]])
+    np.testing.assert_array_equal(keep, [3, 4, 7])
+    # a duplicate row is counted once: alt 2 of n_valid 1 -> dropped
+    g2 = np.array([[2, 0], [-1, 1], [-1, 0]], dtype=np.int8)
+    np.testing.assert_array_equal(O.union_biallelic_keep_geno(g2, [[0, 1], [0, 2]]), [1])
+    # two valid individuals, one het: 0 < 1 < 4 -> kept
+    g3 = np.array([[1], [0], [-1]], dtype=np.int8)
+    np.testing.assert_array_equal(O.union_biallelic_keep_geno(g3, [[0], [1, 2]]), [0])
+
+
+def _allele_slots(col, use_genotypes):
+    """The alleles a site contributes: each valid individual's dosage g as g
+    alternative and 2 - g reference alleles; a haplotype row as one allele,
+    with missing (-1) counted as reference over the constant |pop| of
+    moments_ld.py:366-368."""
+    out = []
+    for x in col:
+        if use_genotypes:
+            if x >= 0:
+                out += [1] * int(x) + [0] * (2 - int(x))
+        else:
+            out.append(1 if x == 1 else 0)
+    return out
+
+
+@pytest.mark.parametrize("use_genotypes", [False, True])
+def test_heterozygosity_bruteforce_allele_pairs(use_genotypes):
+    """H_i_i is the probability that two distinct alleles drawn from
+    population i differ, H_i_j that an allele of i and one of j differ, both
+    summed over sites (moments_ld.py:346-388); enumerated pair by pair."""
+    rng = np.random.default_rng(44 + use_genotypes)
+    hi = 3 if use_genotypes else 2
+    mats = [rng.integers(-1, hi, size=(n, 6)).astype(np.int8) for n in (3, 4, 5)]
+    for M in mats:
+        M[0, :] = np.maximum(M[0, :], 0)
+        M[1, :] = np.maximum(M[1, :], 0)     # at least two valid rows per site
+    got = O.heterozygosity(mats, use_genotypes=use_genotypes)
+    for a in range(3):
+        for b in range(a, 3):
+            tot = 0.0
+            for s in range(6):
+                A = _allele_slots(mats[a][:, s], use_genotypes)
+                B = _allele_slots(mats[b][:, s], use_genotypes)
+                if a == b:
+                    diff = sum(A[x] != A[y] for x in range(len(A)) for y in range(len(A)) if x != y)
+                    tot += diff / (len(A) * (len(A) - 1))
+                else:
+                    tot += sum(x != y for x in A for y in B) / (len(A) * len(B))
+            assert got[f"H_{a}_{b}"] == pytest.approx(tot, rel=1e-13, abs=1e-15), (a, b)
 
 
 # ---------------------------------------------------------------------------
