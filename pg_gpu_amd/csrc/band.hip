@@ -101,6 +101,10 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   using PL = Planes<FAMILY, MISSING>;
   using SS = StatSet<FAMILY, P, SET>;
   constexpr int NC = PL::NC;
+  // haplotype within-population sets evaluate the moments straight from the
+  // accumulators (hap_self_stats); their fast-path rows are unrolled
+  constexpr bool HAP_SELF = FAMILY == PGLD_FAMILY_HAP && (SET == PGLD_SET_SELF || P == 1);
+  constexpr bool FAST_UNROLL = false;
   constexpr int NS = SS::N + 1;  // statistics + pair count
   // smem layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
@@ -215,15 +219,23 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       }
     };
     if (fast_key >= 0) {
-#pragma unroll 1
-      for (int r = 0; r < 16; ++r) {
+      // every lane holds 16 valid pairs of one key
+      auto row = [&](int r) {
         const int64_t i = (int64_t)tile.i0 + wr + acc_row(r, lane);
         double out[M];
         eval(r, i, out);
 #pragma unroll
         for (int s = 0; s < M; ++s) lacc[s] += out[s];
-        lacc[M] += 1.0;
+      };
+      if constexpr (FAST_UNROLL) {
+        // straight-line: accumulator elements addressed statically
+#pragma unroll
+        for (int r = 0; r < 16; ++r) row(r);
+      } else {
+#pragma unroll 1
+        for (int r = 0; r < 16; ++r) row(r);
       }
+      lacc[M] = 16.0;
       ck = fast_key + key_base;
       flush(true);
       return;
@@ -286,10 +298,16 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       const int kmode = tile.n_slots > 255 ? 0 : (p == 0 ? 1 : 2);
       if (wave_live)
         epilogue(std::integral_constant<int, S1::N>{}, p, P, p == 0, kmode, [&](int r, int64_t i, double* out) {
-          int c[NC];
-          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, i, j, pa.n_samples[p], c);
-          typename PopVal<FAMILY>::T V = PopVal<FAMILY>::make(c);
-          S1::eval(&V, out);
+          if constexpr (HAP_SELF) {
+            double n, a, b, s;
+            hap_moments<MISSING>(acc, r, pa.marg[p], i, j, pa.n_samples[p], n, a, b, s);
+            hap_self_stats(n, a, b, s, out);
+          } else {
+            int c[NC];
+            elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, i, j, pa.n_samples[p], c);
+            typename PopVal<FAMILY>::T V = PopVal<FAMILY>::make(c);
+            S1::eval(&V, out);
+          }
         });
     }
   } else {
@@ -313,18 +331,24 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     }
     if (wave_live)
       epilogue(std::integral_constant<int, SS::N>{}, 0, 1, true, 0, [&](int r, int64_t i, double* out) {
-        typename PopVal<FAMILY>::T V[P];
+        if constexpr (HAP_SELF && P == 1) {
+          double n, a, b, s;
+          hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
+          hap_self_stats(n, a, b, s, out);
+        } else {
+          typename PopVal<FAMILY>::T V[P];
 #pragma unroll
-        for (int p = 0; p < P - 1; ++p) {
+          for (int p = 0; p < P - 1; ++p) {
+            int c[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
+            V[p] = PopVal<FAMILY>::make(c);
+          }
           int c[NC];
-#pragma unroll
-          for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
-          V[p] = PopVal<FAMILY>::make(c);
+          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
+          V[P - 1] = PopVal<FAMILY>::make(c);
+          SS::eval(V, out);
         }
-        int c[NC];
-        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
-        V[P - 1] = PopVal<FAMILY>::make(c);
-        SS::eval(V, out);
       });
   }
   __syncthreads();
@@ -1282,9 +1306,10 @@ static int ensure_partials(pgld_band_plan* pl, int64_t ns_total) {
 // region batches read their populations from the bound region table.  With
 // allow_alloc false (region batches, which may be captured into a graph) the
 // workspace must already be large enough.
+// A caller-supplied workspace (work, work_cap doubles) replaces the plan's.
 static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const PopArgs& pa, int32_t with_missing,
                     int32_t stat_set, double* sums, double* key_pairs, hipStream_t s, bool allow_alloc,
-                    bool fp4 = false) {
+                    bool fp4 = false, double* work = nullptr, size_t work_cap = 0) {
   const bool miss = with_missing != 0;
   SumsLauncher L = launcher_for(family, n_pops, miss, stat_set, fp4);
   if (!L) {
@@ -1300,10 +1325,19 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
               n_stats);
     return PGLD_EUNSUPPORTED;
   }
-  if (allow_alloc) {
+  const size_t need = (size_t)(std::max<int64_t>(1, pl->n_slots) + pl->n_red_chunks) * ns_total;
+  double* partials = pl->d_partials;
+  if (work) {
+    if (work_cap < need) {
+      set_error("band sums: the plan needs %zu workspace doubles, the fixed workspace holds %zu", need, work_cap);
+      return PGLD_ECAPACITY;
+    }
+    partials = work;
+  } else if (allow_alloc) {
     int rc = ensure_partials(pl, ns_total);
     if (rc) return rc;
-  } else if (pl->partials_cap < (size_t)(std::max<int64_t>(1, pl->n_slots) + pl->n_red_chunks) * ns_total) {
+    partials = pl->d_partials;
+  } else if (pl->partials_cap < need) {
     set_error("band sums: workspace not bound for this statistic set");
     return PGLD_EINVAL;
   }
@@ -1321,7 +1355,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
   ba.width = pl->width;
   ba.key_mode = pl->key_mode;
   ba.n_keys = pl->n_keys_local;
-  ba.partials = pl->d_partials;
+  ba.partials = partials;
   ba.order = pl->n_regions == 0 ? pl->d_sums_order : nullptr;
   if (pl->n_tiles > 0) {
     if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "band sums event");
@@ -1331,8 +1365,8 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
     if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[1], s), "band sums event");
   }
   if (pl->n_red_chunks > 0) {
-    double* part2 = pl->d_partials + (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)pl->n_red_chunks), dim3(256), 0, s, pl->d_partials,
+    double* part2 = partials + (size_t)std::max<int64_t>(1, pl->n_slots) * ns_total;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)pl->n_red_chunks), dim3(256), 0, s, partials,
                        pl->d_key_list, pl->d_red_lo, ns_total, part2);
     int rc = check_launch("k_reduce_chunks");
     if (rc) return rc;
@@ -1340,7 +1374,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
                        (const int64_t*)nullptr, ns_total, n_stats, sums, key_pairs);
     return check_launch("k_band_reduce");
   }
-  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)pl->n_keys), dim3(256), 0, s, pl->d_partials, pl->d_key_off,
+  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)pl->n_keys), dim3(256), 0, s, partials, pl->d_key_off,
                      pl->d_key_list, ns_total, n_stats, sums, key_pairs);
   return check_launch("k_band_reduce");
 }

@@ -346,6 +346,50 @@ __device__ __forceinline__ void elem_counts(const ACC& acc, int r,
   }
 }
 
+// Centred haplotype moments of accumulator element r: n = vv, Sa = 2 hv - vv,
+// Sb = 2 vh - vv, Sab = 4 hh - 2 hv - 2 vh + vv (a' = 2a - 1 over the
+// pair-valid samples).  FP4 accumulators are floats holding integers below
+// 2^24, and these combinations stay below 2^24 in magnitude, so the float
+// arithmetic is exact; int8 accumulators use int32.
+template <bool MISSING, class ACC>
+__device__ __forceinline__ void hap_moments(const ACC& acc, int r, const int32_t* __restrict__ marg_, int64_t i,
+                                            int64_t j, int32_t S, double& n, double& a, double& b, double& s) {
+  const GLOBAL_AS int32_t* marg = (const GLOBAL_AS int32_t*)marg_;
+  if constexpr (std::is_same<typename ACC::T, v16f>::value) {
+    const float hh = acc.c[0][0][r];
+    float hv, vh, vv;
+    if constexpr (MISSING) {
+      hv = acc.c[0][1][r];
+      vh = acc.c[1][0][r];
+      vv = acc.c[1][1][r];
+    } else {
+      hv = (float)marg[i];
+      vh = (float)marg[j];
+      vv = (float)S;
+    }
+    n = vv;
+    a = fmaf(2.0f, hv, -vv);
+    b = fmaf(2.0f, vh, -vv);
+    s = fmaf(4.0f, hh, fmaf(-2.0f, hv + vh, vv));
+  } else {
+    const int hh = acc.c[0][0][r];
+    int hv, vh, vv;
+    if constexpr (MISSING) {
+      hv = acc.c[0][1][r];
+      vh = acc.c[1][0][r];
+      vv = acc.c[1][1][r];
+    } else {
+      hv = marg[i];
+      vh = marg[j];
+      vv = S;
+    }
+    n = vv;
+    a = 2 * hv - vv;
+    b = 2 * vh - vv;
+    s = 4 * hh - 2 * (hv + vh) + vv;
+  }
+}
+
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
 // Wave-wide totals of NV per-lane values, in groups of CH = min(CHMAX, NV
