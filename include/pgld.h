@@ -205,6 +205,21 @@ int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pop
                    const int32_t* n_samples, int32_t with_missing, int32_t stat_set, double* sums,
                    double* key_pairs, void* stream);
 
+/* pgld_band_sums_fp4 without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial-row workspace is a fixed 32 MiB device array (2^22 doubles), one
+ * copy per GPU held until the process ends, instead of the plan's own buffer;
+ * plans needing more ((n_slots + reduction chunks) x (n_stats + 1) doubles)
+ * return PGLD_ECAPACITY.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race.  Nothing in the
+ * array needs clearing between calls (every used element is written before
+ * it is read).  Same arguments and results as pgld_band_sums_fp4. */
+int pgld_band_sums_fp4_noalloc(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
+                               const int8_t* const* planes, const int32_t* const* marg,
+                               const int64_t* k_pad, const int32_t* n_samples, int32_t with_missing,
+                               int32_t stat_set, double* sums, double* key_pairs, void* stream);
+
 /* Counts of every band pair through the MFMA path, written per pair in the
  * plan's pair order (i ascending, then j): counts (int32, n_pairs x 4 or x 9)
  * for ONE population.  Used to check the tiled counts bit-exactly against

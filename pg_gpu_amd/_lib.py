@@ -71,6 +71,8 @@ _SIGNATURES = {
                        C.c_int),
     "pgld_band_sums_fp4": ([_vp, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32), _i32, _i32, _vp, _vp,
                             _vp], C.c_int),
+    "pgld_band_sums_fp4_noalloc": ([_vp, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32), _i32, _i32, _vp,
+                                    _vp, _vp], C.c_int),
     "pgld_band_counts": ([_vp, _i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp], C.c_int),
 }
 

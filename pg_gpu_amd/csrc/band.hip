@@ -1415,6 +1415,67 @@ int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pop
                   as_stream(stream), true, true);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free fused sums: the partial-row workspace is a fixed device
+// array (one copy per GPU, resident until the process ends) instead of the
+// plan's hipMalloc'd buffer.  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace pgld {
+constexpr size_t NOALLOC_WORK_DOUBLES = (size_t)1 << 22;  // 32 MiB
+static __device__ __align__(256) double scratch[NOALLOC_WORK_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+static double* g_scratch_ptr[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of this GPU's copy of the array, looked up once per device
+// with that device current (the array's name is a host-side stand-in).
+static int scratch_on_current_device(double** out) {
+  int dev = 0;
+  PGLD_TRY_HIP(hipGetDevice(&dev), "pgld_band_sums_fp4_noalloc: hipGetDevice");
+  PGLD_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "pgld_band_sums_fp4_noalloc: device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    PGLD_TRY_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "pgld_band_sums_fp4_noalloc: workspace address");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return PGLD_OK;
+}
+}  // namespace pgld
+
+extern "C" {
+
+int pgld_band_sums_fp4_noalloc(const pgld_band_plan* plan, int32_t family, int32_t n_pops,
+                               const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
+                               const int32_t* n_samples, int32_t with_missing, int32_t stat_set, double* sums,
+                               double* key_pairs, void* stream) {
+  PGLD_REQUIRE(plan && planes && marg && k_pad && n_samples && sums, "pgld_band_sums_fp4_noalloc: null pointer");
+  PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_sums_fp4_noalloc: single bands only");
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_band_sums_fp4_noalloc: bad family");
+  PGLD_REQUIRE(n_pops >= 1 && n_pops <= 4, "pgld_band_sums_fp4_noalloc: n_pops must be 1..4");
+  PGLD_REQUIRE(stat_set == PGLD_SET_MOMENTS || stat_set == PGLD_SET_SELF, "pgld_band_sums_fp4_noalloc: bad stat_set");
+  for (int p = 0; p < n_pops; ++p)
+    PGLD_REQUIRE(k_pad[p] % 256 == 0, "pgld_band_sums_fp4_noalloc: k_pad must be a multiple of 256 (FP4 planes)");
+  PopArgs pa;
+  int rc = fill_pop_args(n_pops, planes, marg, k_pad, n_samples, pa);
+  if (rc) return rc;
+  const int64_t ns_total = n_stats_of(family, n_pops, stat_set) + 1;
+  const size_t need = (size_t)(std::max<int64_t>(1, plan->n_slots) + plan->n_red_chunks) * ns_total;
+  if (need > NOALLOC_WORK_DOUBLES) {
+    set_error("pgld_band_sums_fp4_noalloc: the plan needs %zu workspace doubles, more than the fixed %zu "
+              "(use pgld_band_sums_fp4)", need, NOALLOC_WORK_DOUBLES);
+    return PGLD_ECAPACITY;
+  }
+  double* work = nullptr;
+  rc = scratch_on_current_device(&work);
+  if (rc) return rc;
+  return run_sums(const_cast<pgld_band_plan*>(plan), family, n_pops, pa, with_missing, stat_set, sums, key_pairs,
+                  as_stream(stream), false, true, work, NOALLOC_WORK_DOUBLES);
+}
+
 static int batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int32_t stat_set,
                       const int8_t* const* planes, const int32_t* const* marg, const int64_t* k_pad,
                       const int32_t* n_samples, bool fp4) {
