@@ -1,20 +1,23 @@
 """Benchmark of the LD pair-statistics path on one MI355X.
 
-Default workload (the headline): batched windowed haplotype LD -- 100,000 SNPs
-x 2,000 haplotypes (two populations of 1,000), 2% MCAR missing, heterogeneous
-positions over 10 Mb; all pairs within 100 kb; the 15 two-population moments
-statistics summed in FP64 per 100 kb window (keyed by the left SNP).  One step
-= pack both populations' indicator planes from the resident int8 matrix + one
-fused tiled int8-MFMA band pass + the per-window reduction.
+Default workload (the headline, the largest single-GPU configuration):
+ragged_regions -- 512 independent regions of 500-20,000 SNPs x 50-5,000
+haplotypes in 4 populations, 3% MCAR missing, positions uniform over 1 Mb,
+all pairs of each region, per-region 16 log-bin DD/Dz/pi2 sums of every
+population (FP64).  One step = a HIP graph that repacks every region's
+population planes from the resident int8 matrices and runs one autotuned
+persistent fused launch + the per-key reduction.  The line also carries the
+error of every statistic against the FP64 oracle on the regions of least
+work (all their pairs).
 
-Other workloads (--workload): genotype_bins (50,000 SNPs x 1,000 diploids, 5%
-missing, all pairs in 1 Mb, 16 log distance bins), plain_hap (2,000 SNPs x
-200 haplotypes, per-pair r^2/DD/Dz/pi2 within 20 kb through the plain
-per-pair kernels), dense_r2 (40,000 SNPs x 10,000 haplotypes, r^2 of all
-7.9998e8 pairs, 100-bin histogram + sparse list of pairs with r^2 > 0.2),
-ragged_regions (512 independent regions of 500-20,000 SNPs x 50-5,000
-haplotypes in 4 populations, 3% missing, all pairs, per-region 16-bin
-DD/Dz/pi2 sums per population in one graph-captured persistent launch).
+Other workloads (--workload): windowed_hap (100,000 SNPs x 2,000 haplotypes
+in two populations, 2% missing, pairs within 100 kb, the 15 two-population
+moments statistics per 100 kb window), genotype_bins (50,000 SNPs x 1,000
+diploids, 5% missing, all pairs in 1 Mb, 16 log distance bins), plain_hap
+(2,000 SNPs x 200 haplotypes, per-pair r^2/DD/Dz/pi2 within 20 kb through the
+plain per-pair kernels), dense_r2 (40,000 SNPs x 10,000 haplotypes, r^2 of
+all 7.9998e8 pairs, 100-bin histogram + sparse list of pairs with
+r^2 > 0.2).
 
 Prints one JSON line: value is Gpair*samples/s for the whole job; roofline is
 for the dominant kernel; cpu_baseline is the NumPy oracle on a bounded sample
@@ -49,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="windowed_hap",
+    ap.add_argument("--workload", default="ragged_regions",
                     choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2", "ragged_regions"])
     ap.add_argument("--n-var", type=int, default=None)
     ap.add_argument("--n-samples", "--n-hap", dest="n_samples", type=int, default=None)
@@ -57,6 +60,8 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the FP64-oracle error check of the ragged batch")
+    ap.add_argument("--oracle-regions", type=int, default=4)
     ap.add_argument("--planes", default="fp4", choices=["int8", "fp4"],
                     help="operand planes of the fused sums (windowed and genotype workloads): int8 MFMA, or FP4 "
                          "(e2m1 0 / 1) on the FP4 MFMA")
@@ -550,6 +555,14 @@ def run_ragged_regions(args, torch):
         log(f"[bench] region-by-region launches {plain_ms:.1f} ms, max |batch - per-region| {check:.3e}")
         del plans
 
+    # error of every statistic against the FP64 oracle, in full, on the
+    # regions of least work (all their pairs, every bin)
+    oracle_err = None
+    if not args.no_oracle:
+        oracle_err = ragged_oracle_error(datas, pos, rows, S, edges, res.cpu().numpy(), n_check=args.oracle_regions)
+        log(f"[bench] oracle check on regions {oracle_err['regions']}: max rel err {oracle_err['max_rel_err']:.2e}, "
+            f"max scaled err {oracle_err['max_scaled_err']:.2e}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import ld_oracle as O
@@ -568,7 +581,8 @@ def run_ragged_regions(args, torch):
         cpu_s = time.perf_counter() - t2
         cpu = {"value": n_ps / cpu_s / 1e9, "unit": "Gpair*samples/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_pairs or 300_000} random pairs from each of regions 0-3, 4-population counts + "
-                         f"12 self statistics + bin sums, NumPy oracle (single thread), {cpu_s:.1f} s"}
+                         f"12 self statistics + bin sums, NumPy oracle (single thread), {cpu_s:.1f} s",
+               "host": host_cpu()}
         log(f"[bench] cpu oracle {cpu['value']:.4f} Gpair*samples/s over {cpu_s:.1f} s")
     line = base_line(args, value, ms_step,
                      f"ragged persistent batch: {n_regions} independent regions, SNPs uniform 500-20,000, "
@@ -580,8 +594,64 @@ def run_ragged_regions(args, torch):
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "plain": "one band plan and one fused launch per region", "batch_vs_plain_max_abs": check,
-                 "cpu_baseline": cpu})
+                 "plan_bind_s": plan_s, "oracle_error": oracle_err, "cpu_baseline": cpu})
     return line
+
+
+RAGGED_STATS = [f"{t}_{p}" for t in ("DD", "Dz", "pi2") for p in range(4)]
+
+
+def ragged_oracle_error(datas, pos, rows, S, edges, res, n_check=4):
+    """Every statistic of the ragged batch against the FP64 NumPy oracle on
+    the n_check regions of least work (all pairs, all 16 bins): per
+    statistic the largest |gpu - oracle| / |oracle| over bins with
+    |oracle| > 1e-6 Sum|terms|, and the largest |gpu - oracle| / Sum|terms|
+    (the scale of a sum that cancels)."""
+    from oracle import ld_oracle as O
+    specs = [(t, [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(4)]
+    work = np.array([len(p) * (len(p) - 1) // 2 * int(s) for p, s in zip(pos, S)])
+    pick = [int(r) for r in np.argsort(work, kind="stable")[:n_check]]
+    rel = np.zeros(len(specs))
+    scaled = np.zeros(len(specs))
+    n_pairs = 0
+    for r in pick:
+        h = datas[r].cpu().numpy()
+        ii, jj = O.pairs_within_distance(pos[r], np.inf)
+        n_pairs += len(ii)
+        key = O.bin_index(pos[r][jj] - pos[r][ii], edges)
+        want = np.zeros((len(edges) - 1, len(specs)))
+        absw = np.zeros_like(want)
+        for c0 in range(0, len(ii), 50_000):
+            a, b = ii[c0:c0 + 50_000], jj[c0:c0 + 50_000]
+            cl = [O.hap_counts(h, a, b, rw) for rw in rows[r]]
+            st = O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl], specs)
+            want += O.keyed_sums(key[c0:c0 + 50_000], st, len(edges) - 1)
+            absw += O.keyed_sums(key[c0:c0 + 50_000], np.abs(st), len(edges) - 1)
+        # the batch stores column s * P + p = statistic s of population p, as the specs above
+        got = res[r]
+        err = np.abs(got - want)
+        ok = np.abs(want) > 1e-6 * absw
+        rel = np.maximum(rel, np.max(np.where(ok, err / np.where(ok, np.abs(want), 1.0), 0.0), axis=0))
+        scaled = np.maximum(scaled, np.max(err / np.maximum(absw, 1e-300), axis=0))
+    return {"regions": pick, "pairs": int(n_pairs), "oracle": "NumPy FP64 (oracle/ld_oracle.py), all pairs",
+            "max_rel_err": float(rel.max()), "max_scaled_err": float(scaled.max()),
+            "max_rel_err_per_stat": dict(zip(RAGGED_STATS, map(float, rel))),
+            "max_scaled_err_per_stat": dict(zip(RAGGED_STATS, map(float, scaled)))}
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def main():

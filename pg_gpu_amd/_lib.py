@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpgld.so")
 
 FAMILY_HAP, FAMILY_GENO = 0, 1
+EINVAL, EDOMAIN, ECAPACITY, EUNSUPPORTED = -1, -2, -3, -4
 STAT_DD, STAT_DZ, STAT_PI2, STAT_PI2_SYM, STAT_SIGMA_D2 = 0, 1, 2, 3, 4
 KEY_BIN_ABS, KEY_BIN, KEY_WINDOW = 0, 1, 2
 EST_R2, EST_R2_NAIVE, EST_SIGMA_D2, EST_RH_R, EST_DIP_R2 = 0, 1, 2, 3, 4
@@ -80,7 +81,11 @@ _lib = None
 
 
 class PgldError(RuntimeError):
-    """A libpgld call returned a nonzero status."""
+    """A libpgld call returned a nonzero status (kept in .status)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def lib():
@@ -106,7 +111,7 @@ def exported_symbols():
 def check(rc, name):
     if rc != 0:
         msg = lib().pgld_last_error().decode(errors="replace")
-        raise PgldError(f"{name} failed with status {rc}: {msg}")
+        raise PgldError(f"{name} failed with status {rc}: {msg}", rc)
 
 
 def call(name, *args):

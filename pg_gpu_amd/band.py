@@ -280,3 +280,58 @@ class BandPlan:
                L.ptr(pop.marg), pop.k_pad, pop.n_samples, int(bool(with_missing)), None, L.ptr(out),
                L.stream_handle())
         return out
+
+
+# ---------------------------------------------------------------------------
+# Fused band pass where a fused kernel exists, the per-pair path otherwise
+# ---------------------------------------------------------------------------
+
+FUSED_MAX_SAMPLES = 32767  # int16 count stash and 24-bit products of the fused kernels
+
+
+def fused_supported(family, n_pops, stat_set, pop_sizes):
+    """Whether the fused band kernels cover this statistic set: haplotype
+    moments sets for 1-3 populations, within-population sets for 1-4;
+    genotype sets for 1-2; at most FUSED_MAX_SAMPLES samples per population."""
+    if max(pop_sizes) > FUSED_MAX_SAMPLES:
+        return False
+    if family == "hap":
+        return n_pops <= (4 if stat_set == "self" else 3)
+    return n_pops <= 2
+
+
+def ld_sums(data, family, positions, rows, max_dist, *, edges=None, window=None, n_keys=None, origin=0.0,
+            key_coord=None, abs_dist=False, stat_set="moments", method="auto"):
+    """Per-key FP64 sums of the LD statistics of every pair within max_dist
+    (keys: distance bins ``edges`` of ``key_coord`` or the positions, or
+    windows of the left SNP).  method "auto" runs the fused band pass and
+    falls back to the per-pair path (plain counts -> statistics -> keyed
+    sums, as moments_ld.py:294-343) when no fused kernel covers the set, a
+    population exceeds the fused kernels' sample limit, or a tile's keys x
+    statistics exceed the fused kernel's LDS; "tiled" / "plain" force one.
+    Returns (sums (n_keys, n_stats), pairs per key) as device tensors."""
+    pos = np.ascontiguousarray(L.to_numpy(positions), dtype=np.float64)
+    n_rows = data.shape[0]
+    sizes = [n_rows if r is None else len(r) for r in rows]
+    P = len(rows)
+    if window is not None:
+        mode = L.KEY_WINDOW
+        if n_keys is None:
+            n_keys = int(np.floor((pos[-1] - origin) / window)) + 1 if len(pos) else 1
+    else:
+        mode = L.KEY_BIN_ABS if abs_dist else L.KEY_BIN
+        n_keys = len(edges) - 1
+    if method == "tiled" or (method == "auto" and fused_supported(family, P, stat_set, sizes)):
+        if window is not None:
+            plan = BandPlan(pos, max_dist, window=window, n_keys=n_keys, origin=origin)
+        else:
+            plan = BandPlan(pos, max_dist, edges=edges, key_coord=key_coord, abs_dist=abs_dist)
+        packed = [pack_population(data, r, family, fp4=True) for r in rows]
+        try:
+            return plan.sums(packed, stat_set=stat_set, want_pairs=True)
+        except L.PgldError as e:
+            if method == "tiled" or e.status != L.EUNSUPPORTED:
+                raise
+    from . import ld_pipeline as LP
+    return LP.plain_keyed_sums(data, family, pos, rows, max_dist, mode, n_keys, edges=edges, origin=origin,
+                               width=1.0 if window is None else window, key_coord=key_coord, stat_set=stat_set)

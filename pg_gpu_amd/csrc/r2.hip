@@ -658,6 +658,8 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
   PGLD_REQUIRE(plan->n_regions == 0, "pgld_band_r2: not defined for region batches");
   PGLD_REQUIRE(k_pad % KSTEP == 0 && k_pad >= KSTEP && k_pad >= n_samples && n_samples >= 0,
                "pgld_band_r2: k_pad must be a positive multiple of 128 >= n_samples");
+  // the 24-bit count products of hap_pair_value need counts below 2^15
+  PGLD_REQUIRE(n_samples <= 32767, "pgld_band_r2: at most 32767 samples per population");
   PGLD_REQUIRE(!hist || n_hist >= 1, "pgld_band_r2: n_hist must be >= 1");
   PGLD_REQUIRE(!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2_MAX_HIST_LDS,
                "pgld_band_r2: %d keys per tile x %d histogram bins exceed LDS", plan->max_tile_slots, n_hist + 1);

@@ -206,7 +206,5 @@ class GenotypeMatrix:
             pops = list(self.sample_sets)
         rows = [self.sample_sets[p] if isinstance(p, str) else p for p in pops]
         bins = np.asarray(bp_bins, dtype=np.float64)
-        plan = B.BandPlan(self._pos_np(), float(bins[-1]), edges=bins)
-        packed = [B.pack_population(self._genotypes, r, "geno", fp4=True) for r in rows]
-        sums, kp = plan.sums(packed, want_pairs=True)
+        sums, kp = B.ld_sums(self._genotypes, "geno", self._pos_np(), rows, float(bins[-1]), edges=bins)
         return sums, kp, LP.ld_names(len(rows))

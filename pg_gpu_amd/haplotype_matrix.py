@@ -311,9 +311,8 @@ class HaplotypeMatrix:
         """Fused band pass keyed by distance bins: (sums (n_bins, S), pairs per bin)."""
         self.transfer_to_gpu()
         bins = np.asarray(bp_bins, dtype=np.float64)
-        plan = B.BandPlan(self._pos_np(), float(bins[-1]), edges=bins)
-        packed = [B.pack_population(self._haplotypes, r, "hap", fp4=True) for r in pops]
-        sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
+        sums, kp = B.ld_sums(self._haplotypes, "hap", self._pos_np(), pops, float(bins[-1]), edges=bins,
+                             stat_set=stat_set)
         return sums.cpu().numpy(), kp.cpu().numpy()
 
     def compute_ld_statistics_gpu_single_pop(self, bp_bins, raw=False, ac_filter=True, chunk_size="auto"):
@@ -475,9 +474,8 @@ class HaplotypeMatrix:
             pops = list(self.sample_sets)
         rows = [self.sample_sets[p] if isinstance(p, str) else p for p in pops]
         md = float(window if max_dist is None else max_dist)
-        plan = B.BandPlan(self._pos_np(), md, window=float(window), n_keys=n_windows, origin=origin)
-        packed = [B.pack_population(self._haplotypes, r, "hap", fp4=True) for r in rows]
-        sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
+        sums, kp = B.ld_sums(self._haplotypes, "hap", self._pos_np(), rows, md, window=float(window),
+                             n_keys=n_windows, origin=origin, stat_set=stat_set)
         P = len(rows)
         if stat_set == "moments" or P == 1:
             names = LP.ld_names(P)

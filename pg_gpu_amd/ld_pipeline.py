@@ -266,3 +266,55 @@ class PopData:
         self.qA = self.c3 + self.c4
         self.pB = self.c1 + self.c3
         self.qB = self.c2 + self.c4
+
+
+# ---------------------------------------------------------------------------
+# Keyed sums through the per-pair path (the structure of moments_ld.py:294-343)
+# ---------------------------------------------------------------------------
+
+def self_stat_specs(num_pops):
+    """The within-population set DD_p_p, Dz_p_p_p, pi2_p_p_p_p in the fused
+    kernel's column order (statistic s of population p at s * P + p)."""
+    if num_pops == 1:
+        return generate_stat_specs(1)
+    out = []
+    for kind, t, k in (("DD", "dd", 2), ("Dz", "dz", 3), ("pi2", "pi2", 4)):
+        for p in range(num_pops):
+            out.append(("_".join([kind] + [str(p)] * k), [(1.0, t, (p,) * k)]))
+    return out
+
+
+def plain_keyed_sums(data, family, positions, rows, max_dist, key_mode, n_keys, edges=None, origin=0.0, width=1.0,
+                     key_coord=None, stat_set="moments"):
+    """FP64 sums per key of the moments (or within-population) statistics of
+    every pair within max_dist, one pair chunk at a time: plain counts ->
+    per-pair statistics -> pair keys -> keyed sums.  Keys as the fused band
+    pass: a distance bin of key_coord (|difference| for KEY_BIN_ABS) or the
+    window of the left SNP.  Returns (sums (n_keys, n_stats), pairs per key)
+    as float64 device tensors."""
+    T = L.torch()
+    P = len(rows)
+    if stat_set == "self":
+        specs = self_stat_specs(P)
+    else:
+        specs = generate_stat_specs(P)
+    pi2_form = "pi2_sym" if (family == "hap" and P == 2 and stat_set != "self") else "pi2"
+    n_stats = len(specs)
+    pos = np.asarray(L.to_numpy(positions), dtype=np.float64)
+    coord = L.to_device(pos if key_coord is None else key_coord, np.float64)
+    ed = L.to_device(np.zeros(1) if edges is None else edges, np.float64)
+    acc = L.zeros((n_keys, n_stats + 1), np.float64)
+    counts_fn = compute_counts_for_pairs if family == "hap" else compute_genotype_counts_for_pairs
+    n_rows = data.shape[0]
+    max_s = max(n_rows if r is None else len(r) for r in rows)
+    chunk = estimate_ld_chunk_size(max_s, num_pops=P)
+    for ii, jj in iter_pairs_within_distance(pos, max_dist, chunk):
+        cl = [counts_fn(data, ii, jj, r) for r in rows]
+        st = stats_from_counts(family, [c for c, _ in cl], [nv for _, nv in cl], specs, pi2_form=pi2_form)
+        st = T.cat([st, T.ones((st.shape[0], 1), dtype=T.float64, device=st.device)], dim=1).contiguous()
+        keys = L.empty((ii.shape[0],), np.int32)
+        L.call("pgld_pair_keys", L.ptr(coord), L.ptr(ii), L.ptr(jj), ii.shape[0], key_mode, L.ptr(ed), n_keys,
+               float(origin), float(width), L.ptr(keys), L.stream_handle())
+        L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], n_stats + 1, n_keys, L.ptr(acc),
+               L.stream_handle())
+    return acc[:, :n_stats].contiguous(), acc[:, n_stats].contiguous()

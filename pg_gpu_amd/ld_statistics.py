@@ -208,16 +208,18 @@ def zns(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
     (ld_statistics.py:679-731).  For a HaplotypeMatrix the pairs are never
     materialised: one pass of the MFMA tiles yields per-SNP sums.
     estimator 'sigma_d2' (the 'auto' choice) averages the unbiased
-    D^2 / pi_2 over pairs with n >= 4; 'r2' averages the frequency r^2 over
-    all pairs.  A precomputed square r^2 array is averaged off the diagonal."""
+    D^2 / pi_2 over pairs with n >= 4 of all segregating sites (missing_data
+    does not apply: the reference maps this estimator to its internal
+    'project' mode, ld_statistics.py:721-726); 'r2' -- and 'rogers_huff',
+    which the reference routes to the same naive path for haplotypes --
+    averages the frequency r^2 over all pairs.  A precomputed square r^2
+    array is averaged off the diagonal."""
     from .genotype_matrix import GenotypeMatrix
     from .haplotype_matrix import HaplotypeMatrix
     is_hm = isinstance(r2_matrix_or_matrix, HaplotypeMatrix)
     est = _resolve_ld_estimator(estimator, is_hm)
     if is_hm:
-        if est == "rogers_huff":
-            raise NotImplementedError("the rogers_huff estimator is outside this build")
-        mat = _segregating(r2_matrix_or_matrix, missing_data)
+        mat = _segregating(r2_matrix_or_matrix, "include" if est == "sigma_d2" else missing_data)
         m = mat.num_variants
         if m < 2:
             return 0.0
@@ -273,14 +275,14 @@ def omega(r2_matrix_or_matrix, missing_data="include", estimator="auto"):
     """Kim and Nielsen's Omega: the largest ratio of mean LD within the two
     blocks of a split to mean LD across it, over the upper triangle
     (ld_statistics.py:761-862).  For a HaplotypeMatrix one pass of the MFMA
-    tiles yields the per-SNP sums the split ratios need."""
+    tiles yields the per-SNP sums the split ratios need; 'rogers_huff' on a
+    HaplotypeMatrix takes the frequency r^2 of pairwise_r2, as the reference
+    (_resolve_r2_matrix, ld_statistics.py:926-970)."""
     from .genotype_matrix import GenotypeMatrix
     from .haplotype_matrix import HaplotypeMatrix
     is_hm = isinstance(r2_matrix_or_matrix, HaplotypeMatrix)
     est = _resolve_ld_estimator(estimator, is_hm)
     if is_hm:
-        if est == "rogers_huff":
-            raise NotImplementedError("the rogers_huff estimator is outside this build")
         mat = _segregating(r2_matrix_or_matrix, missing_data)
         if mat.num_variants < 5:
             return 0.0

@@ -135,10 +135,6 @@ def _union_biallelic_keep(mat, pops):
     return np.nonzero((alt > 0) & (alt < 2 * valid) & (valid >= 2))[0]
 
 
-def _fused_supported(use_genotypes, P):
-    return P <= 2 if use_genotypes else P <= 3
-
-
 def _compute_ld_sums(mat, pops, bins, gen, max_bp, use_genotypes, method="auto"):
     P = len(pops)
     if use_genotypes:
@@ -152,41 +148,20 @@ def _compute_ld_sums(mat, pops, bins, gen, max_bp, use_genotypes, method="auto")
     pos = np.asarray(L.to_numpy(mat.positions), dtype=np.float64)
     rows = [mat.sample_sets[p] for p in pops]
     fam = "geno" if use_genotypes else "hap"
-    if method == "tiled" or (method == "auto" and _fused_supported(use_genotypes, P)):
-        plan = B.BandPlan(pos, max_bp, edges=bins, key_coord=gen, abs_dist=gen is not None)
-        packed = [B.pack_population(data, r, fam, fp4=True) for r in rows]
-        return plan.sums(packed).cpu().numpy()
-    return _plain_ld_sums(data, pos, rows, bins, gen, max_bp, use_genotypes)
+    if method == "plain":
+        return _plain_ld_sums(data, pos, rows, bins, gen, max_bp, use_genotypes)
+    # fused band pass, or the per-pair path where no fused kernel covers the set
+    sums, _ = B.ld_sums(data, fam, pos, rows, max_bp, edges=bins, key_coord=gen, abs_dist=gen is not None,
+                        method=method)
+    return sums.cpu().numpy()
 
 
 def _plain_ld_sums(data, pos, rows, bins, gen, max_bp, use_genotypes):
     """The per-pair path: pair chunks -> counts -> statistics -> keyed sums
     (the structure of moments_ld.py:294-343)."""
-    P = len(rows)
-    n_bins = len(bins) - 1
-    names = LP.ld_names(P)
-    specs = LP.generate_stat_specs(P)
-    sums = L.zeros((n_bins, len(names)), np.float64)
-    coord = L.to_device(gen if gen is not None else pos, np.float64)
-    edges = L.to_device(bins, np.float64)
     mode = L.KEY_BIN_ABS if gen is not None else L.KEY_BIN
-    max_s = max(len(r) for r in rows)
-    chunk = LP.estimate_ld_chunk_size(max_s, num_pops=P)
-    for ii, jj in LP.iter_pairs_within_distance(pos, max_bp, chunk):
-        if use_genotypes:
-            cl = [LP.compute_genotype_counts_for_pairs(data, ii, jj, r) for r in rows]
-            st = LP.stats_from_counts("geno", [c for c, _ in cl], [nv for _, nv in cl], specs)
-        elif P == 2:
-            cl = [LP.compute_counts_for_pairs(data, ii, jj, r) for r in rows]
-            st = LP.compute_two_pop_statistics_batch(cl[0][0], cl[1][0], cl[0][1], cl[1][1])
-        else:
-            cl = [LP.compute_counts_for_pairs(data, ii, jj, r) for r in rows]
-            st = LP.stats_from_counts("hap", [c for c, _ in cl], [nv for _, nv in cl], specs)
-        keys = L.empty((ii.shape[0],), np.int32)
-        L.call("pgld_pair_keys", L.ptr(coord), L.ptr(ii), L.ptr(jj), ii.shape[0], mode, L.ptr(edges), n_bins,
-               0.0, 1.0, L.ptr(keys), L.stream_handle())
-        L.call("pgld_keyed_sums", L.ptr(st), L.ptr(keys), ii.shape[0], len(names), n_bins, L.ptr(sums),
-               L.stream_handle())
+    sums, _ = LP.plain_keyed_sums(data, "geno" if use_genotypes else "hap", pos, rows, max_bp, mode, len(bins) - 1,
+                                  edges=np.asarray(bins, dtype=np.float64), key_coord=gen)
     return sums.cpu().numpy()
 
 

@@ -25,6 +25,8 @@ if not torch.cuda.is_available():  # pragma: no cover - CPU container
 from pg_gpu_amd import GenotypeMatrix, HaplotypeMatrix, moments_ld  # noqa: E402
 from pg_gpu_amd import ld_statistics_genotype as LSG  # noqa: E402
 from pg_gpu_amd import synthetic as SY  # noqa: E402
+from pg_gpu_amd import _lib as L  # noqa: E402
+from pg_gpu_amd import band  # noqa: E402
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 VCF = os.path.join(GOLD, "im-parsing-example.vcf")
@@ -97,6 +99,50 @@ def test_moments_hap_pop_counts(P, method):
     close_sums(np.array(res["sums"][:-1]), want, scale=scale)
     H = O.heterozygosity([h[sets[n]] for n in names])
     np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
+
+
+def test_moments_many_bins_fall_back_to_the_per_pair_path():
+    """Three populations and 2-bp bins: a 64-SNP tile touches more bins than
+    the fused kernel's LDS holds (PGLD_EUNSUPPORTED); moments_ld and the
+    HaplotypeMatrix band sums take the per-pair path instead and still
+    match the oracle."""
+    rng = np.random.default_rng(91)
+    n_var = 260
+    h = SY.neutral_haplotypes(60, n_var, rng, missing=0.03)
+    pos = np.arange(n_var, dtype=np.int64) * 2
+    names = ["a", "b", "c"]
+    sets = {n: list(range(20 * k, 20 * (k + 1))) for k, n in enumerate(names)}
+    bins = np.arange(0.0, 125.0, 2.0)   # 62 bins
+    rows = [sets[n] for n in names]
+    with pytest.raises(L.PgldError) as e:
+        band.ld_sums(h, "hap", pos, rows, float(bins[-1]), edges=bins, method="tiled")
+    assert e.value.status == L.EUNSUPPORTED
+    hm = HaplotypeMatrix(h, pos, sample_sets=sets)
+    res = moments_ld.compute_ld_statistics(haplotype_matrix=hm, pops=names, bp_bins=bins, report=False)
+    want = O.moments_ld_sums(h, pos, rows, bins)
+    scale = O.moments_ld_sums(h, pos, rows, bins, absolute=True)
+    close_sums(np.array(res["sums"][:-1]), want, scale=scale)
+    sums, kp = band.ld_sums(h, "hap", pos, rows, float(bins[-1]), edges=bins, stat_set="self")
+    ii, jj = O.pairs_within_distance(pos, float(bins[-1]))
+    key = O.bin_index(pos[jj] - pos[ii], bins)
+    nb = len(bins) - 1
+    np.testing.assert_array_equal(np_(kp), np.bincount(key[(key >= 0) & (key < nb)], minlength=nb))
+
+
+def test_population_above_the_fused_sample_limit_takes_the_per_pair_path():
+    """A population of more than 32,767 haplotypes is beyond the fused
+    kernels (int16 count stash); the band sums run per pair."""
+    rng = np.random.default_rng(92)
+    n_var = 40
+    h = SY.neutral_haplotypes(33000, n_var, rng, missing=0.01)
+    pos = SY.uniform_positions(n_var, 4000, rng)
+    bins = np.array([0.0, 300.0, 1000.0, 4000.0])
+    assert not band.fused_supported("hap", 1, "moments", [33000])
+    hm = HaplotypeMatrix(h, pos)
+    got = hm.compute_ld_statistics_gpu_single_pop(bins, raw=True, ac_filter=False)
+    want = O.moments_ld_sums(h, pos, [list(range(33000))], bins)
+    scale = O.moments_ld_sums(h, pos, [list(range(33000))], bins, absolute=True)
+    close_sums(np.array([got[(float(bins[k]), float(bins[k + 1]))] for k in range(3)]), want, scale=scale)
 
 
 @pytest.mark.parametrize("use_genotypes", [False, True])
@@ -290,9 +336,20 @@ def test_zns_omega_pairwise_r2(missing):
     assert LS.omega(naive) == pytest.approx(O.omega_from_matrix(naive), rel=1e-12)
     full = np_(HaplotypeMatrix(h, np.arange(h.shape[1]) * 100).pairwise_r2())
     np.testing.assert_array_equal(full, O.naive_r2_matrix(h))
-    if missing == 0.0:
-        ex = HaplotypeMatrix(h, np.arange(h.shape[1]) * 100)
-        assert LS.zns(ex, missing_data="exclude") == pytest.approx(O.zns_from_matrix(sig, valid), rel=1e-10)
+    # sigma_d2 ZnS keeps every segregating site whatever missing_data says (the
+    # reference's internal 'project' mode, ld_statistics.py:721-726); the r2
+    # ZnS and Omega honour 'exclude'; 'rogers_huff' on a HaplotypeMatrix is the
+    # frequency r^2 path (ld_statistics.py:721-731, :800-808)
+    ex = HaplotypeMatrix(h, np.arange(h.shape[1]) * 100)
+    assert LS.zns(ex, missing_data="exclude") == pytest.approx(O.zns_from_matrix(sig, valid), rel=1e-10)
+    assert LS.zns(hm, estimator="rogers_huff") == pytest.approx(O.zns_from_matrix(naive), rel=1e-10)
+    assert LS.omega(hm, estimator="rogers_huff") == pytest.approx(O.omega_from_matrix(naive), rel=1e-10)
+    hx = h[:, np.nonzero((h >= 0).all(axis=0))[0]]
+    hx = hx[:, O.segregating_sites(hx)]
+    sigx, _ = O.sigma_d2_matrix(hx)
+    assert LS.zns(hm, estimator="r2", missing_data="exclude") == pytest.approx(
+        O.zns_from_matrix(O.naive_r2_matrix(hx)), rel=1e-10, abs=1e-300)
+    assert LS.omega(hm, missing_data="exclude") == pytest.approx(O.omega_from_matrix(sigx), rel=1e-10, abs=1e-300)
 
 
 
