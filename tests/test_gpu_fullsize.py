@@ -176,3 +176,28 @@ def test_plain_haplotype_panel_full_size_vs_oracle():
     plan = band.BandPlan(pos, 20_000.0, window=1e9, n_keys=1)
     mc = np_(plan.counts(band.pack_population(hap_d, None, "hap")))
     np.testing.assert_array_equal(mc, want_c)
+
+
+# ---------------------------------------------------------------------------
+def test_windowed_panel_fp4_equals_int8_full_size(windowed_panel):
+    """The FP4 planes the benchmark runs on give the int8 planes' window sums
+    bit for bit on the whole 100,000-SNP x 2,000-haplotype panel (1,000
+    samples per population: 4 FP4 K chunks)."""
+    hap, pos, pops = windowed_panel
+    hap_d = torch.as_tensor(hap, device=L.device())
+    plan = band.BandPlan(pos, 100_000.0, window=100_000.0, n_keys=100)
+    s8, k8 = plan.sums([band.pack_population(hap_d, r, "hap") for r in pops], want_pairs=True)
+    s4, k4 = plan.sums([band.pack_population(hap_d, r, "hap", fp4=True) for r in pops], want_pairs=True)
+    np.testing.assert_array_equal(np_(s4), np_(s8))
+    np.testing.assert_array_equal(np_(k4), np_(k8))
+
+
+def test_genotype_panel_fp4_equals_int8_full_size(genotype_panel):
+    """The same on the whole 50,000-SNP x 1,000-diploid genotype panel (all
+    1.25e9 pairs, 16 log bins; 4 FP4 K chunks with three planes)."""
+    g, pos = genotype_panel
+    g_d = torch.as_tensor(g, device=L.device())
+    plan = band.BandPlan(pos, 1e6, edges=np.logspace(0, 6, 17))
+    s8 = np_(plan.sums([band.pack_population(g_d, None, "geno")]))
+    s4 = np_(plan.sums([band.pack_population(g_d, None, "geno", fp4=True)]))
+    np.testing.assert_array_equal(s4, s8)

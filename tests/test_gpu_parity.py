@@ -500,3 +500,83 @@ def test_band_r2_fp4_64x64_equal_int8(family, estimator, missing, snp_sums):
             assert a[k] == b[k], k
         else:
             np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
+
+
+def _oracle_band_sums(d, family, pos, pops, edges, stat_set):
+    """Per-bin oracle sums (and sums of |terms|) of the moments or
+    within-population statistics over the pairs within the last edge."""
+    ii, jj = O.pairs_within_distance(pos, float(edges[-1]))
+    cnt = O.hap_counts if family == "hap" else O.geno_counts
+    cl = [cnt(d, ii, jj, r) for r in pops]
+    P = len(pops)
+    if stat_set == "self" and P > 1:
+        specs = [(t, [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(P)]
+    else:
+        specs = None
+    if family == "hap":
+        st = (O.two_pop_stats(cl[0][0], cl[1][0], cl[0][1], cl[1][1]) if (P == 2 and specs is None)
+              else O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl], specs))
+    else:
+        st = O.multi_pop_stats_geno([c for c, _ in cl], [nv for _, nv in cl], specs)
+    key = O.bin_index(pos[jj] - pos[ii], edges)
+    n = len(edges) - 1
+    return O.keyed_sums(key, st, n), O.keyed_sums(key, np.abs(st), n)
+
+
+@pytest.mark.parametrize("family,sizes,missing,stat_set", [
+    ("hap", (650, 650), True, "moments"), ("hap", (1100,), False, "moments"),
+    ("hap", (610, 700, 530, 900), True, "self"), ("hap", (520, 777, 600), True, "moments"),
+    ("geno", (700,), True, "moments"), ("geno", (1030,), False, "moments"), ("geno", (620, 545), True, "moments"),
+    ("geno", (513, 800), True, "self")])
+def test_band_sums_fp4_multichunk_vs_oracle_and_int8(family, sizes, missing, stat_set):
+    """Populations of 513-1,100 samples: 3-5 FP4 K chunks of 256 samples
+    (odd and even counts, both register-stage pipelines), sums against the
+    oracle and bit-identical to the int8 planes."""
+    n = sum(sizes)
+    if family == "hap":
+        d, pos = hap_data(140 + len(sizes), n, 320, 0.03 if missing else 0.0)
+        tol = 1e-12
+    else:
+        d, pos = geno_data(150 + len(sizes), n, 320, 0.04 if missing else 0.0)
+        tol = 1e-11
+    cut = np.concatenate([[0], np.cumsum(sizes)])
+    pops = [list(range(cut[p], cut[p + 1])) for p in range(len(sizes))]
+    print(f"fp4 multichunk {family} sizes={sizes} missing={missing} {stat_set}", flush=True)
+    edges = np.array([0.0, 300.0, 1000.0, 2500.0, 4000.0])
+    plan = band.BandPlan(pos, float(edges[-1]), edges=edges)
+    p4 = [band.pack_population(d, r, family, fp4=True) for r in pops]
+    p8 = [band.pack_population(d, r, family) for r in pops]
+    assert min(p.k_pad for p in p4) >= 768
+    s4 = np_(plan.sums(p4, stat_set=stat_set))
+    s8 = np_(plan.sums(p8, stat_set=stat_set))
+    np.testing.assert_array_equal(s4, s8)
+    want, scale = _oracle_band_sums(d, family, pos, pops, edges, stat_set)
+    err = np.max(np.abs(s4 - want) / np.maximum(scale, 1e-300))
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("family,estimator,n,missing,snp_sums", [
+    ("hap", "r2", 700, 0.02, False), ("hap", "sigma_d2", 1100, 0.03, True), ("hap", "naive", 1300, 0.0, True),
+    ("geno", "dosage_r2", 650, 0.03, False), ("geno", "dosage_r2", 1030, 0.0, True),
+    ("geno", "rogers_huff_r", 700, 0.0, False)])
+def test_band_r2_fp4_multichunk_equal_int8(family, estimator, n, missing, snp_sums):
+    """The FP4 r^2 kernels over 3-6 K chunks (missing data, genotype
+    estimators, per-SNP sums): every output bit-identical to int8 planes."""
+    d, pos = (hap_data(160, n, 500, missing) if family == "hap" else geno_data(161, n, 500, missing))
+    plan = band.BandPlan(pos, 12_000.0, window=1e300, n_keys=1)
+    p8 = band.pack_population(d, None, family)
+    p4 = band.pack_population(d, None, family, fp4=True)
+    kw = dict(n_hist=40, thresh=0.1, per_pair=not snp_sums, estimator=estimator, snp_sums=snp_sums)
+    a = plan.r2(p8, **kw)
+    b = plan.r2(p4, **kw)
+    for k in a:
+        if isinstance(a[k], (int, float)):
+            assert a[k] == b[k], k
+        else:
+            np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
+    if family == "hap" and estimator != "sigma_d2":
+        full = band.BandPlan(pos, np.inf, window=1e300, n_keys=1)
+        a = full.r2(p8, n_hist=50, thresh=0.2, per_pair=True, estimator=estimator)
+        b = full.r2(p4, n_hist=50, thresh=0.2, per_pair=True, estimator=estimator)
+        for k in ("hist", "nan_pairs", "sparse_i", "sparse_j", "sparse_r2", "r2"):
+            np.testing.assert_array_equal(np_(b[k]), np_(a[k]), err_msg=k)
