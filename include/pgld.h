@@ -262,10 +262,12 @@ int pgld_band_r2(const pgld_band_plan* plan, const int8_t* planes, const int32_t
                                (_tile_rogers_huff_r, ld_statistics.py:410-443); NaN for a constant site */
 #define PGLD_EST_DIP_R2 4   /* genotype planes: dosage r^2 about site means over valid samples
                                (_r2_matrix_diploid, ld_statistics.py:975-1021); 0 where undefined */
+#define PGLD_EST_D_NAIVE 5  /* haplotype planes: signed D = p_AB - p_i p_j with the frequencies of
+                               PGLD_EST_R2_NAIVE (pairwise_LD_v, haplotype_matrix.py:1092-1136) */
 
 /* pgld_band_r2 with a choice of estimator and two more optional outputs
- * (planes / marg are haplotype planes for estimators 0-2, genotype planes
- * for 3-4):
+ * (planes / marg are haplotype planes for estimators 0-2 and 5, genotype
+ * planes for 3-4):
  * values     FP64 per band pair in (i, j) order (as r2_out);
  * snp_sums   FP64 (2 * n_var): [v] = sum over pairs (v, j > v), [n_var + v] =
  *            sum over pairs (i < v, v) of the estimator (NaN pairs skipped);

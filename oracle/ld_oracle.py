@@ -30,8 +30,9 @@ Pinning (how each part is checked independently of this transcription):
 * heterozygosity, haplotype and genotype modes -- enumeration of allele pairs
   per site (missing haplotypes counted as reference over the constant |pop|).
 * bin/window sums -- brute-force per-key sums over explicit pair lists.
-* dense r^2 / sigma_D^2 / dosage r^2 matrices, ZnS, Omega -- equal to the
-  pinned count r^2 / numerator ratio / Rogers-Huff r^2 where they coincide;
+* dense D / r^2 / sigma_D^2 / dosage r^2 matrices, ZnS, Omega -- equal to
+  the pinned count D, r^2 / numerator ratio / Rogers-Huff r^2 where they
+  coincide;
   Omega by brute force over partitions.
 * locate_unlinked -- known answers and the reference's property tests
   (tests/test_ld_r_squared.py:124-146).
@@ -1062,10 +1063,10 @@ def segregating_sites(hap):
     return np.nonzero((dac > 0) & (dac < nv))[0]
 
 
-def naive_r2_matrix(hap):
-    """pairwise_r2 (haplotype_matrix.py:1092-1170): per-site p over every
-    valid sample, p_AB over the pair's valid samples, D = p_AB - p_i p_j,
-    r^2 = D^2 / (p_i q_i p_j q_j), 0 where that is not > 0; diagonal 0."""
+def _naive_d_and_p(hap):
+    """_pairwise_ld_core (haplotype_matrix.py:1092-1130): per-site p over
+    every valid sample, p_AB over the pair's valid samples (0 where none),
+    D = p_AB - p_i p_j."""
     h = np.asarray(hap)
     v = (h >= 0).astype(np.float64)
     hc = np.where(h >= 0, h, 0).astype(np.float64)
@@ -1074,7 +1075,22 @@ def naive_r2_matrix(hap):
     jn = v.T @ v
     j11 = hc.T @ hc
     pab = np.where(jn > 0, j11 / np.where(jn > 0, jn, 1.0), 0.0)
-    D = pab - np.outer(p, p)
+    return pab - np.outer(p, p), p
+
+
+def naive_d_matrix(hap):
+    """pairwise_LD_v (haplotype_matrix.py:1132-1136): the D matrix of
+    _pairwise_ld_core with a zero diagonal."""
+    D, _ = _naive_d_and_p(hap)
+    np.fill_diagonal(D, 0.0)
+    return D
+
+
+def naive_r2_matrix(hap):
+    """pairwise_r2 (haplotype_matrix.py:1092-1170): per-site p over every
+    valid sample, p_AB over the pair's valid samples, D = p_AB - p_i p_j,
+    r^2 = D^2 / (p_i q_i p_j q_j), 0 where that is not > 0; diagonal 0."""
+    D, p = _naive_d_and_p(hap)
     den = np.outer(p * (1 - p), p * (1 - p))
     with np.errstate(divide="ignore", invalid="ignore"):
         r2 = np.where(den > 0, (D ** 2) / np.where(den > 0, den, 1.0), 0.0)

@@ -18,7 +18,7 @@ FAMILY_HAP, FAMILY_GENO = 0, 1
 EINVAL, EDOMAIN, ECAPACITY, EUNSUPPORTED = -1, -2, -3, -4
 STAT_DD, STAT_DZ, STAT_PI2, STAT_PI2_SYM, STAT_SIGMA_D2 = 0, 1, 2, 3, 4
 KEY_BIN_ABS, KEY_BIN, KEY_WINDOW = 0, 1, 2
-EST_R2, EST_R2_NAIVE, EST_SIGMA_D2, EST_RH_R, EST_DIP_R2 = 0, 1, 2, 3, 4
+EST_R2, EST_R2_NAIVE, EST_SIGMA_D2, EST_RH_R, EST_DIP_R2, EST_D_NAIVE = 0, 1, 2, 3, 4, 5
 SET_MOMENTS, SET_SELF = 0, 1
 
 TERM_DTYPE = np.dtype([("type", "<i4"), ("pops", "<i4", (4,)), ("pad", "<i4"), ("weight", "<f8")])

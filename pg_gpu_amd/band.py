@@ -198,7 +198,8 @@ class BandPlan:
 
         estimator: "r2" (from the pair's counts, NaN where undefined),
         "naive" (per-site frequencies, 0 where undefined, as pairwise_r2),
-        "sigma_d2" (unbiased D^2 / pi_2); for a genotype population
+        "sigma_d2" (unbiased D^2 / pi_2), "d_naive" (signed D = p_AB - p_i p_j
+        on the frequencies of "naive", as pairwise_LD_v); for a genotype population
         "rogers_huff_r" (signed dosage r, no missing data) or "dosage_r2"
         (dosage r^2 about site means).  Returns a dict with
           hist      int64 (n_keys, n_hist) pairs per bin [b/n, (b+1)/n),
@@ -213,16 +214,16 @@ class BandPlan:
             raise ValueError("the popcount variant takes int8 planes (pack without fp4)")
         geno_est = estimator in ("rogers_huff_r", "dosage_r2")
         if (pop.family == L.FAMILY_GENO) != geno_est:
-            raise ValueError("estimators 'r2', 'naive', 'sigma_d2' take haplotype planes; 'rogers_huff_r' and "
-                             "'dosage_r2' take genotype planes")
+            raise ValueError("estimators 'r2', 'naive', 'sigma_d2', 'd_naive' take haplotype planes; "
+                             "'rogers_huff_r' and 'dosage_r2' take genotype planes")
         if with_missing is None:
             with_missing = pop.has_missing
             if with_missing is None:
                 raise ValueError("with_missing must be given for populations packed with validate=False")
         est = {"r2": L.EST_R2, "naive": L.EST_R2_NAIVE, "sigma_d2": L.EST_SIGMA_D2, "rogers_huff_r": L.EST_RH_R,
-               "dosage_r2": L.EST_DIP_R2}.get(estimator)
+               "dosage_r2": L.EST_DIP_R2, "d_naive": L.EST_D_NAIVE}.get(estimator)
         if est is None:
-            raise ValueError("estimator is 'r2', 'naive', 'sigma_d2', 'rogers_huff_r' or 'dosage_r2'")
+            raise ValueError("estimator is 'r2', 'naive', 'sigma_d2', 'd_naive', 'rogers_huff_r' or 'dosage_r2'")
         if method not in ("mfma", "popcount"):
             raise ValueError("method is 'mfma' or 'popcount'")
         if method == "popcount" and (est != L.EST_R2 or snp_sums):

@@ -288,12 +288,19 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     // Independent per-population statistics (column s * P + p): each
     // population's products are followed at once by its own epilogue, so no
     // counts are parked in LDS and more workgroups fit on a CU.
+    // The K chunks of all populations form one load stream (PopStream): the
+    // next population's first operands are in flight during this one's
+    // epilogue.
     using S1 = StatSet<FAMILY, 1, PGLD_SET_SELF>;
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
+    using PS = PopStream<FAMILY, MISSING, NIB>;
+    v4i pre0[PS::ITEMS], pre1[PS::ITEMS];
+    bool pre1_ok;
+    PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
-                                                  wave_live, opbuf, acc);
+      PS::template products<P>(pa, p, ba.n_var, tile.i0, tile.j0, wr, wc, wave_live, opbuf, acc, pre0, pre1,
+                               pre1_ok);
       // the first population's pass records the pair keys, the others reuse them
       const int kmode = tile.n_slots > 255 ? 0 : (p == 0 ? 1 : 2);
       if (wave_live)

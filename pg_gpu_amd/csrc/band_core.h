@@ -293,6 +293,185 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   __builtin_amdgcn_s_setprio(0);
 }
 
+// K loop of population p of a tile whose populations are evaluated one
+// after another (the within-population sets), with the operand loads run
+// across population boundaries: chunk c of population p is staged in
+// register set pre[c % 2] (pre0 / pre1, owned by the caller and live across
+// calls), consuming a chunk issues the load two chunks ahead, and the loads
+// issued by the last two chunks are population p + 1's chunks 0 and 1.  They
+// stay in flight through p's epilogue (which issues no vector-memory loads),
+// so only the tile's first chunks wait for memory.  On entry pre0 holds (or
+// is loading) chunk 0 of p and pre1 chunk 1 (anything when p has one chunk).
+// All four waves must call it (it contains barriers).
+template <int FAMILY, bool MISSING, bool NIB>
+struct PopStream {
+  using PL = Planes<FAMILY, MISSING>;
+  static constexpr int N = PL::N;
+  static constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;
+  static_assert(ITEMS == 4 * N, "copy layout");
+
+  __device__ __forceinline__ static int64_t row_bytes(int64_t k_pad) { return NIB ? k_pad / 2 : k_pad; }
+  // the same value in every lane, kept in scalar registers
+  __device__ __forceinline__ static int64_t uniform(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+  }
+  // One population's operand source: planes (from the first plane used),
+  // bytes per row and per plane, K chunks.
+  struct Src {
+    const GLOBAL_AS int8_t* base;
+    int64_t row, plane;
+    int nc;
+  };
+  __device__ __forceinline__ static Src source(const PopArgs& pa, int p, int64_t n_var) {
+    Src s;
+    s.row = uniform(row_bytes(pa.k_pad[p]));
+    s.plane = n_var * s.row;
+    s.base = (const GLOBAL_AS int8_t*)uniform((int64_t)pa.planes[p]) + (int64_t)PL::FIRST * s.plane;
+    s.nc = (int)(s.row / KSTEP);
+    return s;
+  }
+  // Operand rows of this thread's copies: copy m of thread t covers bytes
+  // 16 (t % 8) .. of row 32 (m % 2) + t / 8 of side (m / 2) / N, plane
+  // (m / 2) % N, and lands at LDS offset (t / 8) OP_ROW + 16 (t % 8) + 32 OP_ROW m.
+  struct Rows {
+    int snp[2][2];  // side, row half -> SNP (clamped to the panel)
+  };
+  __device__ __forceinline__ static Rows rows(int64_t n_var, int i0, int j0) {
+    Rows r;
+    const int t = threadIdx.x >> 3;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      r.snp[0][h] = (int)min((int64_t)i0 + t + 32 * h, n_var - 1);
+      r.snp[1][h] = (int)min((int64_t)j0 + t + 32 * h, n_var - 1);
+    }
+    return r;
+  }
+  __device__ __forceinline__ static void load(v4i (&pre)[ITEMS], const Src& s, const Rows& r, int chunk) {
+    const GLOBAL_AS int8_t* b = s.base + (int64_t)chunk * KSTEP + 16 * (threadIdx.x & 7);
+#pragma unroll
+    for (int m = 0; m < ITEMS; ++m) {
+      const int a = (m >> 1) % N, sd = (m >> 1) / N, h = m & 1;
+      pre[m] = *(const GLOBAL_AS v4i*)(b + a * s.plane + (int64_t)r.snp[sd][h] * s.row);
+    }
+  }
+
+  // K loop of population p of a tile whose populations are evaluated one
+  // after another (the within-population sets), with the operand loads run
+  // across population boundaries: chunk c of p is staged in register set
+  // pre0 (c even) or pre1 (c odd), owned by the caller and live across
+  // calls; consuming a chunk issues the load two chunks ahead, and the loads
+  // issued by p's last two chunks are p + 1's chunks 0 and 1.  They stay in
+  // flight through p's epilogue (which issues no vector-memory loads), so
+  // only the tile's first chunks wait for memory.  On entry pre0 holds chunk
+  // 0 of p and, if pre1_ok, pre1 chunk 1 (pre1_ok is false after a one-chunk
+  // population, whose K loop refills pre0 only); on return pre1_ok says the
+  // same of p + 1.  All four waves must call it (it contains barriers).
+  template <int P>
+  __device__ __forceinline__ static void products(const PopArgs& pa, int p, int64_t n_var, int i0, int j0, int wr,
+                                                  int wc, bool wave_live, unsigned char* __restrict__ opbuf,
+                                                  WaveAcc<N, NIB>& acc, v4i (&pre0)[ITEMS], v4i (&pre1)[ITEMS],
+                                                  bool& pre1_ok) {
+    const Src cur = source(pa, p, n_var);
+    const bool has_next = p + 1 < P;
+    const Src nxt = source(pa, has_next ? p + 1 : p, n_var);
+    const Rows rw = rows(n_var, i0, j0);
+    const int nc = cur.nc;
+    const int lane = threadIdx.x & 63;
+    const int dbase = (threadIdx.x >> 3) * OP_ROW + 16 * (threadIdx.x & 7);
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int b = 0; b < N; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+    __builtin_amdgcn_s_setprio(2);
+    const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
+    auto mfmas = [&]() {
+      v4i A[2][N], B[2][N];
+      auto frag = [&](int st, v4i* a_, v4i* b_) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+          a_[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
+          b_[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
+        }
+      };
+      frag(0, A[0], B[0]);
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        if (st < 3) frag(st + 1, A[(st + 1) & 1], B[(st + 1) & 1]);
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+#pragma unroll
+          for (int b = 0; b < N; ++b) {
+            if constexpr (NIB) {
+              const v4i x = A[st & 1][a], y = B[st & 1][b];
+              const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
+              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc.c[a][b], 4, 4, 0,
+                                                                            FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
+            } else {
+              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
+            }
+          }
+      }
+    };
+    auto stage = [&](v4i (&pre)[ITEMS]) {
+      __syncthreads();  // the previous chunk has been read by every wave
+#pragma unroll
+      for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dbase + 32 * OP_ROW * m) = pre[m];
+      __syncthreads();
+    };
+    // refill pre after consuming chunk c: chunk c + 2 of p, else chunk c % 2
+    // of p + 1 (chunk 0 when p + 1 has a single chunk, or of p again past the
+    // last population: read, never used).  One load sequence on every path,
+    // so the older register set's wait leaves the younger set's loads in flight.
+    auto refill = [&](v4i (&pre)[ITEMS], int c) {
+      const bool same = c + 2 < nc;
+      const int par = c & 1;
+      const int chunk = same ? c + 2 : ((has_next && nxt.nc > par) ? par : 0);
+      Src s;
+      s.base = same ? cur.base : nxt.base;
+      s.row = same ? cur.row : nxt.row;
+      s.plane = same ? cur.plane : nxt.plane;
+      load(pre, s, rw, chunk);
+    };
+    // chunk 0 outside the loop; pre1's late load (after a one-chunk
+    // population) is issued once pre0 is in LDS
+    stage(pre0);
+    if (!pre1_ok && nc > 1) load(pre1, cur, rw, 1);
+    refill(pre0, 0);
+    if (wave_live) mfmas();
+    int c = 1;
+#pragma unroll 1
+    for (; c + 2 <= nc; c += 2) {
+      stage(pre1);
+      refill(pre1, c);
+      if (wave_live) mfmas();
+      stage(pre0);
+      refill(pre0, c + 1);
+      if (wave_live) mfmas();
+    }
+    if (c < nc) {
+      stage(pre1);
+      refill(pre1, c);
+      if (wave_live) mfmas();
+    }
+    pre1_ok = nc > 1;
+    __builtin_amdgcn_s_setprio(0);
+  }
+
+  // first two chunks of a tile's population 0
+  __device__ __forceinline__ static void start(const PopArgs& pa, int64_t n_var, int i0, int j0,
+                                               v4i (&pre0)[ITEMS], v4i (&pre1)[ITEMS], bool& pre1_ok) {
+    const Src s = source(pa, 0, n_var);
+    const Rows rw = rows(n_var, i0, j0);
+    load(pre0, s, rw, 0);
+    load(pre1, s, rw, s.nc > 1 ? 1 : 0);
+    pre1_ok = s.nc > 1;
+  }
+};
+
 // Joint counts of accumulator element r (row i, col j) in the reference's
 // column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
 template <int FAMILY, bool MISSING, class ACC>

@@ -48,6 +48,7 @@ enum R2Est : int32_t {
   EST_SIGMA_D2 = 2,  // unbiased D^2 / pi_2 (_tile_sigma_d2); 0 unless n >= 4 and pi_2 != 0
   EST_RH_R = 3,      // genotypes: signed Rogers-Huff r of 0/1/2 dosages; NaN where a site is constant
   EST_DIP_R2 = 4,    // genotypes: dosage r^2 over valid samples, site means (_r2_matrix_diploid); 0 where undefined
+  EST_D_NAIVE = 5,   // signed D = p_AB - p_i p_j with the frequencies of EST_R2_NAIVE (pairwise_LD_v)
 };
 
 constexpr int R2_MAX_HIST_LDS = 8192;  // ints of per-tile histogram kept in LDS
@@ -72,7 +73,7 @@ __device__ __forceinline__ double hap_pair_value(int32_t n11, int32_t nA, int32_
     r2 = den > 0.0 ? num / den : __builtin_nan("");
     sval = is_nan ? 0.0 : r2;
     scnt = !is_nan;
-  } else if constexpr (EST == EST_R2_NAIVE) {
+  } else if constexpr (EST == EST_R2_NAIVE || EST == EST_D_NAIVE) {
 #pragma clang fp contract(off)
     // per-site frequencies over every valid sample of the site, joint
     // frequency over the pair's valid samples (the reference's order of
@@ -83,8 +84,12 @@ __device__ __forceinline__ double hap_pair_value(int32_t n11, int32_t nA, int32_
     const double pj = vj > 0.0 ? (double)marg[j] / vj : 0.0;
     const double pab = n > 0 ? (double)n11 / (double)n : 0.0;
     const double D = pab - pi * pj;
-    const double den = (pi * (1.0 - pi)) * (pj * (1.0 - pj));
-    r2 = den > 0.0 ? (D * D) / den : 0.0;
+    if constexpr (EST == EST_D_NAIVE) {
+      r2 = D;
+    } else {
+      const double den = (pi * (1.0 - pi)) * (pj * (1.0 - pj));
+      r2 = den > 0.0 ? (D * D) / den : 0.0;
+    }
     sval = r2;
     scnt = 1;
   } else {
@@ -650,7 +655,7 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
                    uint64_t* n_sparse, double* r2_out, double* snp_sums, int64_t* snp_counts, void* stream,
                    bool fp4 = false) {
   if (fp4) PGLD_REQUIRE(k_pad % 256 == 0, "pgld_band_ld_pairs_fp4: k_pad must be a multiple of 256 (FP4 planes)");
-  PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_DIP_R2, "pgld_band_ld_pairs: unknown estimator %d",
+  PGLD_REQUIRE(estimator >= EST_R2 && estimator <= EST_D_NAIVE, "pgld_band_ld_pairs: unknown estimator %d",
                estimator);
   PGLD_REQUIRE(!(estimator == EST_RH_R && with_missing),
                "pgld_band_ld_pairs: Rogers-Huff r needs genotypes without missing values");
@@ -717,7 +722,7 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
   dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
   if (plan->timing) PGLD_TRY_HIP(hipEventRecord(plan->ev[0], s), "pgld_band_r2 event");
   // haplotypes without missing data and no per-SNP sums: the 128 x 128 kernel
-  const bool wide = !geno && !miss && !popc && !snp_sums && plan->n_super > 0 &&
+  const bool wide = !geno && !miss && !popc && !snp_sums && plan->n_super > 0 && estimator != EST_D_NAIVE &&
                     (!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2W_HIST_PER_WAVE);
   if (wide && fp4) {
     dim3 gw((unsigned)plan->n_super);
@@ -773,6 +778,9 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
     else PGLD_R2(PGLD_FAMILY_GENO, false, false, EST_DIP_R2);
   } else if (popc) {
     if (miss) PGLD_R2_EST(true, true); else PGLD_R2_EST(false, true);
+  } else if (estimator == EST_D_NAIVE) {
+    if (miss) PGLD_R2(PGLD_FAMILY_HAP, true, false, EST_D_NAIVE);
+    else PGLD_R2(PGLD_FAMILY_HAP, false, false, EST_D_NAIVE);
   } else {
     if (miss) PGLD_R2_EST(true, false); else PGLD_R2_EST(false, false);
   }

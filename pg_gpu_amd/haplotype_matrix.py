@@ -352,6 +352,26 @@ class HaplotypeMatrix:
             out[key] = OrderedDict((n, float(v)) for n, v in zip(names, vals))
         return out
 
+    def pairwise_LD_v(self):
+        """Pairwise D = p_AB - p_i p_j of every variant pair as a dense (m, m)
+        float64 GPU tensor, zero diagonal: per-site frequencies over all
+        valid samples, joint frequency over the pair's valid samples (0 where
+        no sample is valid), as _pairwise_ld_core (haplotype_matrix.py:
+        1092-1136).  The joint counts come from the MFMA tiles."""
+        self.transfer_to_gpu()
+        T = L.torch()
+        m = self.num_variants
+        out = T.zeros((m, m), dtype=T.float64, device=L.device())
+        if m < 2:
+            return out
+        plan = B.BandPlan(np.arange(m, dtype=np.float64), np.inf, window=1e300, n_keys=1)
+        pk = B.pack_population(self._haplotypes, None, "hap", fp4=True)
+        vals = plan.r2(pk, histogram=False, sparse_cap=0, per_pair=True, estimator="d_naive")["r2"]
+        iu = T.triu_indices(m, m, offset=1, device=out.device)
+        out[iu[0], iu[1]] = vals
+        out[iu[1], iu[0]] = vals
+        return out
+
     def pairwise_r2(self, estimator: str = "r2"):
         """Pairwise r^2 of every variant pair as a dense (m, m) float64 GPU
         tensor: per-site frequencies over all valid samples, joint frequency

@@ -336,6 +336,9 @@ def test_zns_omega_pairwise_r2(missing):
     assert LS.omega(naive) == pytest.approx(O.omega_from_matrix(naive), rel=1e-12)
     full = np_(HaplotypeMatrix(h, np.arange(h.shape[1]) * 100).pairwise_r2())
     np.testing.assert_array_equal(full, O.naive_r2_matrix(h))
+    # the D matrix of pairwise_LD_v (haplotype_matrix.py:1132-1136), same frequencies
+    D = np_(HaplotypeMatrix(h, np.arange(h.shape[1]) * 100).pairwise_LD_v())
+    np.testing.assert_allclose(D, O.naive_d_matrix(h), rtol=0, atol=1e-15)
     # sigma_d2 ZnS keeps every segregating site whatever missing_data says (the
     # reference's internal 'project' mode, ld_statistics.py:721-726); the r2
     # ZnS and Omega honour 'exclude'; 'rogers_huff' on a HaplotypeMatrix is the

@@ -421,6 +421,12 @@ def test_naive_r2_and_sigma_matrices_pins():
     w = O.r_squared(c)
     np.testing.assert_allclose(R[ii, jj], np.where(np.isnan(w), 0.0, w), rtol=1e-12, atol=1e-15)
     np.testing.assert_array_equal(R, R.T)
+    # the D matrix: without missing data the count D (n11 n00 - n10 n01) / n^2
+    Dm = O.naive_d_matrix(h)
+    np.testing.assert_allclose(Dm[ii, jj], (c[:, 0] * c[:, 3] - c[:, 1] * c[:, 2]) / 40.0 ** 2, rtol=1e-12,
+                               atol=1e-15)
+    np.testing.assert_array_equal(Dm, Dm.T)
+    assert np.all(np.diag(Dm) == 0.0)
     S, valid = O.sigma_d2_matrix(h)
     s2, _ = O.sigma_d2_hap_from_counts(c)
     np.testing.assert_allclose(S[ii, jj], s2, rtol=1e-13, atol=1e-15)
