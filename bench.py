@@ -5,8 +5,8 @@ ragged_regions -- 512 independent regions of 500-20,000 SNPs x 50-5,000
 haplotypes in 4 populations, 3% MCAR missing, positions uniform over 1 Mb,
 all pairs of each region, per-region 16 log-bin DD/Dz/pi2 sums of every
 population (FP64).  One step = a HIP graph that repacks every region's
-population planes from the resident int8 matrices and runs one autotuned
-persistent fused launch + the per-key reduction.  The line also carries the
+population planes from the resident int8 matrices (one batched launch) and
+runs one autotuned persistent fused launch + the per-key reduction.  The line also carries the
 error of every statistic against the FP64 oracle on the regions of least
 work (all their pairs).
 
@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ragged_regions",
-                    choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2", "ragged_regions"])
+                    choices=["windowed_hap", "genotype_bins", "plain_hap", "dense_r2", "ragged_regions", "banded_r2"])
     ap.add_argument("--n-var", type=int, default=None)
     ap.add_argument("--n-samples", "--n-hap", dest="n_samples", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=1)
@@ -66,7 +66,8 @@ def parse():
                     help="operand planes of the fused sums (windowed and genotype workloads): int8 MFMA, or FP4 "
                          "(e2m1 0 / 1) on the FP4 MFMA")
     ap.add_argument("--method", default="mfma", choices=["mfma", "popcount"],
-                    help="tile products of the dense r^2 workload: int8 MFMA or AND + popcount on bit planes")
+                    help="tile products of the r^2 workloads: MFMA (int8 or FP4 planes) or AND + popcount on bit "
+                         "planes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 1 GiB (the HBM-counter reference of the traffic figure)")
     return ap.parse_args()
@@ -483,6 +484,59 @@ def run_dense_r2(args, torch):
     return line
 
 
+def run_banded_r2(args, torch):
+    """r^2 of every pair within 100 kb on the windowed panel (2% missing data),
+    histogram per 100 kb window of the left SNP: the banded missing-data r^2
+    workload on which the MFMA and popcount tile products are compared."""
+    from pg_gpu_amd import band, synthetic
+    from pg_gpu_amd import _lib as L
+
+    n_var = args.n_var or 100_000
+    n_hap = args.n_samples or 2000
+    macs = 4  # haplotypes with missing data: hh, hv, vh, vv
+    log("[bench] generating workload")
+    hap, pos, _ = synthetic.windowed_haplotype_panel(n_hap=n_hap, n_var=n_var)
+    dev = L.device()
+    hap_d = torch.as_tensor(hap, device=dev)
+    plan = band.BandPlan(pos, 100_000.0, window=100_000.0, n_keys=100, origin=0.0)
+    pair_samples = plan.n_pairs * n_hap
+    log(f"[bench] pairs={plan.n_pairs} tiles={plan.n_tiles} pair_samples={pair_samples:.4e}")
+    fp4 = args.planes == "fp4" and args.method == "mfma"
+    pk = band.pack_population(hap_d, None, "hap", fp4=fp4)
+    assert pk.has_missing
+    cap = 1 << 22
+
+    def step():
+        p2 = band.pack_population(hap_d, None, "hap", validate=False, out=pk, fp4=fp4)
+        return plan.r2(p2, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=True, method=args.method)
+
+    ms_step, out = timed_steps(torch, step, args.steps, args.warmup)
+    value = pair_samples / (ms_step * 1e-3) / 1e9
+    log(f"[bench] step {ms_step:.3f} ms, {out['n_sparse']} pairs with r^2 > 0.2")
+    plan.set_timing(True)
+    kms = []
+    for _ in range(max(3, args.steps)):
+        plan.r2(pk, n_hist=100, thresh=0.2, sparse_cap=cap, with_missing=True, method=args.method)
+        kms.append(plan.kernel_ms())
+    plan.set_timing(False)
+    k_ms = float(np.mean(kms))
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_band_r2", macs, fp4=fp4)
+    if args.method == "popcount":
+        roof["ops"] = "pair*sample MACs x 2 as AND + popcount on the VALU (bit planes), against the int8 MFMA peak"
+        roof["kernel"] = "k_band_r2<popcount>"
+    log(f"[bench] k_band_r2 ({args.method}, {'fp4' if fp4 else 'int8/bits'}) {k_ms:.3f} ms")
+    hist = out["hist"].cpu().numpy()
+    line = base_line(args, value, ms_step,
+                     f"banded r^2 with missing data: {n_var:,} SNPs x {n_hap:,} haplotypes, 2% missing, pairs within "
+                     "100 kb, 100-bin r^2 histogram per 100 kb window + sparse list for r^2 > 0.2",
+                     {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
+                      "method": args.method, "planes": "fp4" if fp4 else ("bits" if args.method == "popcount" else
+                                                                          "int8"),
+                      "hist_total": int(hist.sum())}, dtype="fp4" if fp4 else "int8")
+    line.update({"roofline": roof, "cpu_baseline": None})
+    return line
+
+
 def run_ragged_regions(args, torch):
     from pg_gpu_amd import band, batch, synthetic
     from pg_gpu_amd import _lib as L
@@ -511,8 +565,10 @@ def run_ragged_regions(args, torch):
     out = torch.empty((n_regions, 16, 12), dtype=torch.float64, device=dev)
     kp = torch.empty((n_regions, 16), dtype=torch.float64, device=dev)
 
+    packer = batch.PackBatch(datas, rows_d, packed)  # every population of every region in one launch
+
     def step_body():
-        batch.pack_regions(datas, rows_d, out=packed, validate=False, fp4=fp4)
+        packer.run()
         b.sums(with_missing, out=out, key_pairs=kp)
 
     graph = batch.Graph().capture(step_body)
@@ -524,7 +580,7 @@ def run_ragged_regions(args, torch):
     ms_step, res = timed_steps(torch, step, args.steps, args.warmup)
     res = res.clone()
     value = pair_samples / (ms_step * 1e-3) / 1e9
-    log(f"[bench] graph step {ms_step:.3f} ms (2,048 packs + one persistent pass)")
+    log(f"[bench] graph step {ms_step:.3f} ms (one batched pack of 2,048 populations + one persistent pass)")
     b.set_timing(True)
     kms = []
     for _ in range(max(3, min(args.steps, 5))):
@@ -588,7 +644,7 @@ def run_ragged_regions(args, torch):
                      f"ragged persistent batch: {n_regions} independent regions, SNPs uniform 500-20,000, "
                      "haplotypes uniform 50-5,000 in 4 populations, 3% missing, positions uniform over 1 Mb, all "
                      "pairs; per-region 16 log-bin DD/Dz/pi2 sums per population (FP64) from one HIP graph "
-                     "(packing + one autotuned persistent launch)",
+                     "(one batched packing launch + one autotuned persistent launch)",
                      {"regions": n_regions, "pairs": b.n_pairs, "pair_samples": pair_samples, "tiles": b.n_tiles,
                       "schedule": choice, "planes": args.planes}, dtype="fp4" if fp4 else "int8")
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
@@ -672,7 +728,7 @@ def main():
         log(f"[bench] copied {n} bytes 4 times")
         return
     run = {"windowed_hap": run_windowed_hap, "genotype_bins": run_genotype_bins, "plain_hap": run_plain_hap,
-           "dense_r2": run_dense_r2, "ragged_regions": run_ragged_regions}
+           "dense_r2": run_dense_r2, "ragged_regions": run_ragged_regions, "banded_r2": run_banded_r2}
     line = run[args.workload](args, torch)
     print(json.dumps(line), flush=True)
 

@@ -153,6 +153,25 @@ int pgld_pack_planes_fp4(int32_t family, const int8_t* data, int64_t n_rows, int
                          const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
                          int32_t* flags, void* stream);
 
+/* Many pgld_pack_planes(_fp4) calls as one launch (no counterpart in the
+ * reference: the populations of a batch of regions are packed once per step,
+ * and thousands of small launches would cost more than the packing).  Job j
+ * packs data[j] (n_rows[j] x n_var[j]), rows[j] (n_sel[j] entries, or NULL =
+ * all rows) into planes[j] / marg[j] / flags[j] with k_pad[j], exactly as the
+ * single call would (same bytes, margins and flags); all jobs share family
+ * and layout (fp4 != 0: FP4 planes).  Arguments are HOST arrays of n_jobs
+ * entries (device pointers for the buffers); create validates them and
+ * uploads the job table once; run (capturable into a HIP graph: two kernel
+ * launches, no allocation, no memset) may be repeated while the buffers stay
+ * valid. */
+typedef struct pgld_pack_batch pgld_pack_batch;
+int pgld_pack_batch_create(int32_t family, int32_t fp4, int32_t n_jobs, const int8_t* const* data,
+                           const int64_t* n_rows, const int64_t* n_var, const int32_t* const* rows,
+                           const int64_t* n_sel, const int64_t* k_pad, int8_t* const* planes,
+                           int32_t* const* marg, int32_t* const* flags, pgld_pack_batch** out);
+int pgld_pack_batch_run(const pgld_pack_batch* pb, void* stream);
+int pgld_pack_batch_destroy(pgld_pack_batch* pb);
+
 typedef struct pgld_band_plan pgld_band_plan;
 
 /* Build the tile schedule of the distance band {(i, j): i < j,
@@ -342,12 +361,23 @@ int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, do
  * blocks of one region dealt to the 8 XCDs by work and interleaved so that
  * position p runs on XCD p % 8; use grid 0 or a multiple of 8); xcd_split =
  * each XCD walks a contiguous eighth of the order. */
-int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split);
-/* Autotuner: time the schedule candidates on this plan, keep the fastest
- * (also cached per shape class for the process); outputs may be NULL;
- * *best_ms = -1 when the cache answered. */
+int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split,
+                        int32_t kernel_variant);
+/* kernel_variant (bit-identical sums; used by the FP4 within-population
+ * haplotype sets, ignored by the others): 0 = K loop streamed across the
+ * populations of a tile, fast-path rows unrolled; 1 = one K loop per
+ * population; 2 = streamed, rows in a rolled loop. */
+/* Autotuner: time the schedule and kernel-variant candidates on this plan,
+ * keep the fastest.  The choice is cached per shape class (GPU architecture,
+ * CUs, family, planes, populations, set, missing data, log2 of tiles, mean
+ * region size, pairs per SNP and mean population size) in the process and in
+ * a text file -- $PGLD_TUNE_CACHE, else $XDG_CACHE_HOME/pgld/batch_tune.txt or
+ * ~/.cache/pgld/batch_tune.txt; PGLD_TUNE_CACHE="" keeps it in the process --
+ * so later processes skip the timing.  Outputs may be NULL; *best_ms = -1
+ * when the cache answered. */
 int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream,
-                        int32_t* grid, int32_t* order, int32_t* xcd_split, float* best_ms);
+                        int32_t* grid, int32_t* order, int32_t* xcd_split, int32_t* kernel_variant,
+                        float* best_ms);
 
 /* HIP graph capture of a sequence of libpgld calls on one created stream:
  * pgld_graph_begin(stream); ... calls on that stream ...; pgld_graph_end. The

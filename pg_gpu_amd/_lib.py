@@ -42,6 +42,10 @@ _SIGNATURES = {
     "pgld_keyed_sums": ([_vp, _vp, _i64, _i32, _i32, _vp, _vp], C.c_int),
     "pgld_pack_planes": ([_i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp], C.c_int),
     "pgld_pack_planes_fp4": ([_i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp], C.c_int),
+    "pgld_pack_batch_create": ([_i32, _i32, _i32, _PP, C.POINTER(_i64), C.POINTER(_i64), _PP, C.POINTER(_i64),
+                                C.POINTER(_i64), _PP, _PP, _PP, C.POINTER(_vp)], C.c_int),
+    "pgld_pack_batch_run": ([_vp, _vp], C.c_int),
+    "pgld_pack_batch_destroy": ([_vp], C.c_int),
     "pgld_band_plan_create": ([_vp, _i64, _f64, _i32, _vp, _vp, _i32, _f64, _f64, C.POINTER(_vp)], C.c_int),
     "pgld_band_plan_info": ([_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)], C.c_int),
     "pgld_band_plan_destroy": ([_vp], C.c_int),
@@ -58,9 +62,9 @@ _SIGNATURES = {
     "pgld_batch_bind": ([_vp, _i32, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32)], C.c_int),
     "pgld_batch_bind_fp4": ([_vp, _i32, _i32, _i32, _PP, _PP, C.POINTER(_i64), C.POINTER(_i32)], C.c_int),
     "pgld_batch_sums": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
-    "pgld_batch_schedule": ([_vp, _i32, _i32, _i32], C.c_int),
+    "pgld_batch_schedule": ([_vp, _i32, _i32, _i32, _i32], C.c_int),
     "pgld_batch_autotune": ([_vp, _i32, _vp, _vp, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32),
-                             C.POINTER(C.c_float)], C.c_int),
+                             C.POINTER(_i32), C.POINTER(C.c_float)], C.c_int),
     "pgld_graph_begin": ([_vp], C.c_int),
     "pgld_graph_end": ([_vp, C.POINTER(_vp)], C.c_int),
     "pgld_graph_launch": ([_vp, _vp], C.c_int),

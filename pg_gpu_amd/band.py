@@ -37,7 +37,8 @@ class PackedPop:
     flags: object = None  # int32 tensor [any missing, any out of domain]
     bits: object = None   # uint32-as-int32 tensor of bit planes (popcount variant), made on demand
     bits_fresh: bool = False
-    fp4: bool = False     # planes are FP4 (pgld_pack_planes_fp4): fused band sums only
+    fp4: bool = False     # planes are FP4 (pgld_pack_planes_fp4): fused sums and r^2 passes; counts and
+    #                       the popcount variant take int8 planes
 
     def bit_planes(self):
         """Bit planes of this population (32 samples per word), rebuilt from
@@ -58,7 +59,8 @@ def pack_population(data, rows=None, family="hap", k_pad=None, validate=True, ou
     """Pack one population's rows of a (rows, n_var) int8 matrix.
 
     fp4=True packs the indicators as FP4 (two samples per byte) for the
-    FP4-MFMA fused band sums (BandPlan.sums); the other band passes take the
+    FP4-MFMA passes (BandPlan.sums and BandPlan.r2, every estimator, with or
+    without missing data); BandPlan.counts and the popcount variant take the
     int8 planes.
 
     validate=True reads the kernel's value flags back (one small sync) to

@@ -106,21 +106,27 @@ class RegionBatch:
 
     _ORDERS = ("cost", "natural", "blocked")
 
-    def schedule(self, grid=0, order="cost", xcd_split=False):
+    def schedule(self, grid=0, order="cost", xcd_split=False, kernel_variant=0):
         """grid 0 = one workgroup per tile; order "cost" (descending work),
-        "natural" or "blocked" (8 x 8 tile blocks per XCD for L2 reuse)."""
+        "natural" or "blocked" (8 x 8 tile blocks per XCD for L2 reuse);
+        kernel_variant 0 / 1 / 2 (FP4 within-population haplotype sets: K
+        loop streamed across populations / per population / streamed with
+        rolled epilogue rows; the sums are identical)."""
         if order not in self._ORDERS:
             raise ValueError(f"order must be one of {self._ORDERS}")
-        L.call("pgld_batch_schedule", self._h, int(grid), self._ORDERS.index(order), int(bool(xcd_split)))
+        L.call("pgld_batch_schedule", self._h, int(grid), self._ORDERS.index(order), int(bool(xcd_split)),
+               int(kernel_variant))
 
     def autotune(self, with_missing):
-        """Time the schedule candidates, keep the fastest; returns the choice."""
+        """Time the schedule and kernel-variant candidates, keep the fastest;
+        returns the choice ("ms" is None when the in-process or on-disk
+        cache answered)."""
         out, _ = self._out(None, None, False)
-        g, o, x, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        g, o, x, kv, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
         L.call("pgld_batch_autotune", self._h, int(bool(with_missing)), L.ptr(out), None, L.stream_handle(),
-               C.byref(g), C.byref(o), C.byref(x), C.byref(ms))
+               C.byref(g), C.byref(o), C.byref(x), C.byref(kv), C.byref(ms))
         return {"grid": g.value, "order": self._ORDERS[o.value], "xcd_split": bool(x.value),
-                "ms": None if ms.value < 0 else float(ms.value)}
+                "kernel_variant": kv.value, "ms": None if ms.value < 0 else float(ms.value)}
 
     def set_timing(self, enable=True):
         L.call("pgld_band_plan_set_timing", self._h, int(bool(enable)))
@@ -192,6 +198,68 @@ class Graph:
             pass
 
 
+class PackBatch:
+    """Repacks the populations of many regions into existing PackedPop
+    buffers with one launch (pgld_pack_batch_*): the same planes, margins
+    and flags as pack_population on each, without thousands of small
+    launches.  datas[r], rows_per_region[r] and packed[r] as pack_regions;
+    the matrices, row lists and buffers must stay alive while it is used."""
+
+    def __init__(self, datas, rows_per_region, packed):
+        T = L.torch()
+        jobs = []
+        self._keep = []
+        for d, rows, pk in zip(datas, rows_per_region, packed):
+            g = L.to_device(d, np.int8)
+            self._keep.append(g)
+            for rw, p in zip(rows, pk):
+                if rw is None:
+                    rt, ns = None, g.shape[0]
+                else:
+                    rt = rw if (L.is_tensor(rw) and rw.is_cuda and rw.dtype == T.int32) else \
+                        L.to_device(np.asarray(L.to_numpy(rw), dtype=np.int64), np.int32)
+                    ns = int(rt.shape[0])
+                    self._keep.append(rt)
+                jobs.append((g, rt, ns, p))
+        if not jobs:
+            raise ValueError("nothing to pack")
+        fam = jobs[0][3].family
+        fp4 = jobs[0][3].fp4
+        if any(p.family != fam or p.fp4 != fp4 for _, _, _, p in jobs):
+            raise ValueError("all populations must share family and plane layout")
+        n = len(jobs)
+        data, _k1 = L.ptr_array([g for g, _, _, _ in jobs])
+        rows_p, _k2 = L.ptr_array([rt for _, rt, _, _ in jobs])
+        planes, _k3 = L.ptr_array([p.planes for _, _, _, p in jobs])
+        margs, _k4 = L.ptr_array([p.marg for _, _, _, p in jobs])
+        flags, _k5 = L.ptr_array([p.flags for _, _, _, p in jobs])
+        n_rows = (C.c_int64 * n)(*[g.shape[0] for g, _, _, _ in jobs])
+        n_var = (C.c_int64 * n)(*[g.shape[1] for g, _, _, _ in jobs])
+        n_sel = (C.c_int64 * n)(*[ns for _, _, ns, _ in jobs])
+        k_pad = (C.c_int64 * n)(*[p.k_pad for _, _, _, p in jobs])
+        for _, _, ns, p in jobs:
+            if p.n_samples != ns:
+                raise ValueError("packed buffers do not match the row lists")
+        h = C.c_void_p()
+        L.call("pgld_pack_batch_create", fam, int(fp4), n, data, n_rows, n_var, rows_p, n_sel, k_pad, planes, margs,
+               flags, C.byref(h))
+        self._h = h
+
+    def run(self, stream=None):
+        L.call("pgld_pack_batch_run", self._h, stream if stream is not None else L.stream_handle())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().pgld_pack_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def pack_regions(datas, rows_per_region, family="hap", out=None, validate=True, fp4=False):
     """PackedPop lists for every region: datas[r] is region r's int8 matrix,
     rows_per_region[r] its population row lists (fp4: FP4 planes)."""
@@ -203,4 +271,4 @@ def pack_regions(datas, rows_per_region, family="hap", out=None, validate=True, 
     return res
 
 
-__all__ = ["RegionBatch", "Graph", "pack_regions", "PackedPop"]
+__all__ = ["RegionBatch", "Graph", "PackBatch", "pack_regions", "PackedPop"]

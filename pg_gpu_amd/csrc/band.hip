@@ -24,6 +24,10 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <cstdio>
+#include <cstdlib>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -95,7 +99,12 @@ constexpr int band_min_waves(int family, int P, int set) {
 // region * n_keys_local for the tiles of a region batch.  The pair sums are
 // folded over the wave by the halving butterfly in groups of SCAT values
 // (more values per group: fewer shuffles, more live registers).
-template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8, bool NIB = false>
+// Kernel variants (KV, bit-identical results; the batch autotuner chooses):
+// bit 0 set = each population's K loop on its own (no load stream across
+// populations), bit 1 set = fast-path rows in a rolled loop.
+constexpr int KV_PER_POP = 1, KV_ROLLED = 2;
+
+template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8, bool NIB = false, int KV = 0>
 __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
                                                int32_t key_base, unsigned char* smem) {
   using PL = Planes<FAMILY, MISSING>;
@@ -104,7 +113,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // haplotype within-population sets evaluate the moments straight from the
   // accumulators (hap_self_stats); their fast-path rows are unrolled
   constexpr bool HAP_SELF = FAMILY == PGLD_FAMILY_HAP && (SET == PGLD_SET_SELF || P == 1);
-  constexpr bool FAST_UNROLL = false;
+  constexpr bool FAST_UNROLL = HAP_SELF && !(KV & KV_ROLLED);
   constexpr int NS = SS::N + 1;  // statistics + pair count
   // smem layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
@@ -228,9 +237,14 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         for (int s = 0; s < M; ++s) lacc[s] += out[s];
       };
       if constexpr (FAST_UNROLL) {
-        // straight-line: accumulator elements addressed statically
+        // straight-line: accumulator elements addressed statically; rows are
+        // scheduled two at a time (two independent FP64 chains interleave,
+        // without the register growth of scheduling all 16 together)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) row(r);
+        for (int r = 0; r < 16; ++r) {
+          row(r);
+          if (r & 1) __builtin_amdgcn_sched_barrier(0);
+        }
       } else {
 #pragma unroll 1
         for (int r = 0; r < 16; ++r) row(r);
@@ -296,11 +310,15 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     using PS = PopStream<FAMILY, MISSING, NIB>;
     v4i pre0[PS::ITEMS], pre1[PS::ITEMS];
     bool pre1_ok;
-    PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
+    if constexpr (!(KV & KV_PER_POP)) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      PS::template products<P>(pa, p, ba.n_var, tile.i0, tile.j0, wr, wc, wave_live, opbuf, acc, pre0, pre1,
-                               pre1_ok);
+      if constexpr (KV & KV_PER_POP)
+        block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
+                                                    wave_live, opbuf, acc);
+      else
+        PS::template products<P>(pa, p, ba.n_var, tile.i0, tile.j0, wr, wc, wave_live, opbuf, acc, pre0, pre1,
+                                 pre1_ok);
       // the first population's pass records the pair keys, the others reuse them
       const int kmode = tile.n_slots > 255 ? 0 : (p == 0 ? 1 : 2);
       if (wave_live)
@@ -398,7 +416,7 @@ constexpr int batch_min_waves(int family, int P, int set) {
   return (family == PGLD_FAMILY_HAP && (P <= 3 || set == PGLD_SET_SELF)) ? band_min_waves(family, P, set) : 1;
 }
 
-template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
+template <int FAMILY, bool MISSING, int P, int SET, bool NIB, int KV = 0>
 __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void k_batch_sums(
     BandArgs ba, const RegionDesc* __restrict__ regions, const int32_t* __restrict__ order, int64_t n_tiles,
     int32_t xcd_split) {
@@ -427,7 +445,8 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     __syncthreads();  // the previous tile's LDS is no longer read
     // rd.pa stays in global memory: the population loop indexes it at run time
     // one value per butterfly for the statistic sets that would otherwise spill
-    band_sums_tile<FAMILY, MISSING, P, SET, (SET == PGLD_SET_SELF ? 8 : 1), NIB>(lb, rd.pa, tile, key_base, smem);
+    band_sums_tile<FAMILY, MISSING, P, SET, (SET == PGLD_SET_SELF ? 8 : 1), NIB, KV>(lb, rd.pa, tile, key_base,
+                                                                                      smem);
   }
 }
 
@@ -561,23 +580,17 @@ __device__ __forceinline__ void set_flag_once(int32_t* flags, int f) {
 
 // NIB: FP4 planes (two samples per byte as e2m1 0 / 1.0 codes, rows of
 // k_pad / 2 bytes) instead of int8 0/1 bytes.
-template <int FAMILY, bool WORDS, bool NIB = false>
-__global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
-                                              const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
-                                              int8_t* __restrict__ planes, int32_t* __restrict__ marg,
-                                              int32_t* __restrict__ flags) {
+// One 128-sample x 128-variant tile t of a population (variant blocks
+// fastest: nx = ceil(n_var / 128) tiles per sample block).
+template <int FAMILY, bool WORDS, bool NIB>
+__device__ __forceinline__ void pack_tile(const int8_t* __restrict__ data, int64_t n_var,
+                                          const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
+                                          int8_t* __restrict__ planes, int32_t* __restrict__ marg,
+                                          int32_t* __restrict__ flags, int64_t t, int64_t nx, uint32_t* tile,
+                                          int64_t* srow) {
   constexpr int NPL = FAMILY == PGLD_FAMILY_HAP ? 2 : 3;
-  __shared__ uint32_t tile[PK_S * PK_Q];  // [sample][variant quad ^ pk_swz(sample)]
-  __shared__ int64_t srow[PK_S];
   const int tid = threadIdx.x;
   static_assert(PK_V == 128 && PK_S == 128, "index math below assumes 128 x 128 tiles");
-  // Block (x, y) of the launch grid packs tile t = xcd_tile(linear id),
-  // variant blocks fastest: each XCD takes a contiguous run of the tiles, so
-  // neighbouring variant blocks of the same sample rows run on one XCD — the
-  // 128-byte lines that straddle two of them (rows of n_var bytes are rarely
-  // line-aligned) are fetched into one L2, and reads stay in the same DRAM rows.
-  const int64_t nx = gridDim.x;
-  const int64_t t = xcd_tile((int64_t)blockIdx.x + (int64_t)blockIdx.y * nx, nx * gridDim.y);
   const int64_t v0 = (t % nx) * PK_V, k0 = (t / nx) * PK_S;
   if (tid < PK_S) srow[tid] = k0 + tid < n_sel ? (int64_t)(rows ? rows[k0 + tid] : k0 + tid) * n_var : -1;
   __syncthreads();
@@ -711,6 +724,73 @@ __global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, i
   if ((tid & 63) == 0) {
     if (mneg) set_flag_once(flags, 0);
     if (mbad) set_flag_once(flags, 1);
+  }
+}
+
+template <int FAMILY, bool WORDS, bool NIB = false>
+__global__ __launch_bounds__(256) void k_pack(const int8_t* __restrict__ data, int64_t n_var,
+                                              const int32_t* __restrict__ rows, int64_t n_sel, int64_t k_pad,
+                                              int8_t* __restrict__ planes, int32_t* __restrict__ marg,
+                                              int32_t* __restrict__ flags) {
+  __shared__ uint32_t tile[PK_S * PK_Q];  // [sample][variant quad ^ pk_swz(sample)]
+  __shared__ int64_t srow[PK_S];
+  // Block (x, y) of the launch grid packs tile t = xcd_tile(linear id),
+  // variant blocks fastest: each XCD takes a contiguous run of the tiles, so
+  // neighbouring variant blocks of the same sample rows run on one XCD -- the
+  // 128-byte lines that straddle two of them (rows of n_var bytes are rarely
+  // line-aligned) are fetched into one L2, and reads stay in the same DRAM rows.
+  const int64_t nx = gridDim.x;
+  const int64_t t = xcd_tile((int64_t)blockIdx.x + (int64_t)blockIdx.y * nx, nx * gridDim.y);
+  pack_tile<FAMILY, WORDS, NIB>(data, n_var, rows, n_sel, k_pad, planes, marg, flags, t, nx, tile, srow);
+}
+
+// Many populations (of many regions) in one launch: job j owns the global
+// tiles [job.tile0, job.tile0 + nx ny); the block's job is found by binary
+// search over the table (one workgroup-uniform search per block).
+struct PackJob {
+  const int8_t* data;
+  const int32_t* rows;
+  int8_t* planes;
+  int32_t* marg;
+  int32_t* flags;
+  int64_t n_var, n_sel, k_pad, tile0, nx;
+  int32_t words, pad_;
+};
+
+__device__ __forceinline__ int find_job(const PackJob* __restrict__ jobs, int n_jobs, int64_t t) {
+  int lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (jobs[mid].tile0 <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int FAMILY, bool NIB>
+__global__ __launch_bounds__(256) void k_pack_batch(const PackJob* __restrict__ jobs, int n_jobs, int64_t n_tiles) {
+  __shared__ uint32_t tile[PK_S * PK_Q];
+  __shared__ int64_t srow[PK_S];
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= n_tiles) return;
+  const PackJob& j = jobs[find_job(jobs, n_jobs, t)];
+  const int64_t lt = t - j.tile0;
+  if (j.words)
+    pack_tile<FAMILY, true, NIB>(j.data, j.n_var, j.rows, j.n_sel, j.k_pad, j.planes, j.marg, j.flags, lt, j.nx,
+                                 tile, srow);
+  else
+    pack_tile<FAMILY, false, NIB>(j.data, j.n_var, j.rows, j.n_sel, j.k_pad, j.planes, j.marg, j.flags, lt, j.nx,
+                                  tile, srow);
+}
+
+// Clears every job's margins and flags (the packing adds into them): one
+// grid-stride pass over the concatenated (planes x n_var + 2) words.
+template <int NPL>
+__global__ __launch_bounds__(256) void k_pack_batch_clear(const PackJob* __restrict__ jobs, int n_jobs) {
+  for (int j = blockIdx.x; j < n_jobs; j += gridDim.x) {
+    const PackJob& jb = jobs[j];
+    const int64_t n = (int64_t)NPL * jb.n_var;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) jb.marg[e] = 0;
+    if (threadIdx.x < 2) jb.flags[threadIdx.x] = 0;
   }
 }
 
@@ -986,9 +1066,25 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const Po
     const int m = plan->order_mode;
     const int64_t len = plan->order_len[m];
     const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
-    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB>), dim3((unsigned)grid), dim3(BAND_THREADS), lds, s,
-                       ba, reinterpret_cast<const RegionDesc*>(plan->d_regions), plan->d_order + plan->order_off[m],
-                       len, (int32_t)plan->xcd_split);
+    const auto* rd = reinterpret_cast<const RegionDesc*>(plan->d_regions);
+    const int32_t* ord = plan->d_order + plan->order_off[m];
+    const int32_t xs = (int32_t)plan->xcd_split;
+    const dim3 g((unsigned)grid), b(BAND_THREADS);
+    // kernel variants exist for the FP4 within-population haplotype sets
+    constexpr bool VARIANTS = FAMILY == PGLD_FAMILY_HAP && SET == PGLD_SET_SELF && P > 1 && NIB;
+    if constexpr (VARIANTS) {
+      if (plan->kvar == KV_PER_POP) {
+        hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV_PER_POP>), g, b, lds, s, ba, rd, ord, len,
+                           xs);
+        return;
+      }
+      if (plan->kvar == KV_ROLLED) {
+        hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV_ROLLED>), g, b, lds, s, ba, rd, ord, len,
+                           xs);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB>), g, b, lds, s, ba, rd, ord, len, xs);
   } else {
     hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB>), dim3((unsigned)plan->n_tiles),
                        dim3(BAND_THREADS), lds, s, ba, pa);
@@ -1123,6 +1219,99 @@ int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t
 int pgld_pack_planes_fp4(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var, const int32_t* rows,
                          int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg, int32_t* flags, void* stream) {
   return pack_planes(family, data, n_rows, n_var, rows, n_sel, k_pad, planes, marg, flags, stream, true);
+}
+
+}  // extern "C"
+
+struct pgld_pack_batch {
+  int32_t family = 0;
+  bool fp4 = false;
+  int32_t n_jobs = 0;
+  int64_t n_tiles = 0;
+  pgld::PackJob* d_jobs = nullptr;
+};
+
+extern "C" {
+
+int pgld_pack_batch_create(int32_t family, int32_t fp4, int32_t n_jobs, const int8_t* const* data,
+                           const int64_t* n_rows, const int64_t* n_var, const int32_t* const* rows,
+                           const int64_t* n_sel, const int64_t* k_pad, int8_t* const* planes, int32_t* const* marg,
+                           int32_t* const* flags, pgld_pack_batch** out) {
+  PGLD_REQUIRE(out, "pgld_pack_batch_create: null output");
+  *out = nullptr;
+  PGLD_REQUIRE(family == PGLD_FAMILY_HAP || family == PGLD_FAMILY_GENO, "pgld_pack_batch_create: bad family");
+  PGLD_REQUIRE(n_jobs >= 1 && data && n_rows && n_var && rows && n_sel && k_pad && planes && marg && flags,
+               "pgld_pack_batch_create: null pointer or no jobs");
+  std::vector<PackJob> jobs(n_jobs);
+  int64_t t0 = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    const int64_t ns = rows[j] ? n_sel[j] : n_rows[j];
+    PGLD_REQUIRE(n_var[j] >= 0 && ns >= 0, "pgld_pack_batch_create: negative size (job %d)", j);
+    PGLD_REQUIRE(k_pad[j] % (fp4 ? 256 : 128) == 0 && k_pad[j] >= ns && k_pad[j] > 0,
+                 "pgld_pack_batch_create: k_pad must be a positive multiple of %d >= n_sel (job %d)", fp4 ? 256 : 128,
+                 j);
+    PGLD_REQUIRE(ns <= 32767, "pgld_pack_batch_create: at most 32767 samples per population (job %d)", j);
+    PGLD_REQUIRE(data[j] && planes[j] && marg[j] && flags[j], "pgld_pack_batch_create: null pointer (job %d)", j);
+    PackJob& b = jobs[j];
+    b.data = data[j];
+    b.rows = rows[j];
+    b.planes = planes[j];
+    b.marg = marg[j];
+    b.flags = flags[j];
+    b.n_var = n_var[j];
+    b.n_sel = ns;
+    b.k_pad = k_pad[j];
+    b.nx = ceil_div(n_var[j], PK_V);
+    b.tile0 = t0;
+    b.words = (n_var[j] % 4 == 0) && (reinterpret_cast<uintptr_t>(data[j]) % 4 == 0);
+    b.pad_ = 0;
+    t0 += b.nx * (k_pad[j] / PK_S);
+  }
+  PGLD_REQUIRE(t0 < 0x7fffffff, "pgld_pack_batch_create: too many tiles");
+  auto* pb = new pgld_pack_batch();
+  pb->family = family;
+  pb->fp4 = fp4 != 0;
+  pb->n_jobs = n_jobs;
+  pb->n_tiles = t0;
+  hipError_t e = hipMalloc((void**)&pb->d_jobs, jobs.size() * sizeof(PackJob));
+  if (e == hipSuccess)
+    e = hipMemcpy(pb->d_jobs, jobs.data(), jobs.size() * sizeof(PackJob), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (pb->d_jobs) (void)hipFree(pb->d_jobs);
+    delete pb;
+    set_error("pgld_pack_batch_create: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  *out = pb;
+  return PGLD_OK;
+}
+
+int pgld_pack_batch_run(const pgld_pack_batch* pb, void* stream) {
+  PGLD_REQUIRE(pb && pb->d_jobs, "pgld_pack_batch_run: null batch");
+  hipStream_t s = as_stream(stream);
+  const unsigned clear_blocks = (unsigned)std::min<int64_t>(pb->n_jobs, 4096);
+  if (pb->family == PGLD_FAMILY_HAP)
+    hipLaunchKernelGGL(k_pack_batch_clear<2>, dim3(clear_blocks), dim3(256), 0, s, pb->d_jobs, pb->n_jobs);
+  else
+    hipLaunchKernelGGL(k_pack_batch_clear<3>, dim3(clear_blocks), dim3(256), 0, s, pb->d_jobs, pb->n_jobs);
+  int rc = check_launch("k_pack_batch_clear");
+  if (rc || pb->n_tiles == 0) return rc;
+  const dim3 g((unsigned)pb->n_tiles), b(256);
+  if (pb->family == PGLD_FAMILY_HAP) {
+    if (pb->fp4) hipLaunchKernelGGL((k_pack_batch<PGLD_FAMILY_HAP, true>), g, b, 0, s, pb->d_jobs, pb->n_jobs, pb->n_tiles);
+    else hipLaunchKernelGGL((k_pack_batch<PGLD_FAMILY_HAP, false>), g, b, 0, s, pb->d_jobs, pb->n_jobs, pb->n_tiles);
+  } else {
+    if (pb->fp4) hipLaunchKernelGGL((k_pack_batch<PGLD_FAMILY_GENO, true>), g, b, 0, s, pb->d_jobs, pb->n_jobs, pb->n_tiles);
+    else hipLaunchKernelGGL((k_pack_batch<PGLD_FAMILY_GENO, false>), g, b, 0, s, pb->d_jobs, pb->n_jobs, pb->n_tiles);
+  }
+  return check_launch("k_pack_batch");
+}
+
+int pgld_pack_batch_destroy(pgld_pack_batch* pb) {
+  if (!pb) return PGLD_OK;
+  if (pb->d_jobs) (void)hipFree(pb->d_jobs);
+  delete pb;
+  return PGLD_OK;
 }
 
 int pgld_band_plan_create(const double* pos, int64_t n_var, double max_dist, int32_t key_mode,
@@ -1545,6 +1734,9 @@ static int batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int3
   plan->bound_fp4 = fp4;
   plan->bound_pops = n_pops;
   plan->bound_set = stat_set;
+  double s_sum = 0.0;
+  for (int64_t e = 0; e < (int64_t)R * n_pops; ++e) s_sum += (double)n_samples[e];
+  plan->bound_mean_samples = s_sum / ((double)R * n_pops);
   return PGLD_OK;
 }
 
@@ -1568,67 +1760,169 @@ int pgld_batch_sums(pgld_band_plan* plan, int32_t with_missing, double* sums, do
                   as_stream(stream), false, plan->bound_fp4);
 }
 
-int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split) {
+int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32_t xcd_split,
+                        int32_t kernel_variant) {
   PGLD_REQUIRE(plan && plan->n_regions > 0, "pgld_batch_schedule: not a region batch plan");
   PGLD_REQUIRE(grid >= 0 && order >= 0 && order <= 2, "pgld_batch_schedule: bad grid or order");
+  PGLD_REQUIRE(kernel_variant >= 0 && kernel_variant <= 2, "pgld_batch_schedule: kernel variant must be 0, 1 or 2");
   plan->grid = grid;
   plan->order_mode = order;
   plan->xcd_split = xcd_split != 0;
+  plan->kvar = kernel_variant;
   return PGLD_OK;
 }
 
+}  // extern "C"
+
+namespace pgld {
+
+// Autotuner results, kept for the process and in a text file (one line per
+// shape class: "key grid order xcd_split kernel_variant ms") so that later
+// processes skip the timing.  The file is PGLD_TUNE_CACHE if set (empty: no
+// file), else $XDG_CACHE_HOME/pgld/batch_tune.txt or ~/.cache/pgld/batch_tune.txt.
+struct TuneEntry {
+  int32_t grid, order, xcd, kvar;
+  float ms;
+};
+static std::mutex g_tune_mu;
+static std::map<std::string, TuneEntry> g_tune;
+static bool g_tune_loaded = false;
+
+static std::string tune_file() {
+  const char* e = getenv("PGLD_TUNE_CACHE");
+  if (e) return std::string(e);
+  const char* x = getenv("XDG_CACHE_HOME");
+  std::string dir;
+  if (x && *x) {
+    dir = x;
+  } else {
+    const char* h = getenv("HOME");
+    if (!h || !*h) return "";
+    dir = std::string(h) + "/.cache";
+  }
+  return dir + "/pgld/batch_tune.txt";
+}
+
+static void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const std::string path = tune_file();
+  if (path.empty()) return;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char key[256];
+  TuneEntry t;
+  while (fscanf(f, "%255s %d %d %d %d %f", key, &t.grid, &t.order, &t.xcd, &t.kvar, &t.ms) == 6) {
+    if (t.grid >= 0 && t.order >= 0 && t.order <= 2 && t.kvar >= 0 && t.kvar <= 2) g_tune[key] = t;
+  }
+  fclose(f);
+}
+
+// Rewrites the whole file (write to a temporary, then rename); failures to
+// write only lose the persistence.
+static void tune_store_locked() {
+  const std::string path = tune_file();
+  if (path.empty()) return;
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos && slash > 0) {
+    const std::string dir = path.substr(0, slash);
+    for (size_t i = 1; i <= dir.size(); ++i)
+      if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  }
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  for (const auto& kv : g_tune)
+    fprintf(f, "%s %d %d %d %d %.4f\n", kv.first.c_str(), kv.second.grid, kv.second.order, kv.second.xcd,
+            kv.second.kvar, kv.second.ms);
+  const bool ok = fclose(f) == 0;
+  if (ok) (void)rename(tmp.c_str(), path.c_str());
+  else (void)remove(tmp.c_str());
+}
+
+}  // namespace pgld
+
+extern "C" {
+
 // Autotuner: times every schedule candidate of the persistent launch on this
 // plan (one run each after one warm-up, HIP events) and keeps the fastest.
-// Results are cached per process by (family, populations, set, missing data,
-// tile count and mean K) so a later plan of the same shape class skips timing.
+// Candidates: grid (one workgroup per tile, 1x / 2x / 4x the CU count) x tile
+// order (cost-descending, natural, natural with an XCD split, L2-blocked), and
+// for the kernels that have them the kernel variants (K loop streamed across
+// populations or per population, fast-path rows unrolled or rolled), all with
+// bit-identical sums.  The choice is cached per shape class -- GPU
+// architecture and CU count, family, planes, populations, statistic set,
+// missing data, log2 of the tile count, of the mean region size, of the pairs
+// per SNP (the band width) and of the mean population size -- in the process
+// and on disk.
 int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums, double* key_pairs, void* stream,
-                        int32_t* grid_out, int32_t* order_out, int32_t* xcd_out, float* best_ms) {
+                        int32_t* grid_out, int32_t* order_out, int32_t* xcd_out, int32_t* kvar_out,
+                        float* best_ms) {
   PGLD_REQUIRE(plan && sums, "pgld_batch_autotune: null pointer");
   PGLD_REQUIRE(plan->n_regions > 0 && plan->bound_family >= 0, "pgld_batch_autotune: plan is not a bound batch");
   hipStream_t s = as_stream(stream);
   int dev = 0, cus = 256;
   PGLD_TRY_HIP(hipGetDevice(&dev), "pgld_batch_autotune device");
   PGLD_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "pgld_batch_autotune CUs");
+  hipDeviceProp_t prop;
+  PGLD_TRY_HIP(hipGetDeviceProperties(&prop, dev), "pgld_batch_autotune properties");
+  std::string arch(prop.gcnArchName);
+  for (char& c : arch)
+    if (c == ' ' || c == ':') c = '_';
   double mean_k = 0.0;
   for (int64_t r = 0; r < plan->n_regions; ++r) mean_k += (double)plan->h_region_nvar[r];
-  char key[160];
-  snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%d/%d", plan->bound_family, (int)plan->bound_fp4, plan->bound_pops,
-           plan->bound_set, with_missing != 0, (int)std::log2((double)std::max<int64_t>(1, plan->n_tiles)),
-           (int)std::log2(1.0 + mean_k / plan->n_regions), cus);
-  static std::mutex mu;
-  static std::map<std::string, std::array<int32_t, 3>> cache;
+  mean_k /= (double)plan->n_regions;
+  const double pairs_per_snp = plan->n_var > 0 ? (double)plan->n_pairs / (double)plan->n_var : 0.0;
+  const bool variants = plan->bound_family == PGLD_FAMILY_HAP && plan->bound_set == PGLD_SET_SELF &&
+                        plan->bound_pops > 1 && plan->bound_fp4;
+  char key[256];
+  snprintf(key, sizeof(key), "%s/cu%d/f%d/fp4%d/P%d/set%d/miss%d/t%d/k%d/w%d/s%d", arch.c_str(), cus,
+           plan->bound_family, (int)plan->bound_fp4, plan->bound_pops, plan->bound_set, with_missing != 0,
+           (int)std::log2((double)std::max<int64_t>(1, plan->n_tiles)), (int)std::log2(1.0 + mean_k),
+           (int)std::log2(1.0 + pairs_per_snp), (int)std::log2(1.0 + plan->bound_mean_samples));
+  auto publish = [&](const TuneEntry& t, float ms) {
+    plan->grid = t.grid;
+    plan->order_mode = t.order;
+    plan->xcd_split = t.xcd != 0;
+    plan->kvar = t.kvar;
+    if (grid_out) *grid_out = t.grid;
+    if (order_out) *order_out = t.order;
+    if (xcd_out) *xcd_out = t.xcd;
+    if (kvar_out) *kvar_out = t.kvar;
+    if (best_ms) *best_ms = ms;
+  };
   {
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      plan->grid = it->second[0];
-      plan->order_mode = it->second[1];
-      plan->xcd_split = it->second[2] != 0;
-      if (grid_out) *grid_out = it->second[0];
-      if (order_out) *order_out = it->second[1];
-      if (xcd_out) *xcd_out = it->second[2];
-      if (best_ms) *best_ms = -1.0f;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      publish(it->second, -1.0f);
       return PGLD_OK;
     }
   }
-  struct Cand { int32_t grid, order, xcd; };
-  std::vector<Cand> cands;
+  std::vector<TuneEntry> cands;
   for (int32_t g : {0, cus, 2 * cus, 4 * cus}) {
-    cands.push_back({g, 0, 0});  // cost-descending, strided
-    cands.push_back({g, 1, 0});  // natural, strided
-    cands.push_back({g, 1, 1});  // natural, one contiguous eighth per XCD
-    cands.push_back({g, 2, 0});  // 8 x 8 tile blocks dealt to the XCDs
+    cands.push_back({g, 0, 0, 0, 0.f});  // cost-descending, strided
+    cands.push_back({g, 1, 0, 0, 0.f});  // natural, strided
+    cands.push_back({g, 1, 1, 0, 0.f});  // natural, one contiguous eighth per XCD
+    cands.push_back({g, 2, 0, 0, 0.f});  // 8 x 8 tile blocks dealt to the XCDs
   }
+  if (variants)
+    for (int32_t kv : {1, 2}) {
+      cands.push_back({0, 2, 0, kv, 0.f});
+      cands.push_back({0, 0, 0, kv, 0.f});
+    }
   hipEvent_t e0, e1;
   PGLD_TRY_HIP(hipEventCreate(&e0), "autotune event");
   PGLD_TRY_HIP(hipEventCreate(&e1), "autotune event");
   float best = 1e30f;
-  Cand win = cands[0];
+  TuneEntry win = cands[0];
   int rc = PGLD_OK;
   for (size_t c = 0; c < cands.size() && !rc; ++c) {
     plan->grid = cands[c].grid;
     plan->order_mode = cands[c].order;
     plan->xcd_split = cands[c].xcd != 0;
+    plan->kvar = cands[c].kvar;
     if (c == 0) rc = pgld_batch_sums(plan, with_missing, sums, key_pairs, stream);  // warm-up
     if (!rc) rc = check_hip(hipEventRecord(e0, s), "autotune event");
     if (!rc) rc = pgld_batch_sums(plan, with_missing, sums, key_pairs, stream);
@@ -1644,17 +1938,13 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  plan->grid = win.grid;
-  plan->order_mode = win.order;
-  plan->xcd_split = win.xcd != 0;
+  win.ms = best;
   {
-    std::lock_guard<std::mutex> lk(mu);
-    cache[key] = {win.grid, win.order, win.xcd};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = win;
+    tune_store_locked();
   }
-  if (grid_out) *grid_out = win.grid;
-  if (order_out) *order_out = win.order;
-  if (xcd_out) *xcd_out = win.xcd;
-  if (best_ms) *best_ms = best;
+  publish(win, best);
   return PGLD_OK;
 }
 

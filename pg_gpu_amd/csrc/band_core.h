@@ -50,6 +50,8 @@ struct pgld_band_plan {
   int64_t order_off[3] = {0, 0, 0};    // start of each order in d_order
   int64_t order_len[3] = {0, 0, 0};    // its length (the blocked order holds -1 padding)
   bool xcd_split = false;              // each XCD walks a contiguous eighth of the order
+  int32_t kvar = 0;                    // kernel variant of the fused batch pass (see KV_* in band.hip)
+  double bound_mean_samples = 0.0;     // mean population size of the bound batch (autotuner key)
   int64_t partials_ns = 0;             // statistics + 1 the workspace was sized for
   // per-SNP sums of pair values (single bands): tiles by tile-row / tile-column
   int64_t* d_row_off = nullptr;        // tile-row r -> tiles [row_off[r], row_off[r + 1])

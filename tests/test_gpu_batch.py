@@ -3,6 +3,8 @@ independent regions against the oracle and against the single-band path
 region by region (bit-identical: same tiles, same per-key slot order), under
 every schedule the autotuner may pick, and through a captured HIP graph
 replayed several times."""
+import os
+
 import numpy as np
 import pytest
 
@@ -91,9 +93,11 @@ def test_batch_fp4_equals_int8(small_batch):
     p4 = batch.pack_regions(d_dev, rows_dev, fp4=True)
     b = batch.RegionBatch(pos, 1e6, EDGES).bind(p4, stat_set="self")
     for grid, order in ((0, "blocked"), (256, "cost"), (1024, "natural")):
-        b.schedule(grid, order, False)
-        np.testing.assert_array_equal(np_(b.sums(True)), ref)
-    b.autotune(True)
+        for kv in (0, 1, 2):  # kernel variants: streamed / per-population K loop, rolled rows
+            b.schedule(grid, order, False, kernel_variant=kv)
+            np.testing.assert_array_equal(np_(b.sums(True)), ref)
+    choice = b.autotune(True)
+    assert choice["kernel_variant"] in (0, 1, 2)
     np.testing.assert_array_equal(np_(b.sums(True)), ref)
     mixed = batch.pack_regions(d_dev, rows_dev)
     mixed[0][0] = p4[0][0]
@@ -174,3 +178,92 @@ def test_batch_fp4_multichunk_vs_oracle_and_int8():
     for r in range(3):
         want, scale, _ = oracle_region(datas[r], pos[r], rows[r], 4)
         assert np.all(np.abs(got[r] - want) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300), f"region {r}"
+
+
+def test_autotune_cache_on_disk_serves_a_second_process(tmp_path):
+    """The autotuner's choice is written to its cache file; a second process
+    with the same shape class reads it and times nothing (ms None), and
+    gets the same choice and the same sums."""
+    import json
+    import subprocess
+    import sys
+    cache = str(tmp_path / "tune.txt")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys, json, numpy as np, torch\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "from pg_gpu_amd import batch, synthetic as SY, _lib as L\n"
+        "datas, pos, rows = SY.ragged_regions(seed=34, n_regions=6, m_lo=40, m_hi=300, s_lo=24, s_hi=600, "
+        "span=50_000)\n"
+        "d = [torch.as_tensor(x, device=L.device()) for x in datas]\n"
+        "b = batch.RegionBatch(pos, 1e6, np.logspace(0, 6, 17)).bind(batch.pack_regions(d, rows, fp4=True), "
+        "stat_set='self')\n"
+        "c = b.autotune(True)\n"
+        "s = b.sums(True).cpu().numpy()\n"
+        "print(json.dumps({'choice': c, 'sum': float(np.abs(s).sum()), 'first': float(s.flat[0])}))\n")
+    env = dict(os.environ, PGLD_TUNE_CACHE=cache)
+    out = []
+    for k in range(2):
+        print(f"[batch] autotune cache process {k}", flush=True)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr[-3000:]
+        out.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    assert out[0]["choice"]["ms"] is not None and out[0]["choice"]["ms"] > 0
+    assert out[1]["choice"]["ms"] is None
+    for k in ("grid", "order", "xcd_split", "kernel_variant"):
+        assert out[0]["choice"][k] == out[1]["choice"][k]
+    assert out[0]["sum"] == out[1]["sum"] and out[0]["first"] == out[1]["first"]
+    lines = open(cache).read().split("\n")
+    assert any("/P4/" in ln and "/fp41/" in ln for ln in lines)
+
+
+@pytest.mark.parametrize("family,fp4", [("hap", True), ("hap", False), ("geno", True), ("geno", False)])
+def test_pack_batch_equals_pack_population(family, fp4):
+    """One batched launch packs every population of every region into the
+    same bytes, margins and flags as one pack_population call each (ragged
+    shapes, variant counts not a multiple of 4, a row list of None, missing
+    data and values outside the domain flagged per population); also when
+    replayed from a captured graph."""
+    rng = np.random.default_rng(35 + fp4)
+    dev = L.device()
+    datas, rows = [], []
+    for r, (m, s) in enumerate(((130, 70), (257, 300), (61, 9), (400, 513))):
+        if family == "hap":
+            d = SY.neutral_haplotypes(s, m, rng, missing=0.02 if r != 1 else 0.0)
+        else:
+            d = SY.hwe_genotypes(s, m, rng, missing=0.03 if r != 1 else 0.0)
+        if r == 2:
+            d[0, 3] = 3  # outside the domain of both families
+        datas.append(torch.as_tensor(d, device=dev))
+        cut = sorted(rng.choice(np.arange(1, s), size=2, replace=False))
+        rows.append([None] if r == 3 else [list(range(0, cut[0])), list(range(cut[0], cut[1])),
+                                            list(range(cut[1], s))])
+    ref = [[band.pack_population(d, rw, family, validate=False, fp4=fp4) for rw in rr] for d, rr in zip(datas, rows)]
+    out = [[band.pack_population(d, rw, family, validate=False, fp4=fp4) for rw in rr] for d, rr in zip(datas, rows)]
+    for pl in out:
+        for p in pl:  # stale contents the batch must overwrite
+            p.planes.fill_(7)
+            p.marg.fill_(-5)
+            p.flags.fill_(9)
+    pb = batch.PackBatch(datas, rows, out)
+
+    def check():
+        for a_l, b_l in zip(ref, out):
+            for a, b in zip(a_l, b_l):
+                np.testing.assert_array_equal(np_(b.planes), np_(a.planes))
+                np.testing.assert_array_equal(np_(b.marg), np_(a.marg))
+                np.testing.assert_array_equal(np_(b.flags), np_(a.flags))
+
+    pb.run()
+    check()
+    for pl in out:
+        for p in pl:
+            p.marg.fill_(-5)
+            p.flags.fill_(9)
+    g = batch.Graph().capture(pb.run)
+    for _ in range(3):
+        g.launch()
+        torch.cuda.synchronize()
+        check()
+        out[0][0].marg.fill_(11)  # eager work between replays
+    assert int(np_(ref[2][0].flags)[1]) == 1 and int(np_(ref[1][0].flags)[0]) == 0
