@@ -99,10 +99,23 @@ constexpr int band_min_waves(int family, int P, int set) {
 // region * n_keys_local for the tiles of a region batch.  The pair sums are
 // folded over the wave by the halving butterfly in groups of SCAT values
 // (more values per group: fewer shuffles, more live registers).
-// Kernel variants (KV, bit-identical results; the batch autotuner chooses):
-// bit 0 set = each population's K loop on its own (no load stream across
-// populations), bit 1 set = fast-path rows in a rolled loop.
-constexpr int KV_PER_POP = 1, KV_ROLLED = 2;
+// Kernel variants (KV, bit-identical results; the batch autotuner chooses
+// bits 0 and 1): bit 0 set = each population's K loop on its own (no load
+// stream across populations), bit 1 set = fast-path rows in a rolled loop,
+// bit 2 set = the within-population haplotype sets with missing data read
+// 1 / (n)_4 from an LDS table (BandArgs::rtab_*) instead of computing it per
+// pair (set by the launcher when the table fits).
+constexpr int KV_PER_POP = 1, KV_ROLLED = 2, KV_RTAB = 4;
+
+// Fills the reciprocal table of a KV_RTAB launch; the workgroup barrier that
+// follows the tile set-up orders it before the first read.
+template <int KV>
+__device__ __forceinline__ void fill_recip_table(unsigned char* smem, const BandArgs& ba) {
+  if constexpr ((KV & KV_RTAB) != 0) {
+    double* tab = reinterpret_cast<double*>(smem + ba.rtab_off);
+    for (int x = threadIdx.x; x <= ba.rtab_n; x += BAND_THREADS) tab[x] = hap_self_recip((double)x);
+  }
+}
 
 template <int FAMILY, bool MISSING, int P, int SET, int SCAT = 8, bool NIB = false, int KV = 0>
 __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs& pa, const BandTile& tile,
@@ -114,6 +127,15 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // accumulators (hap_self_stats); their fast-path rows are unrolled
   constexpr bool HAP_SELF = FAMILY == PGLD_FAMILY_HAP && (SET == PGLD_SET_SELF || P == 1);
   constexpr bool FAST_UNROLL = HAP_SELF && !(KV & KV_ROLLED);
+  static_assert(!(KV & KV_RTAB) || (HAP_SELF && MISSING), "reciprocal table: within-population sets, missing data");
+  // 1 / (n)_4 of a pair with n valid samples (n <= the population's size <= rtab_n)
+  const double* rtab = reinterpret_cast<const double*>(smem + ba.rtab_off);
+  auto self_recip = [&](double n) -> double {
+    if constexpr ((KV & KV_RTAB) != 0)
+      return rtab[(int)n];
+    else
+      return hap_self_recip(n);
+  };
   constexpr int NS = SS::N + 1;  // statistics + pair count
   // smem layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
@@ -326,7 +348,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
           if constexpr (HAP_SELF) {
             double n, a, b, s;
             hap_moments<MISSING>(acc, r, pa.marg[p], i, j, pa.n_samples[p], n, a, b, s);
-            hap_self_stats(n, a, b, s, out);
+            hap_self_stats(n, a, b, s, self_recip(n), out);
           } else {
             int c[NC];
             elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, i, j, pa.n_samples[p], c);
@@ -359,7 +381,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         if constexpr (HAP_SELF && P == 1) {
           double n, a, b, s;
           hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
-          hap_self_stats(n, a, b, s, out);
+          hap_self_stats(n, a, b, s, self_recip(n), out);
         } else {
           typename PopVal<FAMILY>::T V[P];
 #pragma unroll
@@ -394,14 +416,15 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   }
 }
 
-template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
+template <int FAMILY, bool MISSING, int P, int SET, bool NIB, int KV = 0>
 __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
                                                                                             PopArgs pa) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
   const BandTile tile = ba.tiles[ba.order ? ba.order[t] : t];
   if (tile.n_slots <= 0) return;
-  band_sums_tile<FAMILY, MISSING, P, SET, 8, NIB>(ba, pa, tile, 0, smem);
+  fill_recip_table<KV>(smem, ba);
+  band_sums_tile<FAMILY, MISSING, P, SET, 8, NIB, KV>(ba, pa, tile, 0, smem);
 }
 
 // Region batch, persistent: workgroup b takes tiles order[b], order[b + grid],
@@ -431,6 +454,7 @@ __global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void
     step = gridDim.x / N_XCD;
     t_end = n_tiles * (x + 1) / N_XCD;
   }
+  fill_recip_table<KV>(smem, ba);  // once per workgroup: the table outlives the tiles
   for (int64_t t = t0; t < t_end; t += step) {
     const int32_t ot = order[t];
     if (ot < 0) continue;  // padding of the XCD-blocked order
@@ -1059,9 +1083,37 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   return rc;
 }
 
+// Whether the reciprocal table (tab bytes) fits beside a workgroup's lds
+// bytes without lowering the workgroups per CU below min(current, 2).
+static bool recip_table_fits(size_t lds, size_t tab) {
+  const size_t cu = 160 * 1024, base = lds + STATIC_LDS_BYTES, with = base + tab;
+  return with <= cu && cu / with >= std::min<size_t>(cu / base, 2);
+}
+
 template <int FAMILY, bool MISSING, int P, int SET, bool NIB>
-static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const PopArgs& pa, size_t lds,
+static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const PopArgs& pa, size_t lds,
                         hipStream_t s) {
+  BandArgs ba = ba_in;
+  ba.rtab_n = -1;
+  ba.rtab_off = 0;
+  // the within-population haplotype sets with missing data take 1 / (n)_4
+  // from an LDS table of n = 0 .. the largest population when it fits
+  constexpr bool TAB = FAMILY == PGLD_FAMILY_HAP && MISSING && (SET == PGLD_SET_SELF || P == 1);
+  bool tab = false;
+  if constexpr (TAB) {
+    int smax = 0;
+    if (plan->n_regions > 0)
+      smax = plan->bound_max_samples;
+    else
+      for (int p = 0; p < P; ++p) smax = std::max(smax, pa.n_samples[p]);
+    const size_t bytes = (((size_t)smax + 1) * 8 + 15) / 16 * 16;
+    if (recip_table_fits(lds, bytes)) {
+      tab = true;
+      ba.rtab_n = smax;
+      ba.rtab_off = (int32_t)lds;
+      lds += bytes;
+    }
+  }
   if (plan->n_regions > 0) {
     const int m = plan->order_mode;
     const int64_t len = plan->order_len[m];
@@ -1070,24 +1122,38 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba, const Po
     const int32_t* ord = plan->d_order + plan->order_off[m];
     const int32_t xs = (int32_t)plan->xcd_split;
     const dim3 g((unsigned)grid), b(BAND_THREADS);
-    // kernel variants exist for the FP4 within-population haplotype sets
+    auto go = [&](auto kv_tag) {
+      constexpr int KV = decltype(kv_tag)::value;
+      hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV>), g, b, lds, s, ba, rd, ord, len, xs);
+    };
+    // the autotuned variants (plan->kvar) exist for the FP4 within-population
+    // haplotype sets; with missing data only in the table form
     constexpr bool VARIANTS = FAMILY == PGLD_FAMILY_HAP && SET == PGLD_SET_SELF && P > 1 && NIB;
-    if constexpr (VARIANTS) {
-      if (plan->kvar == KV_PER_POP) {
-        hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV_PER_POP>), g, b, lds, s, ba, rd, ord, len,
-                           xs);
-        return;
+    if constexpr (TAB) {
+      if (tab) {
+        if constexpr (VARIANTS) {
+          if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_RTAB | KV_PER_POP>{});
+          if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_RTAB | KV_ROLLED>{});
+        }
+        return go(std::integral_constant<int, KV_RTAB>{});
       }
-      if (plan->kvar == KV_ROLLED) {
-        hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV_ROLLED>), g, b, lds, s, ba, rd, ord, len,
-                           xs);
+      return go(std::integral_constant<int, 0>{});
+    } else {
+      if constexpr (VARIANTS) {
+        if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_PER_POP>{});
+        if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_ROLLED>{});
+      }
+      return go(std::integral_constant<int, 0>{});
+    }
+  } else {
+    const dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
+    if constexpr (TAB) {
+      if (tab) {
+        hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB, KV_RTAB>), g, b, lds, s, ba, pa);
         return;
       }
     }
-    hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB>), g, b, lds, s, ba, rd, ord, len, xs);
-  } else {
-    hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB>), dim3((unsigned)plan->n_tiles),
-                       dim3(BAND_THREADS), lds, s, ba, pa);
+    hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB>), g, b, lds, s, ba, pa);
   }
 }
 
@@ -1737,6 +1803,7 @@ static int batch_bind(pgld_band_plan* plan, int32_t family, int32_t n_pops, int3
   double s_sum = 0.0;
   for (int64_t e = 0; e < (int64_t)R * n_pops; ++e) s_sum += (double)n_samples[e];
   plan->bound_mean_samples = s_sum / ((double)R * n_pops);
+  plan->bound_max_samples = *std::max_element(n_samples, n_samples + (size_t)R * n_pops);
   return PGLD_OK;
 }
 

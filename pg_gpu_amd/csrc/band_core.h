@@ -52,6 +52,7 @@ struct pgld_band_plan {
   bool xcd_split = false;              // each XCD walks a contiguous eighth of the order
   int32_t kvar = 0;                    // kernel variant of the fused batch pass (see KV_* in band.hip)
   double bound_mean_samples = 0.0;     // mean population size of the bound batch (autotuner key)
+  int32_t bound_max_samples = 0;       // largest population of the bound batch
   int64_t partials_ns = 0;             // statistics + 1 the workspace was sized for
   // per-SNP sums of pair values (single bands): tiles by tile-row / tile-column
   int64_t* d_row_off = nullptr;        // tile-row r -> tiles [row_off[r], row_off[r + 1])
@@ -573,15 +574,51 @@ __device__ __forceinline__ void hap_moments(const ACC& acc, int r, const int32_t
 
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
+// Lane exchanges of doubles without the LDS crossbar (ds_bpermute would
+// queue behind the operand traffic of the K loops).  DPP moves within 16-lane
+// rows: row_mirror (lane i <- 15 - i), row_half_mirror (i <- 7 - i within
+// eight), quad_perm [2,3,0,1] and [1,0,3,2].
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141, DPP_QUAD_2301 = 0x4E, DPP_QUAD_1032 = 0xB1;
+
+// Halving exchange across lane bit O = 32 or 16 (v_permlane32_swap /
+// v_permlane16_swap swap the odd 32- or 16-lane rows of the first operand
+// with the even rows of the second): returns x(l) + x(l ^ O) in lanes with
+// bit O clear and y(l ^ O) + y(l) in the others.  With x = y every lane gets
+// the pair total, the same bits in both lanes of a pair.
+template <int O>
+__device__ __forceinline__ double swap_sum(double x, double y) {
+  static_assert(O == 32 || O == 16, "permlane swap width");
+  const unsigned xl = (unsigned)__double2loint(x), xh = (unsigned)__double2hiint(x);
+  const unsigned yl = (unsigned)__double2loint(y), yh = (unsigned)__double2hiint(y);
+  if constexpr (O == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+  }
+}
+
 // Wave-wide totals of NV per-lane values, in groups of CH = min(CHMAX, NV
-// rounded up to a power of two).  Each group goes through a halving butterfly: at the
-// level of lane bit o (32, 16, ...) a lane keeps one half of its values, adds
-// the partner lane's copy of that half and hands over the other half, so a
-// group costs CH - 1 + (6 - log2 CH) shuffles instead of 6 CH.  Afterwards
-// lane l holds the total of value base + ((l >> (6 - log2 CH)) & (CH - 1)),
-// identical in the 64 / CH lanes that differ in the low bits; the lane whose
-// low bits are zero calls put(index, total).  The summation tree is fixed, so
-// the totals are deterministic.
+// rounded up to a power of two).  Each group goes through a halving
+// butterfly over lane bits 32, 16, 8, 4, 2, 1: at each of the first log2 CH
+// levels a lane keeps one half of its values, adds the partner lane's copy
+// of that half and hands over the other half (the partner differs in that
+// lane bit: l ^ 32, l ^ 16, then 15 - l within the 16-lane row), so a group
+// costs CH - 1 + (6 - log2 CH) exchanges instead of 6 CH; the remaining
+// levels add full totals (partners 15 - l, 7 - l within eight, l ^ 2,
+// l ^ 1).  Afterwards lane l holds the total of value base + ((l >> (6 -
+// log2 CH)) & (CH - 1)), identical in the 64 / CH lanes that differ in the
+// low bits; the lane whose low bits are zero calls put(index, total).  The
+// summation tree is fixed, so the totals are deterministic.
 template <int NV, int CHMAX = 8, typename F>
 __device__ __forceinline__ void wave_sums_scatter(const double* vals, int lane, F&& put) {
   static_assert(CHMAX == 8 || CHMAX == 4 || CHMAX == 2 || CHMAX == 1, "group size");
@@ -592,22 +629,32 @@ __device__ __forceinline__ void wave_sums_scatter(const double* vals, int lane, 
     double v[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) v[k] = base + k < NV ? vals[base + k] : 0.0;
+    // level 0: lane bit 32
+    if constexpr (LG > 0) {
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-      const int o = 32 >> t;
-      if (t < LG) {
-        const int h = CH >> (t + 1);
-        const bool up = (lane & o) != 0;
-#pragma unroll
-        for (int k = 0; k < h; ++k) {
-          const double send = up ? v[k] : v[k + h];
-          const double keep = up ? v[k + h] : v[k];
-          v[k] = keep + __shfl_xor(send, o);
-        }
-      } else {
-        v[0] += __shfl_xor(v[0], o);
-      }
+      for (int k = 0; k < CH / 2; ++k) v[k] = swap_sum<32>(v[k], v[k + CH / 2]);
+    } else {
+      v[0] = swap_sum<32>(v[0], v[0]);
     }
+    // level 1: lane bit 16
+    if constexpr (LG > 1) {
+#pragma unroll
+      for (int k = 0; k < CH / 4; ++k) v[k] = swap_sum<16>(v[k], v[k + CH / 4]);
+    } else {
+      v[0] = swap_sum<16>(v[0], v[0]);
+    }
+    // level 2: lane bit 8 (partner 15 - l within the row)
+    if constexpr (LG > 2) {
+      const bool up = (lane & 8) != 0;
+      const double send = up ? v[0] : v[1];
+      const double keep = up ? v[1] : v[0];
+      v[0] = keep + dpp_f64<DPP_ROW_MIRROR>(send);
+    } else {
+      v[0] += dpp_f64<DPP_ROW_MIRROR>(v[0]);
+    }
+    v[0] += dpp_f64<DPP_ROW_HALF_MIRROR>(v[0]);
+    v[0] += dpp_f64<DPP_QUAD_2301>(v[0]);
+    v[0] += dpp_f64<DPP_QUAD_1032>(v[0]);
     const int idx = base + ((lane >> (6 - LG)) & (CH - 1));
     if ((lane & ((64 >> LG) - 1)) == 0 && idx < NV) put(idx, v[0]);
   }
@@ -663,6 +710,10 @@ struct BandArgs {
   int32_t* counts_out;       // counts mode
   const int64_t* pair_base;  // counts mode
   const int32_t* order;      // fused sums of a single band: tile launch order (null: list order)
+  // reciprocal table of the within-population haplotype sets with missing
+  // data: 1 / (n)_4 for n = 0 .. rtab_n as doubles at byte rtab_off of the
+  // dynamic LDS (kernel variants with KV_RTAB only)
+  int32_t rtab_n, rtab_off;
 };
 
 }  // namespace pgld

@@ -208,34 +208,39 @@ PGLD_HD double hap_pi2_alldiff(const HapP& a, const HapP& b, const HapP& c, cons
 
 // The within-population haplotype set (DD_i_i, Dz_i_i_i, pi2_i_i_i_i) of one
 // pair straight from its centred moments n, a = Sa, b = Sb, s = Sab, with
-// m = n - 2 and w = n^2 - a^2 - b^2 shared by the three numerators:
+// m = n - 2, t = ab s, q = (ab)^2 and w = n^2 - a^2 - b^2 shared by the
+// three numerators:
 //   poly_dd_single  = (n-1)((n-2) s^2 - 2t) + (n-2)(a^2 + b^2 - n^2) + a^2 b^2
-//                   = m (e - w) + e + q,         e = m s^2 - 2t, t = ab s, q = (ab)^2
+//                   = (m + 1)(m s^2 - 2t) - m w + q
 //   poly_dz_iii     = (n+1) t - (n-1) s^2 + w - q
-//   poly_pi2_iiii   = (m^2 - 2) w + q - 4t + 2 s^2
-// and the common reciprocal 1 / (n)_4 = 1 / (h (h - 2)), h = m^2 + m.  Every
-// intermediate is an integer, so the numerators are exact (terms below 2^53)
-// and equal to the poly_* forms above.  out[] receives the unscaled numerators
-// times 1 / (n)_4 (0 when n < 4); the factors 1/16, 1/4, 1/16 are applied to
-// the sums.
-__device__ __forceinline__ void hap_self_stats(double n, double a, double b, double s, double* out) {
+//   poly_pi2_iiii   = (m^2 - 2) w + q + 2 (s^2 - 2t)
+// Every intermediate is an integer, so the numerators are exact (terms below
+// 2^53) and equal to the poly_* forms above.  out[] receives the unscaled
+// numerators times r = 1 / (n)_4 (hap_self_recip); the factors 1/16, 1/4,
+// 1/16 are applied to the sums.
+__device__ __forceinline__ void hap_self_stats(double n, double a, double b, double s, double r, double* out) {
+  const double m = n - 2.0, m1 = n - 1.0;
   const double ab = a * b, s2 = s * s, t = ab * s, q = ab * ab;
   const double w = fma(-b, b, fma(-a, a, n * n));
-  const double m = n - 2.0;
-  const double e = fma(m, s2, -2.0 * t);
-  const double dd = fma(m, e - w, e + q);
-  const double dz = fma(1.0 - n, s2, fma(n + 1.0, t, w - q));
-  const double pi2 = fma(fma(m, m, -2.0), w, fma(-4.0, t, fma(2.0, s2, q)));
-  const double h = fma(m, m, m);
-  const double g4 = fma(h, h, -2.0 * h);
+  const double t2 = -2.0 * t;
+  out[0] = fma(m1, fma(m, s2, t2), fma(-m, w, q)) * r;
+  out[1] = fma(-m1, s2, fma(n + 1.0, t, w - q)) * r;
+  out[2] = fma(fma(m, m, -2.0), w, fma(2.0, s2 + t2, q)) * r;
+}
+
+// 1 / (n)_4 = 1 / (h (h - 2)), h = (n - 1)(n - 2): v_rcp_f64 refined by one
+// Newton step (within an ulp), 0 when n < 4 (the reference's "invalid
+// denominator -> 0").  The fused kernels either call it per pair or read it
+// from a table of n = 0 .. N filled by the same function, so both give the
+// same bits.
+__device__ __forceinline__ double hap_self_recip(double n) {
+  const double m = n - 2.0, h = fma(m, m, m), g4 = fma(h, h, -2.0 * h);
   double r = 0.0;
   if (g4 > 0.0) {
     r = __builtin_amdgcn_rcp(g4);
     r = fma(fma(-g4, r, 1.0), r, r);
   }
-  out[0] = dd * r;
-  out[1] = dz * r;
-  out[2] = pi2 * r;
+  return r;
 }
 
 // Reciprocal form for the fused epilogue: the four falling-factorial
