@@ -225,7 +225,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       unsigned long long part = __ballot(flag);
       while (part) {
         const int leader = __ffsll((long long)part) - 1;
-        const int kk = __shfl(ck, leader);
+        const int kk = __builtin_amdgcn_readlane(ck, leader);  // leader is wave-uniform
         const bool mine = flag && ck == kk;
         part &= ~__ballot(mine);
         const int slot = kk - tile.key_lo;
@@ -271,9 +271,18 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
 #pragma unroll 1
         for (int r = 0; r < 16; ++r) row(r);
       }
+      // This wave's row for fast_key takes exactly one flush per epilogue
+      // call and the calls write disjoint columns, so the totals are stored
+      // without reading the row back (x + 0.0 is what adding x into the
+      // cleared row gives).
       lacc[M] = 16.0;
-      ck = fast_key + key_base;
-      flush(true);
+      double* frow = wred + (fast_key - klo) * NS;
+      if (count)
+        wave_sums_scatter<M + 1, SCAT>(lacc, lane, [&](int s, double x) {
+          frow[s < M ? col0 + s * cstride : NS - 1] = x + 0.0;
+        });
+      else
+        wave_sums_scatter<M, SCAT>(lacc, lane, [&](int s, double x) { frow[col0 + s * cstride] = x + 0.0; });
       return;
     }
 #pragma unroll 1
