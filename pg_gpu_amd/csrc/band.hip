@@ -265,7 +265,13 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           row(r);
-          if (r & 1) __builtin_amdgcn_sched_barrier(0);
+          if (r & 1) {
+            // the running sums are complete here (an empty asm operand keeps
+            // the compiler from sinking the rows' last products to the end)
+#pragma unroll
+            for (int q = 0; q < M; ++q) asm volatile("" : "+v"(lacc[q]));
+            __builtin_amdgcn_sched_barrier(0);
+          }
         }
       } else {
 #pragma unroll 1
