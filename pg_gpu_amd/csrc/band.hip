@@ -99,6 +99,9 @@ constexpr int band_min_waves(int family, int P, int set) {
 // region * n_keys_local for the tiles of a region batch.  The pair sums are
 // folded over the wave by the halving butterfly in groups of SCAT values
 // (more values per group: fewer shuffles, more live registers).
+#ifndef PGLD_HAP2_GROUP
+#define PGLD_HAP2_GROUP 1
+#endif
 // Kernel variants (KV, bit-identical results; the batch autotuner chooses
 // bits 0 and 1): bit 0 set = each population's K loop on its own (no load
 // stream across populations), bit 1 set = fast-path rows in a rolled loop,
@@ -126,7 +129,12 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // haplotype within-population sets evaluate the moments straight from the
   // accumulators (hap_self_stats); their fast-path rows are unrolled
   constexpr bool HAP_SELF = FAMILY == PGLD_FAMILY_HAP && (SET == PGLD_SET_SELF || P == 1);
-  constexpr bool FAST_UNROLL = HAP_SELF && !(KV & KV_ROLLED);
+  // the two-population haplotype moments set evaluates from centred moments
+  // (hap_two_pop_stats) rather than from the joint counts
+  constexpr bool HAP2 = FAMILY == PGLD_FAMILY_HAP && P == 2 && SET == PGLD_SET_MOMENTS;
+  constexpr bool FAST_UNROLL = (HAP_SELF || HAP2) && !(KV & KV_ROLLED);
+  // rows per scheduling group of the unrolled fast path
+  constexpr int FAST_GROUP = HAP2 ? PGLD_HAP2_GROUP : 2;
   static_assert(!(KV & KV_RTAB) || (HAP_SELF && MISSING), "reciprocal table: within-population sets, missing data");
   // 1 / (n)_4 of a pair with n valid samples (n <= the population's size <= rtab_n)
   const double* rtab = reinterpret_cast<const double*>(smem + ba.rtab_off);
@@ -265,7 +273,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           row(r);
-          if (r & 1) {
+          if ((r + 1) % FAST_GROUP == 0) {
             // the running sums are complete here (an empty asm operand keeps
             // the compiler from sinking the rows' last products to the end)
 #pragma unroll
@@ -374,19 +382,42 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     }
   } else {
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
+    // haplotype populations form one load stream (PopStream): the next
+    // population's first operand chunks are in flight while this one's
+    // counts are parked
+    constexpr bool STREAM = FAMILY == PGLD_FAMILY_HAP && P > 1;
+    using PS = PopStream<FAMILY, MISSING, NIB>;
+    v4i pre0[STREAM ? PS::ITEMS : 1], pre1[STREAM ? PS::ITEMS : 1];
+    bool pre1_ok = false;
+    if constexpr (STREAM) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
-                                                  wave_live, opbuf, acc);
+      if constexpr (STREAM)
+        PS::template products<P>(pa, p, ba.n_var, tile.i0, tile.j0, wr, wc, wave_live, opbuf, acc, pre0, pre1,
+                                 pre1_ok);
+      else
+        block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
+                                                    wave_live, opbuf, acc);
       if (p == P - 1 || !wave_live) continue;
-      // all populations but the last park their counts in LDS as int16
+      // all populations but the last park their counts in LDS as int16 (the
+      // two-population haplotype set parks the centred moments n, Sa, Sb, Sab)
       const int64_t jj = rj + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t i = ri + acc_row(r, lane);
         int c[NC];
-        elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(jj, ba.n_var - 1),
-                                     pa.n_samples[p], c);
+        if constexpr (HAP2) {
+          double n, a, b, s;
+          hap_moments<MISSING>(acc, r, pa.marg[p], min(i, ba.n_var - 1), min(jj, ba.n_var - 1), pa.n_samples[p], n,
+                               a, b, s);
+          c[0] = (int)n;
+          c[1] = (int)a;
+          c[2] = (int)b;
+          c[3] = (int)s;
+        } else {
+          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[p], ba.n_var, min(i, ba.n_var - 1), min(jj, ba.n_var - 1),
+                                       pa.n_samples[p], c);
+        }
 #pragma unroll
         for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
       }
@@ -397,6 +428,15 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
           double n, a, b, s;
           hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
           hap_self_stats(n, a, b, s, self_recip(n), out);
+        } else if constexpr (HAP2) {
+          double n[2], a[2], b[2], s[2];
+          const int16_t* st = stash + (((w * NC) * 16 + r) << 6) + lane;
+          n[0] = st[0];
+          a[0] = st[16 * 64];
+          b[0] = st[2 * 16 * 64];
+          s[0] = st[3 * 16 * 64];
+          hap_moments<MISSING>(acc, r, pa.marg[1], i, j, pa.n_samples[1], n[1], a[1], b[1], s[1]);
+          hap_two_pop_stats(n, a, b, s, out);
         } else {
           typename PopVal<FAMILY>::T V[P];
 #pragma unroll

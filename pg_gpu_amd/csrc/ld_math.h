@@ -270,6 +270,65 @@ __device__ __forceinline__ HapR hap_recips(double n) {
   return r;
 }
 
+// The two-population haplotype moments set (15 statistics in the order and
+// with the power-of-two factors of epi_hap_moments_2) straight from each
+// population's centred moments n, a = Sa, b = Sb, s = Sab, sharing the
+// products the polynomials have in common: with ab_k = a_k b_k, Dp_k = n_k s_k
+// - ab_k, u = a0 b1 + a1 b0, X = n0 n1 - b0 b1 and w_k = n_k^2 - a_k^2 - b_k^2,
+//   poly_dd_between    = Dp0 Dp1
+//   poly_dz_iij + iji  = Dp0 u           (populations (0, 1); (1, 0): Dp1 u)
+//   poly_dz_ijj(0, 1)  = Dp0 (ab1 - s1)
+//   poly_pi2_triple(0, 1) = (n0 - 2)(n0 X - n1 a0^2) + (a0 b1)(ab0 - 2 s0)
+//   poly_pi2_iikk_sym  = n0^2 w1 + n1^2 w0 + (a0 b1)^2 + (a1 b0)^2
+//   poly_pi2_ijij      = (n0 - 1)(n1 - 1)(X - a0 a1) + a0 a1 b0 b1 - ab0 s1 - ab1 s0 + s0 s1
+// and the within-population numerators of hap_self_stats.  Every
+// intermediate is an integer, so the numerators equal the poly_* forms
+// exactly (terms below 2^53) and the statistics are bit-identical to
+// epi_hap_moments_2; the reciprocals are hap_recips'.
+__device__ __forceinline__ void hap_two_pop_stats(const double* n, const double* a, const double* b, const double* s,
+                                                  double* out) {
+  double ab[2], w[2], nn[2], m[2], m1[2], Dp[2], self[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    m[k] = n[k] - 2.0;
+    m1[k] = n[k] - 1.0;
+    ab[k] = a[k] * b[k];
+    nn[k] = n[k] * n[k];
+    const double s2 = s[k] * s[k], t = ab[k] * s[k], q = ab[k] * ab[k];
+    w[k] = fma(-b[k], b[k], fma(-a[k], a[k], nn[k]));
+    const double t2 = -2.0 * t;
+    self[k][0] = fma(m1[k], fma(m[k], s2, t2), fma(-m[k], w[k], q));
+    self[k][1] = fma(-m1[k], s2, fma(n[k] + 1.0, t, w[k] - q));
+    self[k][2] = fma(fma(m[k], m[k], -2.0), w[k], fma(2.0, s2 + t2, q));
+    Dp[k] = fma(n[k], s[k], -ab[k]);
+  }
+  const HapR R0 = hap_recips(n[0]), R1 = hap_recips(n[1]);
+  const double r1 = R0.i2 * R1.i2, r3 = R0.i3 * R1.i1, r4 = R0.i1 * R1.i3;
+  const double a0b1 = a[0] * b[1], a1b0 = a[1] * b[0];
+  const double u = a0b1 + a1b0;
+  const double X = fma(-b[0], b[1], n[0] * n[1]);
+  const double P = a[0] * a[1], Q = b[0] * b[1];
+  const double tri01 = fma(m[0], fma(n[0], X, -n[1] * (a[0] * a[0])), a0b1 * fma(-2.0, s[0], ab[0]));
+  const double tri10 = fma(m[1], fma(n[1], X, -n[0] * (a[1] * a[1])), a1b0 * fma(-2.0, s[1], ab[1]));
+  const double iikk = fma(nn[0], w[1], fma(nn[1], w[0], fma(a0b1, a0b1, a1b0 * a1b0)));
+  const double ijij = fma(m1[0] * m1[1], X - P, fma(P, Q, fma(-ab[0], s[1], fma(-ab[1], s[0], s[0] * s[1]))));
+  out[0] = self[0][0] * R0.i4;
+  out[1] = (Dp[0] * Dp[1]) * r1;
+  out[2] = self[1][0] * R1.i4;
+  out[3] = self[0][1] * R0.i4;
+  out[4] = (Dp[0] * u) * r3;
+  out[5] = (Dp[0] * (ab[1] - s[1])) * r1;
+  out[6] = (Dp[1] * (ab[0] - s[0])) * r1;
+  out[7] = (Dp[1] * u) * r4;
+  out[8] = self[1][1] * R1.i4;
+  out[9] = self[0][2] * R0.i4;
+  out[10] = tri01 * r3;
+  out[11] = iikk * r1;
+  out[12] = ijij * r1;
+  out[13] = tri10 * r4;
+  out[14] = self[1][2] * R1.i4;
+}
+
 // ---------------------------------------------------------------------------
 // Genotype per-population values.  Counts in moments order g[0..8] =
 // g1..g9 = n22, n21, n20, n12, n11, n10, n02, n01, n00
