@@ -155,21 +155,20 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   const int red_n = 4 * tile.n_slots * NS;
   int16_t* stash = reinterpret_cast<int16_t*>(smem + ((red_n * 8 + 15) / 16) * 16);
 
-  for (int t = tid; t < red_n; t += BAND_THREADS) red[t] = 0.0;
+  // The tile's positions, key coordinates and bin edges are loaded into
+  // registers, the population stream's first operand chunks right behind
+  // them (both latencies overlap), and then copied to LDS.
+  double v_pi = 0.0, v_pj = 0.0, v_ki = 0.0, v_kj = 0.0, v_e = 0.0;
   if (tid < BT) {
-    int64_t gi = min((int64_t)tile.i0 + tid, ba.n_var - 1), gj = min((int64_t)tile.j0 + tid, ba.n_var - 1);
-    s_pos_i[tid] = ba.pos[gi];
-    s_pos_j[tid] = ba.pos[gj];
-    s_kc_i[tid] = ba.kc[gi];
-    s_kc_j[tid] = ba.kc[gj];
-    if (ba.key_mode == PGLD_KEY_WINDOW) {
-      const double wk = floor((ba.kc[gi] - ba.origin) / ba.width);
-      s_wkey[tid] = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
-    }
+    const int64_t gi = min((int64_t)tile.i0 + tid, ba.n_var - 1), gj = min((int64_t)tile.j0 + tid, ba.n_var - 1);
+    v_pi = ba.pos[gi];
+    v_pj = ba.pos[gj];
+    v_ki = ba.kc[gi];
+    v_kj = ba.kc[gj];
   }
+  static_assert(MAX_LDS_EDGES <= BAND_THREADS, "one edge per thread");
   const bool edges_in_lds = ba.key_mode != PGLD_KEY_WINDOW && ba.n_keys < MAX_LDS_EDGES;
-  if (edges_in_lds)
-    for (int t = tid; t <= ba.n_keys; t += BAND_THREADS) s_edges[t] = ba.edges[t];
+  if (edges_in_lds && tid <= ba.n_keys) v_e = ba.edges[tid];
   // Every band pair of the tile has its bin in the tile's key range [klo, khi]
   // (the plan bounds it by the tile's smallest and largest distance) or none:
   // only those bins are searched, and a one-bin tile needs two compares.
@@ -179,6 +178,27 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     e_lo = ba.edges[klo];
     e_hi = ba.edges[khi + 1];
   }
+  // sets with several populations run their K loops as one load stream
+  // across the populations (PopStream): every within-population set (except
+  // the per-population kernel variant) and the haplotype moments sets
+  using PS = PopStream<FAMILY, MISSING, NIB>;
+  constexpr bool STREAM =
+      P > 1 && (SET == PGLD_SET_SELF ? !(KV & KV_PER_POP) : FAMILY == PGLD_FAMILY_HAP);
+  v4i pre0[STREAM ? PS::ITEMS : 1], pre1[STREAM ? PS::ITEMS : 1];
+  bool pre1_ok = false;
+  if constexpr (STREAM) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
+  for (int t = tid; t < red_n; t += BAND_THREADS) red[t] = 0.0;
+  if (tid < BT) {
+    s_pos_i[tid] = v_pi;
+    s_pos_j[tid] = v_pj;
+    s_kc_i[tid] = v_ki;
+    s_kc_j[tid] = v_kj;
+    if (ba.key_mode == PGLD_KEY_WINDOW) {
+      const double wk = floor((v_ki - ba.origin) / ba.width);
+      s_wkey[tid] = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
+    }
+  }
+  if (edges_in_lds && tid <= ba.n_keys) s_edges[tid] = v_e;
   __syncthreads();
 
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -352,10 +372,6 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     // epilogue.
     using S1 = StatSet<FAMILY, 1, PGLD_SET_SELF>;
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
-    using PS = PopStream<FAMILY, MISSING, NIB>;
-    v4i pre0[PS::ITEMS], pre1[PS::ITEMS];
-    bool pre1_ok;
-    if constexpr (!(KV & KV_PER_POP)) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
       if constexpr (KV & KV_PER_POP)
@@ -382,14 +398,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     }
   } else {
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
-    // haplotype populations form one load stream (PopStream): the next
-    // population's first operand chunks are in flight while this one's
-    // counts are parked
-    constexpr bool STREAM = FAMILY == PGLD_FAMILY_HAP && P > 1;
-    using PS = PopStream<FAMILY, MISSING, NIB>;
-    v4i pre0[STREAM ? PS::ITEMS : 1], pre1[STREAM ? PS::ITEMS : 1];
-    bool pre1_ok = false;
-    if constexpr (STREAM) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
+    // with the load stream the next population's first operand chunks are in
+    // flight while this one's counts are parked
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
       if constexpr (STREAM)
