@@ -45,6 +45,29 @@ PEAK_HBM = 8.0e12        # bytes/s
 # 1.000): FP4 planes 158,867 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
 # Summary and commands: profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
 TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3294e9, "int8": 1.581e9}
+# The same for k_batch_sums on the 512-region ragged batch (FP4 planes):
+# FETCH_SIZE 2.35183e8 KiB x 2.000 + WRITE_SIZE 1.18279e6 KiB x 1.000 per
+# dispatch.  Summary and commands: profiles/k_batch_sums_ragged_512_regions.md
+RAGGED_TRAFFIC_BYTES = {"fp4": 482.9e9}
+# Issue counters of the same launch (SQ_INSTS_VALU, which counts the MFMAs
+# too, and SQ_INSTS_MFMA per dispatch): the VALU-issue bound of the kernel.
+RAGGED_SQ = {"fp4": {"valu_insts": 1.28954e11, "mfma_insts": 6.82821e9}}
+CLOCK_HZ = 2.4e9         # MI355X peak engine clock
+N_SIMD = 256 * 4         # 256 CUs x 4 SIMDs
+VALU_ISSUE_CYCLES = 4    # a wave64 VALU instruction (FP64 included) holds its SIMD's issue for 4 cycles
+MFMA_FP4_CYCLES = 32     # v_mfma_f32_32x32x64_f8f6f4 with e2m1 operands
+
+
+def valu_issue_bound(sq, kernel_ms):
+    """Issue time of the kernel's non-MFMA VALU instructions spread over every
+    SIMD at the peak clock, against the measured kernel time (the MFMA pipe's
+    own busy time beside it)."""
+    valu = sq["valu_insts"] - sq["mfma_insts"]
+    issue_ms = valu * VALU_ISSUE_CYCLES / N_SIMD / CLOCK_HZ * 1e3
+    mfma_ms = sq["mfma_insts"] * MFMA_FP4_CYCLES / N_SIMD / CLOCK_HZ * 1e3
+    return {"bound": "valu_issue", "valu_insts": valu, "issue_ms": issue_ms, "frac": issue_ms / kernel_ms,
+            "mfma_insts": sq["mfma_insts"], "mfma_busy_ms": mfma_ms, "mfma_busy_frac": mfma_ms / kernel_ms,
+            "source": "SQ_INSTS_VALU and SQ_INSTS_MFMA per dispatch (rocprofv3 --pmc), kernel_ms live"}
 
 
 def parse():
@@ -588,8 +611,12 @@ def run_ragged_regions(args, torch):
         kms.append(b.kernel_ms())
     b.set_timing(False)
     k_ms = float(np.mean(kms))
-    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_batch_sums", macs, fp4=fp4)
-    log(f"[bench] k_batch_sums {k_ms:.3f} ms -> {roof['achieved']:.1f} int8 TOP/s")
+    default_shape = n_regions == 512
+    roof = mfma_roofline(2.0 * macs * pair_samples, k_ms, "k_batch_sums", macs,
+                         RAGGED_TRAFFIC_BYTES.get(args.planes) if default_shape else None, fp4=fp4)
+    valu = valu_issue_bound(RAGGED_SQ[args.planes], k_ms) if default_shape and args.planes in RAGGED_SQ else None
+    log(f"[bench] k_batch_sums {k_ms:.3f} ms -> {roof['achieved']:.1f} TOP/s"
+        + (f", VALU issue {valu['frac']:.2f} of the kernel time, MFMA busy {valu['mfma_busy_frac']:.2f}" if valu else ""))
 
     # the same work region by region: one band plan and one fused launch each
     plain_ms = check = None
@@ -650,7 +677,7 @@ def run_ragged_regions(args, torch):
     line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "plain": "one band plan and one fused launch per region", "batch_vs_plain_max_abs": check,
-                 "plan_bind_s": plan_s, "oracle_error": oracle_err, "cpu_baseline": cpu})
+                 "plan_bind_s": plan_s, "valu_roofline": valu, "oracle_error": oracle_err, "cpu_baseline": cpu})
     return line
 
 
