@@ -203,7 +203,8 @@ int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
  * populations use the fast-path forms of ld_pipeline.py:271-322), then
  * float64 sums per key, reduced in a fixed order (bit-reproducible run to
  * run).  planes / marg / k_pad / n_samples are HOST arrays of n_pops entries
- * (device pointers for planes and marg).  sums (device, n_keys x n_stats) is
+ * (device pointers for planes and marg); n_samples <= 32767 and k_pad <= 32768
+ * per population (as packed).  sums (device, n_keys x n_stats) is
  * overwritten; key_pairs (device, n_keys, may be NULL) receives the number of
  * pairs per key.  Haplotypes: moments set for n_pops <= 3, self set for
  * n_pops <= 4; genotypes: n_pops <= 2.  Replaces the

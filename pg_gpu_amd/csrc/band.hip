@@ -1572,6 +1572,7 @@ static int fill_pop_args(int32_t n_pops, const int8_t* const* planes, const int3
     PGLD_REQUIRE(k_pad[p] % KSTEP == 0 && k_pad[p] >= KSTEP && k_pad[p] >= n_samples[p] && n_samples[p] >= 0,
                  "band: k_pad must be a positive multiple of 128 >= n_samples (pop %d)", p);
     PGLD_REQUIRE(n_samples[p] <= 32767, "band: at most 32767 samples per population");
+    PGLD_REQUIRE(k_pad[p] <= MAX_K_PAD, "band: k_pad must be at most 32768 (pop %d)", p);
     pa.planes[p] = planes[p];
     pa.marg[p] = marg[p];
     pa.k_pad[p] = k_pad[p];

@@ -78,6 +78,10 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 constexpr int BT = 64;       // tile edge (SNPs)
 constexpr int KSTEP = 128;   // K bytes per loop iteration: 4 MFMAs of K = 32
+// Largest K padding of a population's planes: 32,767 samples padded to a
+// multiple of 128 (int8) or 256 (FP4).  Operand offsets within a 64-row block
+// (OpBlock) are 24-bit products that rely on it.
+constexpr int64_t MAX_K_PAD = 32768;
 constexpr int BAND_THREADS = 256;
 constexpr int RED_LDS_BYTES = 64 * 1024;
 constexpr int RED_CHUNK = 256;      // slots per first-level chunk of a long key reduction
@@ -161,6 +165,58 @@ __host__ __device__ constexpr int opbuf_bytes() { return 2 * N * BT * OP_ROW; }
 // padded to a multiple of 128 bytes (1,024 samples) with zero words.
 __host__ __device__ inline int64_t bit_row_bytes(int64_t k_pad) { return 128 * ((k_pad + 1023) / 1024); }
 
+// Wave-uniform value in scalar registers.
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Operand rows of one population for a tile's K loop.  The byte address of
+// a copy is a wave-uniform part -- the side's block base (first plane used,
+// first SNP of the tile's 64-row block on that side), the plane and the K
+// offset -- plus this thread's 32-bit offset within the block: (row - first
+// row) x row bytes + 16 (t % 8), below 64 x 32,768 bytes.  The loads then
+// take their base from scalar registers and need no 64-bit address
+// arithmetic on the VALU.  Copy m of thread t covers bytes 16 (t % 8) .. of
+// row 32 (m % 2) + t / 8 of side (m / 2) / N, plane (m / 2) % N, and lands at
+// LDS offset (t / 8) OP_ROW + 16 (t % 8) + 32 OP_ROW m.
+struct OpBlock {
+  const GLOBAL_AS int8_t* side[2];
+  int64_t plane;
+  uint32_t off[2][2];  // side, row half
+};
+__device__ __forceinline__ OpBlock op_block(const int8_t* planes, int first_plane, int64_t row, int64_t n_var,
+                                            int i0, int j0) {
+  OpBlock s;
+  row = uniform_i64(row);
+  s.plane = n_var * row;
+  const GLOBAL_AS int8_t* base = (const GLOBAL_AS int8_t*)uniform_i64((int64_t)planes) + first_plane * s.plane;
+  s.side[0] = (const GLOBAL_AS int8_t*)uniform_i64((int64_t)(base + (int64_t)i0 * row));
+  s.side[1] = (const GLOBAL_AS int8_t*)uniform_i64((int64_t)(base + (int64_t)j0 * row));
+  const int t = threadIdx.x >> 3;
+  const uint32_t q = 16u * (threadIdx.x & 7);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    // rows past the panel's end read its last row (never used)
+    const int ri = (int)(min((int64_t)i0 + t + 32 * h, n_var - 1) - i0);
+    const int rj = (int)(min((int64_t)j0 + t + 32 * h, n_var - 1) - j0);
+    s.off[0][h] = __umul24((uint32_t)ri, (uint32_t)row) + q;
+    s.off[1][h] = __umul24((uint32_t)rj, (uint32_t)row) + q;
+  }
+  return s;
+}
+// The 2 N BT x 16-byte copies of the K chunk at byte offset k (uniform).
+template <int N>
+__device__ __forceinline__ void op_load(v4i* pre, const OpBlock& s, int64_t k) {
+#pragma unroll
+  for (int m = 0; m < 4 * N; ++m) {
+    const int a = (m >> 1) % N, sd = (m >> 1) / N, h = m & 1;
+    const GLOBAL_AS int8_t* b = s.side[sd] + ((int64_t)a * s.plane + k);
+    pre[m] = *(const GLOBAL_AS v4i*)(b + s.off[sd][h]);
+  }
+}
+
 // POPC = false: int8 0/1 planes (row_bytes = k_pad), products on the int8
 // MFMA.  POPC = true: bit planes (row_bytes = bit_row_bytes(k_pad)), products
 // as AND + popcount on the VALU, accumulated into the same accumulator
@@ -177,21 +233,13 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   using PL = Planes<FAMILY, MISSING>;
   constexpr int N = PL::N;
   constexpr int ITEMS = 2 * N * BT * 8 / BAND_THREADS;  // 16-byte copies per thread per chunk
+  static_assert(ITEMS == 4 * N, "copy layout");
   const int tid = threadIdx.x, lane = tid & 63;
   // plane pointers may come from a region table in memory; the explicit
   // global address space keeps the loads off the flat path (whose counter
   // would make every LDS wait also wait for the prefetch)
-  const GLOBAL_AS int8_t* src[ITEMS];
-  int dst[ITEMS];
-#pragma unroll
-  for (int m = 0; m < ITEMS; ++m) {
-    const int e = tid + BAND_THREADS * m;
-    const int q = e & 7, row = (e >> 3) & (BT - 1), rest = e >> 9;
-    const int a = rest % N, side = rest / N;
-    const int64_t snp = min((int64_t)(side ? j0 : i0) + row, n_var - 1);
-    src[m] = (const GLOBAL_AS int8_t*)planes + ((int64_t)(PL::FIRST + a) * n_var + snp) * k_pad + 16 * q;
-    dst[m] = ((side * N + a) * BT + row) * OP_ROW + 16 * q;
-  }
+  const OpBlock ob = op_block(planes, PL::FIRST, k_pad, n_var, i0, j0);
+  const int dbase = (tid >> 3) * OP_ROW + 16 * (tid & 7);
 #pragma unroll
   for (int a = 0; a < N; ++a)
 #pragma unroll
@@ -206,11 +254,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   // chunk and is never used) so that every path keeps the same number of
   // loads in flight and the wait before a chunk's LDS store covers only that
   // chunk, not the younger prefetch.  k_pad is a multiple of KSTEP.
-  auto load = [&](v4i* pre, int64_t k) {
-    const int64_t kk = k < k_pad ? k : k_pad - KSTEP;
-#pragma unroll
-    for (int m = 0; m < ITEMS; ++m) pre[m] = *(const GLOBAL_AS v4i*)(src[m] + kk);
-  };
+  auto load = [&](v4i* pre, int64_t k) { op_load<N>(pre, ob, k < k_pad ? k : k_pad - KSTEP); };
   load(pre0, 0);
   if constexpr (PF == 2) load(pre1, KSTEP);
   // The two workgroups of a CU alternate between this loop (MFMA, LDS) and
@@ -221,7 +265,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   auto step = [&](v4i* pre, int64_t k0) {
     __syncthreads();  // the previous chunk has been read by every wave
 #pragma unroll
-    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dst[m]) = pre[m];
+    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dbase + 32 * OP_ROW * m) = pre[m];
     __syncthreads();
     load(pre, k0 + PF * KSTEP);
     if (wave_live && POPC) {
@@ -314,50 +358,20 @@ struct PopStream {
   static_assert(ITEMS == 4 * N, "copy layout");
 
   __device__ __forceinline__ static int64_t row_bytes(int64_t k_pad) { return NIB ? k_pad / 2 : k_pad; }
-  // the same value in every lane, kept in scalar registers
-  __device__ __forceinline__ static int64_t uniform(int64_t x) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-  }
-  // One population's operand source: planes (from the first plane used),
-  // bytes per row and per plane, K chunks.
+  // One population's operand rows (OpBlock) and its K chunks.
   struct Src {
-    const GLOBAL_AS int8_t* base;
-    int64_t row, plane;
+    OpBlock ob;
     int nc;
   };
-  __device__ __forceinline__ static Src source(const PopArgs& pa, int p, int64_t n_var) {
+  __device__ __forceinline__ static Src source(const PopArgs& pa, int p, int64_t n_var, int i0, int j0) {
     Src s;
-    s.row = uniform(row_bytes(pa.k_pad[p]));
-    s.plane = n_var * s.row;
-    s.base = (const GLOBAL_AS int8_t*)uniform((int64_t)pa.planes[p]) + (int64_t)PL::FIRST * s.plane;
-    s.nc = (int)(s.row / KSTEP);
+    const int64_t row = row_bytes(pa.k_pad[p]);
+    s.ob = op_block(pa.planes[p], PL::FIRST, row, n_var, i0, j0);
+    s.nc = (int)(uniform_i64(row) / KSTEP);
     return s;
   }
-  // Operand rows of this thread's copies: copy m of thread t covers bytes
-  // 16 (t % 8) .. of row 32 (m % 2) + t / 8 of side (m / 2) / N, plane
-  // (m / 2) % N, and lands at LDS offset (t / 8) OP_ROW + 16 (t % 8) + 32 OP_ROW m.
-  struct Rows {
-    int snp[2][2];  // side, row half -> SNP (clamped to the panel)
-  };
-  __device__ __forceinline__ static Rows rows(int64_t n_var, int i0, int j0) {
-    Rows r;
-    const int t = threadIdx.x >> 3;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      r.snp[0][h] = (int)min((int64_t)i0 + t + 32 * h, n_var - 1);
-      r.snp[1][h] = (int)min((int64_t)j0 + t + 32 * h, n_var - 1);
-    }
-    return r;
-  }
-  __device__ __forceinline__ static void load(v4i (&pre)[ITEMS], const Src& s, const Rows& r, int chunk) {
-    const GLOBAL_AS int8_t* b = s.base + (int64_t)chunk * KSTEP + 16 * (threadIdx.x & 7);
-#pragma unroll
-    for (int m = 0; m < ITEMS; ++m) {
-      const int a = (m >> 1) % N, sd = (m >> 1) / N, h = m & 1;
-      pre[m] = *(const GLOBAL_AS v4i*)(b + a * s.plane + (int64_t)r.snp[sd][h] * s.row);
-    }
+  __device__ __forceinline__ static void load(v4i (&pre)[ITEMS], const Src& s, int chunk) {
+    op_load<N>(pre, s.ob, (int64_t)chunk * KSTEP);
   }
 
   // K loop of population p of a tile whose populations are evaluated one
@@ -376,10 +390,9 @@ struct PopStream {
                                                   int wc, bool wave_live, unsigned char* __restrict__ opbuf,
                                                   WaveAcc<N, NIB>& acc, v4i (&pre0)[ITEMS], v4i (&pre1)[ITEMS],
                                                   bool& pre1_ok) {
-    const Src cur = source(pa, p, n_var);
+    const Src cur = source(pa, p, n_var, i0, j0);
     const bool has_next = p + 1 < P;
-    const Src nxt = source(pa, has_next ? p + 1 : p, n_var);
-    const Rows rw = rows(n_var, i0, j0);
+    const Src nxt = source(pa, has_next ? p + 1 : p, n_var, i0, j0);
     const int nc = cur.nc;
     const int lane = threadIdx.x & 63;
     const int dbase = (threadIdx.x >> 3) * OP_ROW + 16 * (threadIdx.x & 7);
@@ -434,15 +447,19 @@ struct PopStream {
       const int par = c & 1;
       const int chunk = same ? c + 2 : ((has_next && nxt.nc > par) ? par : 0);
       Src s;
-      s.base = same ? cur.base : nxt.base;
-      s.row = same ? cur.row : nxt.row;
-      s.plane = same ? cur.plane : nxt.plane;
-      load(pre, s, rw, chunk);
+      s.ob.side[0] = same ? cur.ob.side[0] : nxt.ob.side[0];
+      s.ob.side[1] = same ? cur.ob.side[1] : nxt.ob.side[1];
+      s.ob.plane = same ? cur.ob.plane : nxt.ob.plane;
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) s.ob.off[sd][h] = same ? cur.ob.off[sd][h] : nxt.ob.off[sd][h];
+      load(pre, s, chunk);
     };
     // chunk 0 outside the loop; pre1's late load (after a one-chunk
     // population) is issued once pre0 is in LDS
     stage(pre0);
-    if (!pre1_ok && nc > 1) load(pre1, cur, rw, 1);
+    if (!pre1_ok && nc > 1) load(pre1, cur, 1);
     refill(pre0, 0);
     if (wave_live) mfmas();
     int c = 1;
@@ -467,10 +484,9 @@ struct PopStream {
   // first two chunks of a tile's population 0
   __device__ __forceinline__ static void start(const PopArgs& pa, int64_t n_var, int i0, int j0,
                                                v4i (&pre0)[ITEMS], v4i (&pre1)[ITEMS], bool& pre1_ok) {
-    const Src s = source(pa, 0, n_var);
-    const Rows rw = rows(n_var, i0, j0);
-    load(pre0, s, rw, 0);
-    load(pre1, s, rw, s.nc > 1 ? 1 : 0);
+    const Src s = source(pa, 0, n_var, i0, j0);
+    load(pre0, s, 0);
+    load(pre1, s, s.nc > 1 ? 1 : 0);
     pre1_ok = s.nc > 1;
   }
 };

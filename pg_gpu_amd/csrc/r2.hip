@@ -665,6 +665,7 @@ static int band_r2(const pgld_band_plan* plan, const int8_t* planes, bool popc, 
                "pgld_band_r2: k_pad must be a positive multiple of 128 >= n_samples");
   // the 24-bit count products of hap_pair_value need counts below 2^15
   PGLD_REQUIRE(n_samples <= 32767, "pgld_band_r2: at most 32767 samples per population");
+  PGLD_REQUIRE(k_pad <= MAX_K_PAD, "pgld_band_r2: k_pad must be at most 32768");
   PGLD_REQUIRE(!hist || n_hist >= 1, "pgld_band_r2: n_hist must be >= 1");
   PGLD_REQUIRE(!hist || (int64_t)plan->max_tile_slots * (n_hist + 1) <= R2_MAX_HIST_LDS,
                "pgld_band_r2: %d keys per tile x %d histogram bins exceed LDS", plan->max_tile_slots, n_hist + 1);
