@@ -145,10 +145,14 @@ int pgld_pack_planes(int32_t family, const int8_t* data, int64_t n_rows, int64_t
                      const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
                      int32_t* flags, void* stream);
 /* The same planes in FP4 (no counterpart in the reference: an operand layout
- * for the gfx950 FP4 MFMA): each 0/1 indicator as an e2m1 code (0x0 or
- * 0x2 = 1.0), two samples per byte, the even sample in the low nibble; k_pad
- * must be a multiple of 256 and planes has n_planes * n_var * k_pad / 2
- * bytes.  marg and flags as above.  Consumed by pgld_band_sums_fp4. */
+ * for the gfx950 FP4 MFMA): each indicator as an e2m1 code (0x0 or
+ * 0x2 = 1.0), two samples per byte, the even sample in the low nibble; the
+ * haplotype allele plane is centred instead: 0x2 (+1) where x == 1, 0xA (-1)
+ * where x == 0, 0x0 where missing, so that its products are the centred
+ * moments the statistics use.  k_pad must be a multiple of 256 and planes
+ * has n_planes * n_var * k_pad / 2 bytes.  marg (the 0/1 indicator sums) and
+ * flags as above.  Consumed by pgld_band_sums_fp4, pgld_batch_bind_fp4 and
+ * pgld_band_ld_pairs_fp4. */
 int pgld_pack_planes_fp4(int32_t family, const int8_t* data, int64_t n_rows, int64_t n_var,
                          const int32_t* rows, int64_t n_sel, int64_t k_pad, int8_t* planes, int32_t* marg,
                          int32_t* flags, void* stream);

@@ -109,9 +109,10 @@ class RegionBatch:
     def schedule(self, grid=0, order="cost", xcd_split=False, kernel_variant=0):
         """grid 0 = one workgroup per tile; order "cost" (descending work),
         "natural" or "blocked" (8 x 8 tile blocks per XCD for L2 reuse);
-        kernel_variant 0 / 1 / 2 (FP4 within-population haplotype sets: K
+        kernel_variant 0 / 1 / 2 / 3 (FP4 within-population haplotype sets: K
         loop streamed across populations / per population / streamed with
-        rolled epilogue rows; the sums are identical)."""
+        rolled epilogue rows / lean: per population with one load stage,
+        three workgroups per CU; the sums are identical)."""
         if order not in self._ORDERS:
             raise ValueError(f"order must be one of {self._ORDERS}")
         L.call("pgld_batch_schedule", self._h, int(grid), self._ORDERS.index(order), int(bool(xcd_split)),

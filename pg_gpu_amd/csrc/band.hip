@@ -103,12 +103,15 @@ constexpr int band_min_waves(int family, int P, int set) {
 #define PGLD_HAP2_GROUP 1
 #endif
 // Kernel variants (KV, bit-identical results; the batch autotuner chooses
-// bits 0 and 1): bit 0 set = each population's K loop on its own (no load
+// bits 0, 1 and 3): bit 0 set = each population's K loop on its own (no load
 // stream across populations), bit 1 set = fast-path rows in a rolled loop,
 // bit 2 set = the within-population haplotype sets with missing data read
 // 1 / (n)_4 from an LDS table (BandArgs::rtab_*) instead of computing it per
-// pair (set by the launcher when the table fits).
-constexpr int KV_PER_POP = 1, KV_ROLLED = 2, KV_RTAB = 4;
+// pair (set by the launcher when the table fits), bit 3 set = lean: each
+// population's K loop on its own with one register stage and one fragment
+// set, registers capped for three workgroups per CU (the extra waves hide
+// the latencies the stages and the table would).
+constexpr int KV_PER_POP = 1, KV_ROLLED = 2, KV_RTAB = 4, KV_LEAN = 8;
 
 // Fills the reciprocal table of a KV_RTAB launch; the workgroup barrier that
 // follows the tile set-up orders it before the first read.
@@ -183,7 +186,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // the per-population kernel variant) and the haplotype moments sets
   using PS = PopStream<FAMILY, MISSING, NIB>;
   constexpr bool STREAM =
-      P > 1 && (SET == PGLD_SET_SELF ? !(KV & KV_PER_POP) : FAMILY == PGLD_FAMILY_HAP);
+      P > 1 && (SET == PGLD_SET_SELF ? !(KV & (KV_PER_POP | KV_LEAN)) : FAMILY == PGLD_FAMILY_HAP);
   v4i pre0[STREAM ? PS::ITEMS : 1], pre1[STREAM ? PS::ITEMS : 1];
   bool pre1_ok = false;
   if constexpr (STREAM) PS::start(pa, ba.n_var, tile.i0, tile.j0, pre0, pre1, pre1_ok);
@@ -374,7 +377,10 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
-      if constexpr (KV & KV_PER_POP)
+      if constexpr ((KV & KV_LEAN) != 0)
+        block_products<FAMILY, MISSING, false, NIB, 1, false>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0,
+                                                              wr, wc, wave_live, opbuf, acc);
+      else if constexpr ((KV & KV_PER_POP) != 0)
         block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
                                                     wave_live, opbuf, acc);
       else
@@ -505,7 +511,7 @@ constexpr int batch_min_waves(int family, int P, int set) {
 }
 
 template <int FAMILY, bool MISSING, int P, int SET, bool NIB, int KV = 0>
-__global__ __launch_bounds__(BAND_THREADS, batch_min_waves(FAMILY, P, SET)) void k_batch_sums(
+__global__ __launch_bounds__(BAND_THREADS, (KV & KV_LEAN) ? 3 : batch_min_waves(FAMILY, P, SET)) void k_batch_sums(
     BandArgs ba, const RegionDesc* __restrict__ regions, const int32_t* __restrict__ order, int64_t n_tiles,
     int32_t xcd_split) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -772,12 +778,21 @@ __device__ __forceinline__ void pack_tile(const int8_t* __restrict__ data, int64
 #pragma unroll
     for (int p = 0; p < NPL; ++p) {
       if constexpr (NIB) {
-        // four 0/1 bytes of a dword -> four nibbles (byte b -> bits 4b), times the e2m1 code of 1.0
+        // four 0/1 bytes of a dword -> four 0/1 nibbles (byte b -> bits 4b),
+        // times the e2m1 code of 1.0; the haplotype allele plane is centred:
+        // +1 (0x2) where x == 1, -1 (0xA) where x == 0, 0 where missing
+        auto nib = [](uint32_t x) {
+          const uint32_t t = (x | (x >> 4)) & 0x00ff00ffu;
+          return (t | (t >> 8)) & 0xffffu;
+        };
         uint32_t h[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          uint32_t t = (pw[p][i] | (pw[p][i] >> 4)) & 0x00ff00ffu;
-          h[i] = ((t | (t >> 8)) & 0xffffu) * (uint32_t)FP4_ONE;
+          // per nibble: valid -> 0xA, minus 0x8 where x == 1 (no borrows: x == 1 implies valid)
+          if (FAMILY == PGLD_FAMILY_HAP && p == 0)
+            h[i] = nib(pw[1][i]) * (uint32_t)FP4_MINUS_ONE - nib(pw[0][i]) * (uint32_t)(FP4_MINUS_ONE - FP4_ONE);
+          else
+            h[i] = nib(pw[p][i]) * (uint32_t)FP4_ONE;
         }
         uint2 val2;
         val2.x = h[0] | (h[1] << 16);
@@ -1164,6 +1179,11 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
   // the within-population haplotype sets with missing data take 1 / (n)_4
   // from an LDS table of n = 0 .. the largest population when it fits
   constexpr bool TAB = FAMILY == PGLD_FAMILY_HAP && MISSING && (SET == PGLD_SET_SELF || P == 1);
+  // the autotuned variants (plan->kvar) exist for the FP4 within-population
+  // haplotype sets of region batches; the lean one (kvar 3) leaves the table
+  // out so that three workgroups fit a CU's LDS
+  constexpr bool VARIANTS = FAMILY == PGLD_FAMILY_HAP && SET == PGLD_SET_SELF && P > 1 && NIB;
+  const bool lean = VARIANTS && plan->n_regions > 0 && plan->kvar == 3;
   bool tab = false;
   if constexpr (TAB) {
     int smax = 0;
@@ -1172,7 +1192,7 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
     else
       for (int p = 0; p < P; ++p) smax = std::max(smax, pa.n_samples[p]);
     const size_t bytes = (((size_t)smax + 1) * 8 + 15) / 16 * 16;
-    if (recip_table_fits(lds, bytes)) {
+    if (!lean && recip_table_fits(lds, bytes)) {
       tab = true;
       ba.rtab_n = smax;
       ba.rtab_off = (int32_t)lds;
@@ -1191,9 +1211,9 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
       constexpr int KV = decltype(kv_tag)::value;
       hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV>), g, b, lds, s, ba, rd, ord, len, xs);
     };
-    // the autotuned variants (plan->kvar) exist for the FP4 within-population
-    // haplotype sets; with missing data only in the table form
-    constexpr bool VARIANTS = FAMILY == PGLD_FAMILY_HAP && SET == PGLD_SET_SELF && P > 1 && NIB;
+    if constexpr (VARIANTS) {
+      if (lean) return go(std::integral_constant<int, KV_LEAN>{});
+    }
     if constexpr (TAB) {
       if (tab) {
         if constexpr (VARIANTS) {
@@ -1897,7 +1917,7 @@ int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32
                         int32_t kernel_variant) {
   PGLD_REQUIRE(plan && plan->n_regions > 0, "pgld_batch_schedule: not a region batch plan");
   PGLD_REQUIRE(grid >= 0 && order >= 0 && order <= 2, "pgld_batch_schedule: bad grid or order");
-  PGLD_REQUIRE(kernel_variant >= 0 && kernel_variant <= 2, "pgld_batch_schedule: kernel variant must be 0, 1 or 2");
+  PGLD_REQUIRE(kernel_variant >= 0 && kernel_variant <= 3, "pgld_batch_schedule: kernel variant must be 0 .. 3");
   plan->grid = grid;
   plan->order_mode = order;
   plan->xcd_split = xcd_split != 0;
@@ -1946,7 +1966,7 @@ static void tune_load_locked() {
   char key[256];
   TuneEntry t;
   while (fscanf(f, "%255s %d %d %d %d %f", key, &t.grid, &t.order, &t.xcd, &t.kvar, &t.ms) == 6) {
-    if (t.grid >= 0 && t.order >= 0 && t.order <= 2 && t.kvar >= 0 && t.kvar <= 2) g_tune[key] = t;
+    if (t.grid >= 0 && t.order >= 0 && t.order <= 2 && t.kvar >= 0 && t.kvar <= 3) g_tune[key] = t;
   }
   fclose(f);
 }
@@ -2041,7 +2061,7 @@ int pgld_batch_autotune(pgld_band_plan* plan, int32_t with_missing, double* sums
     cands.push_back({g, 2, 0, 0, 0.f});  // 8 x 8 tile blocks dealt to the XCDs
   }
   if (variants)
-    for (int32_t kv : {1, 2}) {
+    for (int32_t kv : {1, 2, 3}) {
       cands.push_back({0, 2, 0, kv, 0.f});
       cands.push_back({0, 0, 0, kv, 0.f});
     }

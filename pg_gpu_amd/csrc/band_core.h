@@ -141,14 +141,20 @@ struct WaveAcc {
   T c[N][N];
 };
 
-// FP4 planes: the 0/1 indicators as e2m1 codes (0 -> 0x0, 1 -> 0x2 = 1.0),
-// two samples per byte, the even sample in the low nibble; a row holds
+// FP4 planes: the indicators as e2m1 codes (0 -> 0x0, 1 -> 0x2 = 1.0, and
+// -1 -> 0xA for the centred haplotype allele plane below), two samples per byte, the even sample in the low nibble; a row holds
 // k_pad / 2 bytes, k_pad a multiple of 256.  v_mfma_f32_32x32x64_f8f6f4 with
 // e2m1 operands (cbsz = blgp = 4) and unit scales (E8M0 127) takes the 16
 // bytes a lane reads for one int8 MFMA as 32 samples of row (lane mod 32),
 // K half (lane div 32), nibble k low first -- the same lane layout on both
 // operands, so one FP4 MFMA covers 64 samples where the int8 one covers 32.
 constexpr int FP4_ONE = 0x2;
+// Haplotype FP4 planes hold the allele plane centred: a' = +1 (0x2) for the
+// derived allele, -1 (0xA) for the ancestral one, 0 where missing; the valid
+// plane keeps 0 / 1.  The products then are the centred moments the
+// statistics use (hap_plane_counts, hap_moments).  Genotype FP4 planes hold
+// 0 / 1 indicators on every plane.
+constexpr int FP4_MINUS_ONE = 0xA;
 constexpr int FP4_UNIT_SCALE = 0x7f;
 
 // LDS-staged products for the whole workgroup (all four waves must call it:
@@ -222,7 +228,12 @@ __device__ __forceinline__ void op_load(v4i* pre, const OpBlock& s, int64_t k) {
 // as AND + popcount on the VALU, accumulated into the same accumulator
 // layout as the MFMA (lane l -> column l % 32, rows acc_row(r, l)), so every
 // epilogue works with either.
-template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false>
+// PFS: register stages of operand loads in flight (0 = two for up to two
+// planes per side, else one); FDB: the MFMA operand fragments of the next
+// sub-step are read while the current one's MFMAs run (two fragment sets).
+// One stage and one fragment set keep fewer registers live, for kernels that
+// hide the latency with more waves per SIMD instead.
+template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false, int PFS = 0, bool FDB = true>
 __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
                                                int i0, int j0, int wr, int wc, bool wave_live,
                                                unsigned char* __restrict__ opbuf,
@@ -248,7 +259,7 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
       for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
   // register stages: the loads of chunk c + PF are in flight while the MFMAs
   // of chunk c run (PF = 2 where registers allow: up to 2 planes per side)
-  constexpr int PF = N <= 2 ? 2 : 1;
+  constexpr int PF = PFS > 0 ? PFS : (N <= 2 ? 2 : 1);
   v4i pre0[ITEMS], pre1[PF == 2 ? ITEMS : 1];
   // Loads are issued unconditionally (a chunk past the end re-reads the last
   // chunk and is never used) so that every path keeps the same number of
@@ -297,9 +308,10 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
         }
       }
     } else if (wave_live) {
-      // operand fragments of substep st + 1 are read while the MFMAs of
-      // substep st run
-      v4i A[2][N], B[2][N];
+      // with FDB, operand fragments of substep st + 1 are read while the
+      // MFMAs of substep st run
+      constexpr int NF = FDB ? 2 : 1;
+      v4i A[NF][N], B[NF][N];
       auto frag = [&](int st, v4i* a_, v4i* b_) {
 #pragma unroll
         for (int a = 0; a < N; ++a) {
@@ -307,21 +319,25 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
           b_[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
         }
       };
-      frag(0, A[0], B[0]);
+      if constexpr (FDB) frag(0, A[0], B[0]);
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
-        if (st < 3) frag(st + 1, A[(st + 1) & 1], B[(st + 1) & 1]);
+        if constexpr (FDB) {
+          if (st < 3) frag(st + 1, A[(st + 1) % NF], B[(st + 1) % NF]);
+        } else {
+          frag(st, A[0], B[0]);
+        }
 #pragma unroll
         for (int a = 0; a < N; ++a)
 #pragma unroll
           for (int b = 0; b < N; ++b) {
             if constexpr (NIB) {
-              const v4i x = A[st & 1][a], y = B[st & 1][b];
+              const v4i x = A[st % NF][a], y = B[st % NF][b];
               const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
               acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc.c[a][b], 4, 4, 0,
                                                                             FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
             } else {
-              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
+              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st % NF][a], B[st % NF][b], acc.c[a][b], 0, 0, 0);
             }
           }
       }
@@ -491,6 +507,50 @@ struct PopStream {
   }
 };
 
+// Haplotype plane products of accumulator element r (row i, col j) as the
+// four 0/1 counts hh = #(a = 1, b = 1), hv = #(a = 1, b valid), vh, vv over
+// the pair-valid samples.  int8 planes hold the 0/1 indicators, so the
+// products are these counts.  FP4 planes hold the allele plane centred
+// (a' = 2a - 1 on valid samples, 0 where missing; see FP4_MINUS_ONE), so
+// their products are the centred moments Sab = sum a'b', Sa = sum a' v_j,
+// Sb = sum v_i b' and n = vv, and hh = (n + Sa + Sb + Sab) / 4, hv = (n + Sa)
+// / 2, vh = (n + Sb) / 2 (float accumulators hold these integers exactly).
+// Without missing data only the allele-plane product is formed; the rest
+// follows from the per-SNP allele sums (marg) and the sample count.
+template <bool MISSING, class ACC>
+__device__ __forceinline__ void hap_plane_counts(const ACC& acc, int r, const GLOBAL_AS int32_t* marg, int64_t i,
+                                                 int64_t j, int32_t S, int& hh, int& hv, int& vh, int& vv) {
+  if constexpr (std::is_same<typename ACC::T, v16f>::value) {
+    const int sab = (int)acc.c[0][0][r];
+    int sa, sb;
+    if constexpr (MISSING) {
+      sa = (int)acc.c[0][1][r];
+      sb = (int)acc.c[1][0][r];
+      vv = (int)acc.c[1][1][r];
+      hv = (vv + sa) >> 1;
+      vh = (vv + sb) >> 1;
+    } else {
+      hv = marg[i];
+      vh = marg[j];
+      vv = S;
+      sa = 2 * hv - vv;
+      sb = 2 * vh - vv;
+    }
+    hh = (vv + sa + sb + sab) >> 2;
+  } else {
+    hh = acc.c[0][0][r];
+    if constexpr (MISSING) {
+      hv = acc.c[0][1][r];
+      vh = acc.c[1][0][r];
+      vv = acc.c[1][1][r];
+    } else {
+      hv = marg[i];  // plane 0 (h) sums
+      vh = marg[j];
+      vv = S;
+    }
+  }
+}
+
 // Joint counts of accumulator element r (row i, col j) in the reference's
 // column order: haplotypes [n11, n10, n01, n00]; genotypes n00 .. n22.
 template <int FAMILY, bool MISSING, class ACC>
@@ -500,16 +560,7 @@ __device__ __forceinline__ void elem_counts(const ACC& acc, int r,
   const GLOBAL_AS int32_t* marg = (const GLOBAL_AS int32_t*)marg_;
   if constexpr (FAMILY == PGLD_FAMILY_HAP) {
     int hh, hv, vh, vv;
-    hh = (int)acc.c[0][0][r];  // float accumulators (FP4) hold exact integers
-    if constexpr (MISSING) {
-      hv = (int)acc.c[0][1][r];
-      vh = (int)acc.c[1][0][r];
-      vv = (int)acc.c[1][1][r];
-    } else {
-      hv = marg[i];  // plane 0 (h) sums
-      vh = marg[j];
-      vv = S;
-    }
+    hap_plane_counts<MISSING>(acc, r, marg, i, j, S, hh, hv, vh, vv);
     out[0] = hh;
     out[1] = hv - hh;
     out[2] = vh - hh;
@@ -546,29 +597,24 @@ __device__ __forceinline__ void elem_counts(const ACC& acc, int r,
 
 // Centred haplotype moments of accumulator element r: n = vv, Sa = 2 hv - vv,
 // Sb = 2 vh - vv, Sab = 4 hh - 2 hv - 2 vh + vv (a' = 2a - 1 over the
-// pair-valid samples).  FP4 accumulators are floats holding integers below
-// 2^24, and these combinations stay below 2^24 in magnitude, so the float
-// arithmetic is exact; int8 accumulators use int32.
+// pair-valid samples).  FP4 planes give them straight from the products
+// (centred allele plane, see hap_plane_counts); int8 accumulators combine
+// the counts in int32.
 template <bool MISSING, class ACC>
 __device__ __forceinline__ void hap_moments(const ACC& acc, int r, const int32_t* __restrict__ marg_, int64_t i,
                                             int64_t j, int32_t S, double& n, double& a, double& b, double& s) {
   const GLOBAL_AS int32_t* marg = (const GLOBAL_AS int32_t*)marg_;
   if constexpr (std::is_same<typename ACC::T, v16f>::value) {
-    const float hh = acc.c[0][0][r];
-    float hv, vh, vv;
+    s = acc.c[0][0][r];
     if constexpr (MISSING) {
-      hv = acc.c[0][1][r];
-      vh = acc.c[1][0][r];
-      vv = acc.c[1][1][r];
+      a = acc.c[0][1][r];
+      b = acc.c[1][0][r];
+      n = acc.c[1][1][r];
     } else {
-      hv = (float)marg[i];
-      vh = (float)marg[j];
-      vv = (float)S;
+      a = 2 * marg[i] - S;
+      b = 2 * marg[j] - S;
+      n = S;
     }
-    n = vv;
-    a = fmaf(2.0f, hv, -vv);
-    b = fmaf(2.0f, vh, -vv);
-    s = fmaf(4.0f, hh, fmaf(-2.0f, hv + vh, vv));
   } else {
     const int hh = acc.c[0][0][r];
     int hv, vh, vv;

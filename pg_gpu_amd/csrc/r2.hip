@@ -238,16 +238,8 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
           }
         }
       } else if (valid && FAMILY == PGLD_FAMILY_HAP) {
-        int32_t n11 = acc.c[0][0][r], nA, nB, n;
-        if constexpr (MISSING) {
-          nA = acc.c[0][1][r];
-          nB = acc.c[1][0][r];
-          n = acc.c[1][1][r];
-        } else {
-          nA = marg[i];
-          nB = marg[j];
-          n = S;
-        }
+        int n11, nA, nB, n;
+        hap_plane_counts<MISSING>(acc, r, (const GLOBAL_AS int32_t*)marg, i, j, S, n11, nA, nB, n);
         bool is_nan = false;
         const double r2 = hap_pair_value<MISSING, EST>(n11, nA, nB, n, marg, ra.n_var, i, j, S, is_nan, sval, scnt);
         r2f = (float)r2;
@@ -524,7 +516,10 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
           bool is_nan = false;
           double sval = 0.0;
           int scnt = 0;
-          const double r2 = hap_pair_value<false, EST>(cur[r], marg[i], marg[j], S, marg, ra.n_var, i, j, S, is_nan,
+          // FP4 planes: cur is the centred product Sab; n11 = (S + Sa + Sb + Sab) / 4
+          // with Sa = 2 marg_i - S, Sb = 2 marg_j - S
+          const int n11 = NIB ? (cur[r] + 2 * (marg[i] + marg[j]) - S) >> 2 : cur[r];
+          const double r2 = hap_pair_value<false, EST>(n11, marg[i], marg[j], S, marg, ra.n_var, i, j, S, is_nan,
                                                        sval, scnt);
           r2f = (float)r2;
           if (ra.hist) {

@@ -93,11 +93,11 @@ def test_batch_fp4_equals_int8(small_batch):
     p4 = batch.pack_regions(d_dev, rows_dev, fp4=True)
     b = batch.RegionBatch(pos, 1e6, EDGES).bind(p4, stat_set="self")
     for grid, order in ((0, "blocked"), (256, "cost"), (1024, "natural")):
-        for kv in (0, 1, 2):  # kernel variants: streamed / per-population K loop, rolled rows
+        for kv in (0, 1, 2, 3):  # kernel variants: streamed / per-population K loop, rolled rows, lean
             b.schedule(grid, order, False, kernel_variant=kv)
             np.testing.assert_array_equal(np_(b.sums(True)), ref)
     choice = b.autotune(True)
-    assert choice["kernel_variant"] in (0, 1, 2)
+    assert choice["kernel_variant"] in (0, 1, 2, 3)
     np.testing.assert_array_equal(np_(b.sums(True)), ref)
     mixed = batch.pack_regions(d_dev, rows_dev)
     mixed[0][0] = p4[0][0]

@@ -409,8 +409,11 @@ def test_pack_planes_domain_error(family, bad):
 @pytest.mark.parametrize("family,n_var,n_rows,missing", [("hap", 301, 150, 0.05), ("hap", 256, 600, 0.0),
                                                           ("geno", 259, 131, 0.05), ("geno", 130, 300, 0.0)])
 def test_pack_planes_fp4_definition(family, n_var, n_rows, missing):
-    """Nibble n of an FP4 row byte b holds sample 2b + n as 0x0 / 0x2, rows of
-    k_pad / 2 bytes with k_pad a multiple of 256; marginals as the int8 planes'."""
+    """Nibble n of an FP4 row byte b holds sample 2b + n, rows of k_pad / 2
+    bytes with k_pad a multiple of 256; every plane as 0x0 / 0x2 (the e2m1
+    codes of 0 and 1) except the haplotype allele plane, which is centred:
+    0x2 (+1) where x == 1, 0xA (-1) where x == 0, 0x0 where missing and in the
+    padding.  Marginals as the int8 planes' (0/1 allele counts)."""
     if family == "hap":
         d, _ = hap_data(103, n_rows, n_var, missing)
     else:
@@ -423,9 +426,16 @@ def test_pack_planes_fp4_definition(family, n_var, n_rows, missing):
     b = np_(p4.planes).view(np.uint8)
     assert b.shape == (np_(p8.planes).shape[0], n_var, kp // 2)
     nib = np.stack([b & 0xF, b >> 4], axis=-1).reshape(b.shape[0], n_var, kp)
-    assert set(np.unique(nib)) <= {0, 2}
     want = np.zeros_like(nib)
     want[:, :, :p8.k_pad] = np_(p8.planes) * 2
+    if family == "hap":
+        assert set(np.unique(nib)) <= {0x0, 0x2, 0xA}
+        # allele plane from the data itself: x == 1 -> 0x2, x == 0 -> 0xA, missing -> 0
+        x = np.asarray(d)[rows].T  # (n_var, samples)
+        want[0] = 0
+        want[0, :, :len(rows)] = np.where(x == 1, 0x2, np.where(x == 0, 0xA, 0x0))
+    else:
+        assert set(np.unique(nib)) <= {0, 2}
     np.testing.assert_array_equal(nib, want)
     np.testing.assert_array_equal(np_(p4.marg), np_(p8.marg))
     assert p4.has_missing == p8.has_missing
