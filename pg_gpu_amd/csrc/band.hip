@@ -108,9 +108,10 @@ constexpr int band_min_waves(int family, int P, int set) {
 // bit 2 set = the within-population haplotype sets with missing data read
 // 1 / (n)_4 from an LDS table (BandArgs::rtab_*) instead of computing it per
 // pair (set by the launcher when the table fits), bit 3 set = lean: each
-// population's K loop on its own with one register stage and one fragment
-// set, registers capped for three workgroups per CU (the extra waves hide
-// the latencies the stages and the table would).
+// population's K loop on its own with one register stage, one fragment set
+// and unpadded (swizzled) LDS operand rows, registers and LDS capped for
+// three workgroups per CU (the extra waves hide the latencies the stages
+// would).
 constexpr int KV_PER_POP = 1, KV_ROLLED = 2, KV_RTAB = 4, KV_LEAN = 8;
 
 // Fills the reciprocal table of a KV_RTAB launch; the workgroup barrier that
@@ -149,14 +150,21 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   };
   constexpr int NS = SS::N + 1;  // statistics + pair count
   // smem layout: [reduction rows: 4 * n_slots * NS doubles][stash: (P-1) * 4 * NC * 16 * 64 int16][operands]
+  // (lean: [operands][reduction rows]; then the reciprocal table at rtab_off)
+  static_assert(!(KV & KV_LEAN) || (SET == PGLD_SET_SELF && P > 1), "lean: within-population sets");
   __shared__ double s_pos_i[BT], s_pos_j[BT], s_kc_i[BT], s_kc_j[BT];
   __shared__ int s_wkey[BT];
   __shared__ double s_edges[MAX_LDS_EDGES];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  double* red = reinterpret_cast<double*>(smem);
+  // The lean variant puts the operand buffer first, at a fixed LDS address
+  // (its swizzled addresses then need no per-tile base), the reduction rows
+  // after it.
+  constexpr bool OP_FIRST = (KV & KV_LEAN) != 0;
+  constexpr int OP_FIRST_BYTES = OP_FIRST ? opbuf_bytes_swz<PL::N>() : 0;
+  double* red = reinterpret_cast<double*>(smem + OP_FIRST_BYTES);
   const int red_n = 4 * tile.n_slots * NS;
-  int16_t* stash = reinterpret_cast<int16_t*>(smem + ((red_n * 8 + 15) / 16) * 16);
+  int16_t* stash = reinterpret_cast<int16_t*>(smem + OP_FIRST_BYTES + ((red_n * 8 + 15) / 16) * 16);
 
   // The tile's positions, key coordinates and bin edges are loaded into
   // registers, the population stream's first operand chunks right behind
@@ -374,12 +382,12 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     // next population's first operands are in flight during this one's
     // epilogue.
     using S1 = StatSet<FAMILY, 1, PGLD_SET_SELF>;
-    unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash);
+    unsigned char* opbuf = OP_FIRST ? smem : reinterpret_cast<unsigned char*>(stash);
 #pragma unroll 1
     for (int p = 0; p < P; ++p) {
       if constexpr ((KV & KV_LEAN) != 0)
-        block_products<FAMILY, MISSING, false, NIB, 1, false>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0,
-                                                              wr, wc, wave_live, opbuf, acc);
+        block_products<FAMILY, MISSING, false, NIB, 1, false, true>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0,
+                                                                    tile.j0, wr, wc, wave_live, opbuf, acc);
       else if constexpr ((KV & KV_PER_POP) != 0)
         block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
                                                     wave_live, opbuf, acc);
@@ -1163,6 +1171,13 @@ static int upload_plan(pgld_band_plan* plan, const double* pos, const double* kc
   return rc;
 }
 
+// The lean kernels keep three workgroups per CU: whether the table still
+// fits beside lds bytes of each (allocations rounded up to 1 KiB).
+static bool lean_table_fits(size_t lds, size_t tab) {
+  const size_t with = (lds + STATIC_LDS_BYTES + tab + 1023) / 1024 * 1024;
+  return 3 * with <= 160 * 1024;
+}
+
 // Whether the reciprocal table (tab bytes) fits beside a workgroup's lds
 // bytes without lowering the workgroups per CU below min(current, 2).
 static bool recip_table_fits(size_t lds, size_t tab) {
@@ -1184,6 +1199,7 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
   // out so that three workgroups fit a CU's LDS
   constexpr bool VARIANTS = FAMILY == PGLD_FAMILY_HAP && SET == PGLD_SET_SELF && P > 1 && NIB;
   const bool lean = VARIANTS && plan->n_regions > 0 && plan->kvar == 3;
+  if (lean) lds -= (size_t)opbuf_bytes<Planes<FAMILY, MISSING>::N>() - opbuf_bytes_swz<Planes<FAMILY, MISSING>::N>();
   bool tab = false;
   if constexpr (TAB) {
     int smax = 0;
@@ -1192,7 +1208,7 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
     else
       for (int p = 0; p < P; ++p) smax = std::max(smax, pa.n_samples[p]);
     const size_t bytes = (((size_t)smax + 1) * 8 + 15) / 16 * 16;
-    if (!lean && recip_table_fits(lds, bytes)) {
+    if (lean ? lean_table_fits(lds, bytes) : recip_table_fits(lds, bytes)) {
       tab = true;
       ba.rtab_n = smax;
       ba.rtab_off = (int32_t)lds;
@@ -1212,7 +1228,12 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
       hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV>), g, b, lds, s, ba, rd, ord, len, xs);
     };
     if constexpr (VARIANTS) {
-      if (lean) return go(std::integral_constant<int, KV_LEAN>{});
+      if (lean) {
+        if constexpr (TAB) {
+          if (tab) return go(std::integral_constant<int, KV_LEAN | KV_RTAB>{});
+        }
+        return go(std::integral_constant<int, KV_LEAN>{});
+      }
     }
     if constexpr (TAB) {
       if (tab) {

@@ -212,6 +212,35 @@ __device__ __forceinline__ OpBlock op_block(const int8_t* planes, int first_plan
   }
   return s;
 }
+// The same copies as buffer loads: one resource per side and plane (base in
+// scalar registers), the thread's 32-bit offset, the K offset as the scalar
+// offset -- no per-chunk address arithmetic on the VALU and no 64-bit
+// per-thread pointers kept live.  Every offset stays inside its 64-row block.
+template <int N>
+struct OpRsrc {
+  __amdgpu_buffer_rsrc_t r[2][N];
+};
+constexpr int BUF_RSRC_WORD3 = 0x00020000;  // raw buffer, 32-bit data format
+template <int N>
+__device__ __forceinline__ OpRsrc<N> op_rsrc(const OpBlock& s) {
+  OpRsrc<N> q;
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+      q.r[sd][a] = __builtin_amdgcn_make_buffer_rsrc((void*)(s.side[sd] + (int64_t)a * s.plane), (short)0,
+                                                     0x7fffffff, BUF_RSRC_WORD3);
+  return q;
+}
+template <int N>
+__device__ __forceinline__ void op_load_buf(v4i* pre, const OpRsrc<N>& q, const OpBlock& s, int k) {
+#pragma unroll
+  for (int m = 0; m < 4 * N; ++m) {
+    const int a = (m >> 1) % N, sd = (m >> 1) / N, h = m & 1;
+    pre[m] = __builtin_amdgcn_raw_buffer_load_b128(q.r[sd][a], s.off[sd][h], k, 0);
+  }
+}
+
 // The 2 N BT x 16-byte copies of the K chunk at byte offset k (uniform).
 template <int N>
 __device__ __forceinline__ void op_load(v4i* pre, const OpBlock& s, int64_t k) {
@@ -233,12 +262,23 @@ __device__ __forceinline__ void op_load(v4i* pre, const OpBlock& s, int64_t k) {
 // sub-step are read while the current one's MFMAs run (two fragment sets).
 // One stage and one fragment set keep fewer registers live, for kernels that
 // hide the latency with more waves per SIMD instead.
-template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false, int PFS = 0, bool FDB = true>
+// SWZ: operand rows unpadded (OP_ROW_SWZ bytes) with their 16-byte units
+// permuted by row (unit u of row r at u ^ (r mod 8)), which keeps the
+// MFMA-layout reads bank-conflict free in 1/9 less LDS at one XOR per
+// sub-step.
+constexpr int OP_ROW_SWZ = 128;
+template <int N>
+__host__ __device__ constexpr int opbuf_bytes_swz() { return 2 * N * BT * OP_ROW_SWZ; }
+
+template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false, int PFS = 0, bool FDB = true,
+          bool SWZ = false>
 __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
                                                int i0, int j0, int wr, int wc, bool wave_live,
                                                unsigned char* __restrict__ opbuf,
                                                WaveAcc<Planes<FAMILY, MISSING>::N, NIB>& acc) {
   static_assert(!(POPC && NIB), "bit planes and FP4 planes are separate layouts");
+  static_assert(!(POPC && SWZ), "the popcount form reads the padded layout");
+  constexpr int RS = SWZ ? OP_ROW_SWZ : OP_ROW;  // LDS row stride
   if constexpr (POPC) k_pad = bit_row_bytes(k_pad);  // from here on: bytes per operand row
   if constexpr (NIB) k_pad /= 2;
   using PL = Planes<FAMILY, MISSING>;
@@ -250,13 +290,19 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   // global address space keeps the loads off the flat path (whose counter
   // would make every LDS wait also wait for the prefetch)
   const OpBlock ob = op_block(planes, PL::FIRST, k_pad, n_var, i0, j0);
-  const int dbase = (tid >> 3) * OP_ROW + 16 * (tid & 7);
+  // copy m of thread t: row t / 8 + 32 (m % 2) of block m / 2, unit t % 8
+  // (swizzled by the row, whose low three bits are those of t / 8)
+  const int dbase = (tid >> 3) * RS + 16 * (SWZ ? ((tid & 7) ^ ((tid >> 3) & 7)) : (tid & 7));
+  // the MFMAs of the first sub-step start from zero accumulators; waves
+  // without MFMAs (and the popcount form) clear them here
+  if (POPC || !wave_live) {
 #pragma unroll
-  for (int a = 0; a < N; ++a)
+    for (int a = 0; a < N; ++a)
 #pragma unroll
-    for (int b = 0; b < N; ++b)
+      for (int b = 0; b < N; ++b)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+        for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+  }
   // register stages: the loads of chunk c + PF are in flight while the MFMAs
   // of chunk c run (PF = 2 where registers allow: up to 2 planes per side)
   constexpr int PF = PFS > 0 ? PFS : (N <= 2 ? 2 : 1);
@@ -265,7 +311,15 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   // chunk and is never used) so that every path keeps the same number of
   // loads in flight and the wait before a chunk's LDS store covers only that
   // chunk, not the younger prefetch.  k_pad is a multiple of KSTEP.
-  auto load = [&](v4i* pre, int64_t k) { op_load<N>(pre, ob, k < k_pad ? k : k_pad - KSTEP); };
+  [[maybe_unused]] const OpRsrc<N> rs = SWZ ? op_rsrc<N>(ob) : OpRsrc<N>{};
+  // k_pad <= MAX_K_PAD: K offsets fit an int
+  const int kp = (int)k_pad;
+  auto load = [&](v4i* pre, int k) {
+    if constexpr (SWZ)
+      op_load_buf<N>(pre, rs, ob, k < kp ? k : kp - KSTEP);
+    else
+      op_load<N>(pre, ob, k < kp ? k : kp - KSTEP);
+  };
   load(pre0, 0);
   if constexpr (PF == 2) load(pre1, KSTEP);
   // The two workgroups of a CU alternate between this loop (MFMA, LDS) and
@@ -273,10 +327,13 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   // instructions issue first when both contend, the epilogue fills the gaps.
   __builtin_amdgcn_s_setprio(2);
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
-  auto step = [&](v4i* pre, int64_t k0) {
+  const int c16 = ((lane & 7) << 4) ^ kh;
+  const int swz_a = (ra * RS) ^ c16, swz_b = ((N * BT + rb) * RS) ^ c16;
+  auto step = [&](auto first_tag, v4i* pre, int k0) {
+    constexpr bool FIRST = decltype(first_tag)::value;
     __syncthreads();  // the previous chunk has been read by every wave
 #pragma unroll
-    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dbase + 32 * OP_ROW * m) = pre[m];
+    for (int m = 0; m < ITEMS; ++m) *reinterpret_cast<v4i*>(opbuf + dbase + 32 * RS * m) = pre[m];
     __syncthreads();
     load(pre, k0 + PF * KSTEP);
     if (wave_live && POPC) {
@@ -313,10 +370,28 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
       constexpr int NF = FDB ? 2 : 1;
       v4i A[NF][N], B[NF][N];
       auto frag = [&](int st, v4i* a_, v4i* b_) {
+        if constexpr (SWZ) {
+          // unit 2 st + lane / 32 of rows ra, rb (both = lane mod 8, mod 8)
+          // sits at unit (2 st + lane / 32) ^ (lane mod 8): row bases are
+          // multiples of 128, so the address is base ^ c16 ^ 32 st with
+          // c16 = 16 (lane mod 8) ^ kh -- one XOR per sub-step and side (the
+          // empty asm operands keep the compiler from hoisting all of them
+          // into live registers)
+          int s32 = 32 * st;
+          asm volatile("" : "+s"(s32));
+          s32 &= 0x60;  // known bits again, so the LDS base folds into the offset field
+          const int xa = swz_a ^ s32, xb = swz_b ^ s32;
 #pragma unroll
-        for (int a = 0; a < N; ++a) {
-          a_[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * OP_ROW + 32 * st + kh);
-          b_[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * OP_ROW + 32 * st + kh);
+          for (int a = 0; a < N; ++a) {
+            a_[a] = *reinterpret_cast<const v4i*>(opbuf + xa + a * BT * RS);
+            b_[a] = *reinterpret_cast<const v4i*>(opbuf + xb + a * BT * RS);
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < N; ++a) {
+            a_[a] = *reinterpret_cast<const v4i*>(opbuf + ((0 * N + a) * BT + ra) * RS + 32 * st + kh);
+            b_[a] = *reinterpret_cast<const v4i*>(opbuf + ((1 * N + a) * BT + rb) * RS + 32 * st + kh);
+          }
         }
       };
       if constexpr (FDB) frag(0, A[0], B[0]);
@@ -331,27 +406,37 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
         for (int a = 0; a < N; ++a)
 #pragma unroll
           for (int b = 0; b < N; ++b) {
+            const bool zero = FIRST && st == 0;
             if constexpr (NIB) {
               const v4i x = A[st % NF][a], y = B[st % NF][b];
               const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
-              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc.c[a][b], 4, 4, 0,
-                                                                            FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
+              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                  x8, y8, zero ? v16f{} : acc.c[a][b], 4, 4, 0, FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
             } else {
-              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st % NF][a], B[st % NF][b], acc.c[a][b], 0, 0, 0);
+              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st % NF][a], B[st % NF][b],
+                                                                  zero ? v16i{} : acc.c[a][b], 0, 0, 0);
             }
           }
       }
     }
   };
+  // chunk c is staged from pre0 (c even) or pre1 (c odd); k_pad >= KSTEP
+  const std::true_type first;
+  const std::false_type later;
+  step(first, pre0, 0);
   if constexpr (PF == 2) {
-    int64_t k0 = 0;
-    for (; k0 + 2 * KSTEP <= k_pad; k0 += 2 * KSTEP) {
-      step(pre0, k0);
-      step(pre1, k0 + KSTEP);
+    int k0 = KSTEP;
+    if (k0 < kp) {
+      step(later, pre1, k0);
+      k0 += KSTEP;
     }
-    if (k0 < k_pad) step(pre0, k0);
+    for (; k0 + 2 * KSTEP <= kp; k0 += 2 * KSTEP) {
+      step(later, pre0, k0);
+      step(later, pre1, k0 + KSTEP);
+    }
+    if (k0 < kp) step(later, pre0, k0);
   } else {
-    for (int64_t k0 = 0; k0 < k_pad; k0 += KSTEP) step(pre0, k0);
+    for (int k0 = KSTEP; k0 < kp; k0 += KSTEP) step(later, pre0, k0);
   }
   __builtin_amdgcn_s_setprio(0);
 }
@@ -412,15 +497,19 @@ struct PopStream {
     const int nc = cur.nc;
     const int lane = threadIdx.x & 63;
     const int dbase = (threadIdx.x >> 3) * OP_ROW + 16 * (threadIdx.x & 7);
+    // the first chunk's MFMAs start from zero accumulators; idle waves clear them
+    if (!wave_live) {
 #pragma unroll
-    for (int a = 0; a < N; ++a)
+      for (int a = 0; a < N; ++a)
 #pragma unroll
-      for (int b = 0; b < N; ++b)
+        for (int b = 0; b < N; ++b)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+          for (int r = 0; r < 16; ++r) acc.c[a][b][r] = 0;
+    }
     __builtin_amdgcn_s_setprio(2);
     const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
-    auto mfmas = [&]() {
+    auto mfmas = [&](auto first_tag) {
+      constexpr bool FIRST = decltype(first_tag)::value;
       v4i A[2][N], B[2][N];
       auto frag = [&](int st, v4i* a_, v4i* b_) {
 #pragma unroll
@@ -437,13 +526,15 @@ struct PopStream {
         for (int a = 0; a < N; ++a)
 #pragma unroll
           for (int b = 0; b < N; ++b) {
+            const bool zero = FIRST && st == 0;
             if constexpr (NIB) {
               const v4i x = A[st & 1][a], y = B[st & 1][b];
               const v8i x8 = {x[0], x[1], x[2], x[3], 0, 0, 0, 0}, y8 = {y[0], y[1], y[2], y[3], 0, 0, 0, 0};
-              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x8, y8, acc.c[a][b], 4, 4, 0,
-                                                                            FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
+              acc.c[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                  x8, y8, zero ? v16f{} : acc.c[a][b], 4, 4, 0, FP4_UNIT_SCALE, 0, FP4_UNIT_SCALE);
             } else {
-              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b], acc.c[a][b], 0, 0, 0);
+              acc.c[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[st & 1][a], B[st & 1][b],
+                                                                  zero ? v16i{} : acc.c[a][b], 0, 0, 0);
             }
           }
       }
@@ -474,24 +565,26 @@ struct PopStream {
     };
     // chunk 0 outside the loop; pre1's late load (after a one-chunk
     // population) is issued once pre0 is in LDS
+    const std::true_type first;
+    const std::false_type later;
     stage(pre0);
     if (!pre1_ok && nc > 1) load(pre1, cur, 1);
     refill(pre0, 0);
-    if (wave_live) mfmas();
+    if (wave_live) mfmas(first);
     int c = 1;
 #pragma unroll 1
     for (; c + 2 <= nc; c += 2) {
       stage(pre1);
       refill(pre1, c);
-      if (wave_live) mfmas();
+      if (wave_live) mfmas(later);
       stage(pre0);
       refill(pre0, c + 1);
-      if (wave_live) mfmas();
+      if (wave_live) mfmas(later);
     }
     if (c < nc) {
       stage(pre1);
       refill(pre1, c);
-      if (wave_live) mfmas();
+      if (wave_live) mfmas(later);
     }
     pre1_ok = nc > 1;
     __builtin_amdgcn_s_setprio(0);

@@ -148,7 +148,7 @@ def test_multi_pop_hap(P):
     assert_close(got, O.multi_pop_stats_hap(cl, [None] * P))
 
 
-@pytest.mark.parametrize("P", [1, 2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
 def test_multi_pop_geno(P):
     rng = np.random.default_rng(20 + P)
     cl = _rand_counts(rng, 1200, P, 40, kind="geno")
