@@ -42,16 +42,18 @@ PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters'
 # reading of a 1 GiB device copy (--copy-calibration: FETCH x 2.000, WRITE x
-# 1.000): FP4 planes 158,867 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
+# 1.000): FP4 planes 156,913 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
 # Summary and commands: profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
-TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3294e9, "int8": 1.581e9}
-# The same for k_batch_sums on the 512-region ragged batch (FP4 planes):
-# FETCH_SIZE 2.35183e8 KiB x 2.000 + WRITE_SIZE 1.18279e6 KiB x 1.000 per
-# dispatch.  Summary and commands: profiles/k_batch_sums_ragged_512_regions.md
-RAGGED_TRAFFIC_BYTES = {"fp4": 482.9e9}
+TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3254e9, "int8": 1.581e9}
 # Issue counters of the same launch (SQ_INSTS_VALU, which counts the MFMAs
 # too, and SQ_INSTS_MFMA per dispatch): the VALU-issue bound of the kernel.
-RAGGED_SQ = {"fp4": {"valu_insts": 1.28954e11, "mfma_insts": 6.82821e9}}
+WINDOWED_SQ = {"fp4": {"valu_insts": 3.02461e8, "mfma_insts": 1.44901e7}}
+# The same for k_batch_sums on the 512-region ragged batch (FP4 planes, the
+# autotuner's lean variant): FETCH_SIZE 2.60426e8 KiB x 2.000 + WRITE_SIZE
+# 1.25658e6 KiB x 1.000 per dispatch, and its issue counters.
+# Summary and commands: profiles/k_batch_sums_ragged_512_regions.md
+RAGGED_TRAFFIC_BYTES = {"fp4": 534.6e9}
+RAGGED_SQ = {"fp4": {"valu_insts": 1.04372e11, "mfma_insts": 6.82821e9}}
 CLOCK_HZ = 2.4e9         # MI355X peak engine clock
 N_SIMD = 256 * 4         # 256 CUs x 4 SIMDs
 VALU_ISSUE_CYCLES = 4    # a wave64 VALU instruction (FP64 included) holds its SIMD's issue for 4 cycles
@@ -228,8 +230,11 @@ def run_windowed_hap(args, torch):
     value = pair_samples / (ms_step * 1e-3) / 1e9
     log(f"[bench] fused step {ms_step:.3f} ms")
     band_ms, band_min = band_kernel_ms(plan, packed, with_missing, max(3, args.steps))
+    default_shape = (n_var, n_hap) == (100_000, 2000)
     roof = mfma_roofline(2.0 * macs * pair_samples, band_ms, "k_band_sums", macs,
-                         TRAFFIC_BYTES_PER_LAUNCH[args.planes] if (n_var, n_hap) == (100_000, 2000) else None, fp4)
+                         TRAFFIC_BYTES_PER_LAUNCH[args.planes] if default_shape else None, fp4)
+    valu = (valu_issue_bound(WINDOWED_SQ[args.planes], band_ms)
+            if default_shape and args.planes in WINDOWED_SQ else None)
     log(f"[bench] k_band_sums {band_ms:.3f} ms (min {band_min:.3f}) -> {roof['achieved']:.1f} int8 TOP/s")
 
     # plain per-pair path on identical inputs and outputs (speedup vs plain)
@@ -289,7 +294,7 @@ def run_windowed_hap(args, torch):
                      "within 100 kb, 15 two-pop stats summed per 100 kb window (FP64)",
                      {"n_var": n_var, "n_hap": n_hap, "pairs": plan.n_pairs, "pair_samples": pair_samples,
                       "planes": args.planes}, dtype="fp4" if fp4 else "int8")
-    line.update({"roofline": roof, "plain_ms_per_step": plain_ms,
+    line.update({"roofline": roof, "valu_roofline": valu, "plain_ms_per_step": plain_ms,
                  "speedup_vs_plain": (plain_ms / ms_step) if plain_ms else None,
                  "fused_vs_plain_max_rel": check, "cpu_baseline": cpu,
                  "cpu_baseline_tiled": cpu_tiled if cpu else None})
