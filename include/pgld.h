@@ -371,7 +371,9 @@ int pgld_batch_schedule(pgld_band_plan* plan, int32_t grid, int32_t order, int32
 /* kernel_variant (bit-identical sums; used by the FP4 within-population
  * haplotype sets, ignored by the others): 0 = K loop streamed across the
  * populations of a tile, fast-path rows unrolled; 1 = one K loop per
- * population; 2 = streamed, rows in a rolled loop. */
+ * population; 2 = streamed, rows in a rolled loop; 3 = lean: one K loop per
+ * population with one load stage, registers and LDS capped for three
+ * workgroups per CU. */
 /* Autotuner: time the schedule and kernel-variant candidates on this plan,
  * keep the fastest.  The choice is cached per shape class (GPU architecture,
  * CUs, family, planes, populations, set, missing data, log2 of tiles, mean
