@@ -19,9 +19,12 @@ torch = pytest.importorskip("torch")
 if not torch.cuda.is_available():  # pragma: no cover - CPU container
     pytest.skip("no GPU", allow_module_level=True)
 
-from pg_gpu_amd import band, ld_pipeline as LP, ld_statistics as LS  # noqa: E402
+from pg_gpu_amd import band, batch, ld_pipeline as LP, ld_statistics as LS  # noqa: E402
 from pg_gpu_amd import synthetic as SY  # noqa: E402
 from pg_gpu_amd import _lib as L  # noqa: E402
+
+
+EDGES = np.logspace(0, 6, 17)
 
 
 def np_(t):
@@ -201,3 +204,102 @@ def test_genotype_panel_fp4_equals_int8_full_size(genotype_panel):
     s8 = np_(plan.sums([band.pack_population(g_d, None, "geno")]))
     s4 = np_(plan.sums([band.pack_population(g_d, None, "geno", fp4=True)]))
     np.testing.assert_array_equal(s4, s8)
+
+
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def dense_panel():
+    print("[fullsize] dense r^2 panel 40,000 SNPs x 10,000 haplotypes", flush=True)
+    return SY.dense_r2_panel()
+
+
+def _hap_r2_exact(n11, na, nb, S):
+    """r^2 from integer counts in float64, the kernel's own operation order:
+    (S n11 - nA nB)^2 / ((nA (S - nA)) (nB (S - nB)))."""
+    d = (S * n11 - na * nb).astype(np.float64)
+    den = (na * (S - na)).astype(np.float64) * (nb * (S - nb)).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(den > 0, (d * d) / den, np.nan)
+
+
+def test_dense_r2_full_size(dense_panel):
+    """All 7.9998e8 pairs of the 40,000 x 10,000 dense panel: the histogram
+    holds every pair once; FP4 planes give the int8 planes' histogram and
+    sparse list bit for bit; every listed pair (r^2 > 0.2) has the r^2 of its
+    exact counts recomputed independently (float64 products of the 0/1
+    columns); and on 40 sampled rows the list is complete -- every pair
+    (i, j > i) of those rows with r^2 > 0.2 is in it, no other."""
+    hap, pos = dense_panel
+    dev = L.device()
+    hap_d = torch.as_tensor(hap, device=dev)
+    S, n = hap.shape
+    plan = band.BandPlan(pos, np.inf, window=1e15, n_keys=1)
+    assert plan.n_pairs == n * (n - 1) // 2
+    kw = dict(n_hist=100, thresh=0.2, sparse_cap=1 << 22, with_missing=False)
+    out4 = plan.r2(band.pack_population(hap_d, None, "hap", fp4=True), **kw)
+    out8 = plan.r2(band.pack_population(hap_d, None, "hap"), **kw)
+    h4, h8 = np_(out4["hist"]), np_(out8["hist"])
+    np.testing.assert_array_equal(h4, h8)
+    np.testing.assert_array_equal(np_(out4["nan_pairs"]), np_(out8["nan_pairs"]))
+    assert int(h4.sum()) + int(np_(out4["nan_pairs"]).sum()) == plan.n_pairs
+    assert out4["n_sparse"] == out8["n_sparse"] == len(out4["sparse_i"]) > 0
+    for k in ("sparse_i", "sparse_j", "sparse_r2"):
+        np.testing.assert_array_equal(np_(out4[k]), np_(out8[k]))
+    si, sj, sr = (np_(out4[k]) for k in ("sparse_i", "sparse_j", "sparse_r2"))
+    # listed pairs against counts recomputed independently (float64 dot
+    # products of the 0/1 columns, exact)
+    na = np.sum(hap, axis=0, dtype=np.int64)
+    hf = hap_d.to(torch.float64)
+    ti, tj = (torch.as_tensor(x.astype(np.int64), device=dev) for x in (si, sj))
+    n11 = np_((hf[:, ti] * hf[:, tj]).sum(0).round()).astype(np.int64)
+    want = _hap_r2_exact(n11, na[si], na[sj], S)
+    assert np.all(want > 0.2)
+    np.testing.assert_array_equal(sr, want.astype(np.float32))
+    # completeness on sampled rows: all pairs (i, j > i) through a float64 GEMM
+    rows = np.sort(np.random.default_rng(11).choice(n - 1, size=40, replace=False))
+    n11_rows = np_((hf[:, rows].T @ hf).round()).astype(np.int64)  # 40 x n exact counts
+    listed = set(zip(si.tolist(), sj.tolist()))
+    for q, i in enumerate(rows):
+        j = np.arange(i + 1, n)
+        r2 = _hap_r2_exact(n11_rows[q, j], np.full(j.shape, na[i]), na[j], S)
+        hit = set((int(i), int(x)) for x in j[r2 > 0.2])
+        mine = set(p for p in listed if p[0] == i)
+        assert hit == mine, f"row {i}: {len(hit ^ mine)} pairs differ"
+
+
+def test_ragged_batch_full_size():
+    """The benchmark's 512-region batch (500-20,000 SNPs x 50-5,000 haplotypes
+    in 4 populations, 3 % missing, 3.5e10 pairs, 16 log bins): pair counts
+    per (region, bin) equal the closed form from the positions; FP4 planes
+    equal int8 planes bit for bit; the batch equals one fused launch per region
+    bit for bit; the two regions of least work against the FP64 oracle."""
+    print("[fullsize] ragged batch 512 regions", flush=True)
+    dev = L.device()
+    datas, pos, rows = SY.ragged_regions(n_regions=512, device=dev)
+    rows_d = [[torch.as_tensor(np.asarray(r, dtype=np.int32), device=dev) for r in rr] for rr in rows]
+    p4 = batch.pack_regions(datas, rows_d, fp4=True)
+    b = batch.RegionBatch(pos, 1e6, EDGES).bind(p4, stat_set="self")
+    assert b.n_pairs == sum(len(p) * (len(p) - 1) // 2 for p in pos)
+    b.schedule(0, "blocked", False, kernel_variant=3)  # the schedule and variant the autotuner picks here
+    s4, k4 = b.sums(True, want_pairs=True)
+    s4, k4 = np_(s4), np_(k4)
+    for r in range(len(pos)):
+        np.testing.assert_array_equal(k4[r], pairs_per_key_bins(pos[r], EDGES))
+    b8 = batch.RegionBatch(pos, 1e6, EDGES).bind(batch.pack_regions(datas, rows_d), stat_set="self")
+    np.testing.assert_array_equal(np_(b8.sums(True)), s4)
+    del b8
+    for r in range(len(pos)):
+        single = band.BandPlan(pos[r], 1e6, edges=EDGES)
+        np.testing.assert_array_equal(np_(single.sums(p4[r], stat_set="self", with_missing=True)), s4[r])
+    S = np.array([d.shape[0] for d in datas])
+    work = np.array([len(p) * (len(p) - 1) // 2 * int(s) for p, s in zip(pos, S)])
+    specs = [(t, [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(4)]
+    for r in np.argsort(work, kind="stable")[:2]:
+        h = np_(datas[r])
+        ii, jj = O.pairs_within_distance(pos[r], np.inf)
+        cl = [O.hap_counts(h, ii, jj, rw) for rw in rows[r]]
+        st = O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl], specs)
+        keys = O.bin_index(pos[r][jj] - pos[r][ii], EDGES)
+        want = O.keyed_sums(keys, st, 16)
+        scale = O.keyed_sums(keys, np.abs(st), 16)
+        assert np.all(np.abs(s4[r] - want) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300), f"region {r}"
