@@ -403,6 +403,20 @@ def test_pack_planes_domain_error(family, bad):
     band.pack_population(d, [r for r in range(70) if r != 13], family)
 
 
+def test_band_passes_reject_k_pad_beyond_32768():
+    """Operand offsets within a 64-row block are 24-bit products that assume
+    k_pad <= 32,768 (32,767 samples packed): larger paddings are an argument
+    error with a message, before any launch."""
+    d, pos = hap_data(105, 40, 200, 0.0)
+    plan = band.BandPlan(pos, 1e4, window=1e4)
+    pk = band.pack_population(d, None, "hap", k_pad=32768 + 128)
+    assert pk.k_pad == 32896
+    with pytest.raises(L.PgldError, match="k_pad must be at most 32768"):
+        plan.sums([pk])
+    with pytest.raises(L.PgldError, match="k_pad must be at most 32768"):
+        plan.r2(pk)
+
+
 # ---------------------------------------------------------------------------
 # FP4 planes and the FP4-MFMA fused sums
 # ---------------------------------------------------------------------------
