@@ -139,6 +139,9 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   constexpr bool FAST_UNROLL = (HAP_SELF || HAP2) && !(KV & KV_ROLLED);
   // rows per scheduling group of the unrolled fast path
   constexpr int FAST_GROUP = HAP2 ? PGLD_HAP2_GROUP : 2;
+  // statistics of every pair are finite (integer numerators times reciprocals
+  // that are 0 where a falling factorial is 0)
+  constexpr bool FINITE_STATS = HAP_SELF || HAP2;
   static_assert(!(KV & KV_RTAB) || (HAP_SELF && MISSING), "reciprocal table: within-population sets, missing data");
   // 1 / (n)_4 of a pair with n valid samples (n <= the population's size <= rtab_n)
   const double* rtab = reinterpret_cast<const double*>(smem + ba.rtab_off);
@@ -235,12 +238,20 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   // A sub-tile whose pairs are all valid (j > i everywhere, inside the panel,
   // the farthest pair within max_dist) and share one key (keys are monotone
   // in the left SNP's window and in the distance, so the extreme pairs decide)
-  // needs no per-pair key or validity work: fast_key is that key, else -1.
-  int fast_key = -1;
+  // needs no per-pair key or validity work.
+  // A sub-tile whose pairs are all valid and fall in two consecutive keys
+  // (duo_key, duo_key + 1) is summed into two running sets and flushed once
+  // (sets with finite statistics; the others take the per-pair path):
+  // a pair goes to the second key when its left SNP is in window duo_key + 1
+  // (windows) or its key distance is at least edges[duo_key + 1], the edge
+  // between the two bins (the bin search's own comparison).
+  // Both cases are kept in one wave-uniform scalar through the K loops:
+  // key_sel = 2 * key (one key) or 2 * key + 1 (two keys), -1 otherwise.
+  int key_sel = -1;
   if (wave_live && rj > ri + 31 && (int64_t)rj + 31 < ba.n_var && s_pos_j[wc + 31] - s_pos_i[wr] <= ba.max_dist) {
     if (ba.key_mode == PGLD_KEY_WINDOW) {
       const int k0 = s_wkey[wr], k1 = s_wkey[wr + 31];
-      if (k0 >= 0 && k0 == k1) fast_key = k0;
+      if (k0 >= 0 && (k1 == k0 || k1 == k0 + 1)) key_sel = 2 * k0 + (k1 - k0);
     } else {
       const double d_lo = s_kc_j[wc] - s_kc_i[wr + 31], d_hi = s_kc_j[wc + 31] - s_kc_i[wr];
       if (d_lo >= e_lo && d_hi < e_hi) {
@@ -250,10 +261,11 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         const int b_hi = klo == khi ? klo
                                     : (edges_in_lds ? key_bin_in(d_hi, s_edges, klo, khi + 1)
                                                     : key_bin_in(d_hi, ba.edges, klo, khi + 1));
-        if (b_lo == b_hi) fast_key = b_lo;
+        if (b_hi - b_lo <= 1) key_sel = 2 * b_lo + (b_hi - b_lo);
       }
     }
   }
+  key_sel = __builtin_amdgcn_readfirstlane(key_sel);  // every input above is wave-uniform
   auto epilogue = [&](auto m_tag, int col0, int cstride, bool count, int kmode, auto&& eval) {
     constexpr int M = decltype(m_tag)::value;
     double lacc[M + 1];
@@ -288,7 +300,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         ck = -1;
       }
     };
-    if (fast_key >= 0) {
+    if (key_sel >= 0 && !(key_sel & 1)) {
       // every lane holds 16 valid pairs of one key
       auto row = [&](int r) {
         const int64_t i = (int64_t)tile.i0 + wr + acc_row(r, lane);
@@ -316,18 +328,97 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
 #pragma unroll 1
         for (int r = 0; r < 16; ++r) row(r);
       }
-      // This wave's row for fast_key takes exactly one flush per epilogue
+      // This wave's row for that key takes exactly one flush per epilogue
       // call and the calls write disjoint columns, so the totals are stored
       // without reading the row back (x + 0.0 is what adding x into the
       // cleared row gives).
       lacc[M] = 16.0;
-      double* frow = wred + (fast_key - klo) * NS;
+      double* frow = wred + ((key_sel >> 1) - klo) * NS;
       if (count)
         wave_sums_scatter<M + 1, SCAT>(lacc, lane, [&](int s, double x) {
           frow[s < M ? col0 + s * cstride : NS - 1] = x + 0.0;
         });
       else
         wave_sums_scatter<M, SCAT>(lacc, lane, [&](int s, double x) { frow[col0 + s * cstride] = x + 0.0; });
+      return;
+    }
+    if (FINITE_STATS && key_sel >= 0) {
+      // every lane holds 16 valid pairs, each of key duo_key (running sums
+      // lacc) or duo_key + 1 (lsec).  Keys are monotone in the left SNP, and
+      // the lane's 16 rows ascend with r, so the second key's pairs are the
+      // lane's last n_sec rows (windows) or its first n_sec rows (bins, the
+      // larger distances): n_sec is counted once per sub-tile, with the
+      // per-pair tests of the bin search and the window key, and kept in
+      // kcode between passes.  The statistics are finite, so each pair is
+      // weighted by 1 / 0 into both sets: fma(1, x, acc) = acc + x and
+      // fma(0, x, acc) = acc.  (Sets whose statistics can be NaN take the
+      // per-pair path.)
+      const int duo_key = key_sel >> 1;
+      const bool win = ba.key_mode == PGLD_KEY_WINDOW;
+      int n_sec = 0;
+      if (kmode == 2) {
+        n_sec = (int)kcode[0];
+      } else {
+        if (win) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) n_sec += s_wkey[wr + acc_row(r, lane)] != duo_key ? 1 : 0;
+        } else {
+          const double edge = edges_in_lds ? s_edges[duo_key + 1] : ba.edges[duo_key + 1];
+          const double kc_j = s_kc_j[jl];
+#pragma unroll
+          for (int r = 0; r < 16; ++r) n_sec += edge <= kc_j - s_kc_i[wr + acc_row(r, lane)] ? 1 : 0;
+        }
+        if (kmode == 1) kcode[0] = (uint32_t)n_sec;
+      }
+      // the second key's rows: r in [sec_lo, sec_lo + n_sec)
+      const int sec_lo = win ? 16 - n_sec : 0;
+      double lsec[M];
+#pragma unroll
+      for (int s = 0; s < M; ++s) lsec[s] = 0.0;
+      auto row = [&](int r) {
+        const int64_t i = (int64_t)tile.i0 + wr + acc_row(r, lane);
+        const bool sec = (unsigned)(r - sec_lo) < (unsigned)n_sec;
+        double out[M];
+        eval(r, i, out);
+        const double wb = sec ? 1.0 : 0.0, wa = sec ? 0.0 : 1.0;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          lacc[s] = fma(wa, out[s], lacc[s]);
+          lsec[s] = fma(wb, out[s], lsec[s]);
+        }
+      };
+      if constexpr (FAST_UNROLL) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          row(r);
+          if ((r + 1) % FAST_GROUP == 0) {
+#pragma unroll
+            for (int q = 0; q < M; ++q) asm volatile("" : "+v"(lacc[q]), "+v"(lsec[q]));
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      } else {
+#pragma unroll 1
+        for (int r = 0; r < 16; ++r) row(r);
+      }
+      // both key rows take exactly one flush from this wave per epilogue
+      // call (as in the one-key case above), one row after the other
+      double* frow = wred + (duo_key - klo) * NS;
+      auto put = [&](const double* vals, double* krow) {
+        if (count)
+          wave_sums_scatter<M + 1, SCAT>(vals, lane, [&](int s, double x) {
+            krow[s < M ? col0 + s * cstride : NS - 1] = x + 0.0;
+          });
+        else
+          wave_sums_scatter<M, SCAT>(vals, lane, [&](int s, double x) { krow[col0 + s * cstride] = x + 0.0; });
+      };
+      lacc[M] = (double)(16 - n_sec);
+      put(lacc, frow);
+      double vs[M + 1];
+#pragma unroll
+      for (int s = 0; s < M; ++s) vs[s] = lsec[s];
+      vs[M] = (double)n_sec;
+      put(vs, frow + NS);
       return;
     }
 #pragma unroll 1

@@ -19,6 +19,7 @@ if not torch.cuda.is_available():  # pragma: no cover - CPU container
 from pg_gpu_amd import band, ld_pipeline as LP, ld_statistics as LS  # noqa: E402
 from pg_gpu_amd import haplotype_kernels as HK, genotype_kernels as GK  # noqa: E402
 from pg_gpu_amd import synthetic as SY  # noqa: E402
+from pg_gpu_amd import _lib as L  # noqa: E402
 
 
 def np_(t):
@@ -256,6 +257,59 @@ def test_band_sums_bins(P, missing):
     stats, keys = _sums_oracle("hap", d, pos, pops, md, "moments",
                                lambda ii, jj: O.bin_index(pos[jj] - pos[ii], edges))
     _check_sums(sums, stats, keys, len(edges) - 1)
+
+
+def _two_key_subtiles(pos, md, key_of_pair, n_var):
+    """Count the 32 x 32 sub-tiles (rows and columns at multiples of 32) that
+    lie strictly above the diagonal and inside the panel, whose farthest pair
+    is within md and whose pairs span exactly two consecutive keys (the
+    extreme pairs decide: keys are monotone in the left SNP and the
+    distance)."""
+    nb = n_var // 32
+    n = 0
+    for bi in range(nb):
+        i0, i1 = 32 * bi, 32 * bi + 31
+        for bj in range(bi + 1, nb):
+            j0, j1 = 32 * bj, 32 * bj + 31
+            if pos[j1] - pos[i0] > md:
+                break
+            k = sorted({int(key_of_pair(i0, j1)), int(key_of_pair(i1, j0))})
+            n += len(k) == 2 and k[0] >= 0 and k[1] == k[0] + 1
+    return n
+
+
+@pytest.mark.parametrize("key", ["bins", "windows"])
+@pytest.mark.parametrize("P,stat_set,missing,fp4", [(1, "moments", True, True), (1, "moments", False, True),
+                                                    (3, "self", True, True), (3, "self", False, False),
+                                                    (2, "moments", True, True), (2, "moments", False, False)])
+def test_band_sums_two_key_subtiles(key, P, stat_set, missing, fp4):
+    """Sub-tiles whose pairs fall in two consecutive bins or windows are
+    summed into both keys at once: with SNPs 10 apart, bins 900 wide and
+    windows 450 wide every sub-tile holds one or two keys, and 150 or more
+    hold two.  Sums within 1e-12 of the oracle's per-pair statistics, pair counts
+    exact."""
+    n_var = 1216
+    rng = np.random.default_rng(900 + 10 * P + missing)
+    pos = 10.0 * np.arange(n_var) + rng.integers(0, 4, n_var)
+    d = SY.neutral_haplotypes(30 * P + 11, n_var, rng, missing=0.03 if missing else 0.0)
+    pops = [list(range(p, d.shape[0], P)) for p in range(P)]
+    packed = [band.pack_population(d, r, "hap", fp4=fp4) for r in pops]
+    if key == "bins":
+        edges = 905.0 + 900.0 * np.arange(14)
+        md = float(edges[-1])
+        plan = band.BandPlan(pos, md, edges=edges)
+        n_keys = len(edges) - 1
+        kfun = lambda ii, jj: O.bin_index(pos[jj] - pos[ii], edges)  # noqa: E731
+    else:
+        md, width = 3000.0, 450.0
+        plan = band.BandPlan(pos, md, window=width)
+        n_keys = plan.n_keys
+        kfun = lambda ii, jj: O.window_key(pos[ii], 0.0, width)  # noqa: E731
+    assert _two_key_subtiles(pos, md, lambda i, j: kfun(np.array([i]), np.array([j]))[0], n_var) >= 150
+    sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
+    stats, keys = _sums_oracle("hap", d, pos, pops, md, stat_set, kfun)
+    _check_sums(sums, stats, keys, n_keys)
+    np.testing.assert_array_equal(np_(kp), np.bincount(keys[(keys >= 0) & (keys < n_keys)], minlength=n_keys))
 
 
 @pytest.mark.parametrize("P,missing", [(1, True), (1, False), (2, True)])
