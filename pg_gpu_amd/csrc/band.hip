@@ -139,9 +139,10 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   constexpr bool FAST_UNROLL = (HAP_SELF || HAP2) && !(KV & KV_ROLLED);
   // rows per scheduling group of the unrolled fast path
   constexpr int FAST_GROUP = HAP2 ? PGLD_HAP2_GROUP : 2;
-  // statistics of every pair are finite (integer numerators times reciprocals
-  // that are 0 where a falling factorial is 0)
-  constexpr bool FINITE_STATS = HAP_SELF || HAP2;
+  // statistics of every pair are finite: integer numerators times reciprocals
+  // that are 0 where a falling factorial is 0 (haplotype sets), or divided by
+  // (n)_4 only where n >= 4 (one genotype population)
+  constexpr bool FINITE_STATS = HAP_SELF || HAP2 || (FAMILY == PGLD_FAMILY_GENO && P == 1);
   static_assert(!(KV & KV_RTAB) || (HAP_SELF && MISSING), "reciprocal table: within-population sets, missing data");
   // 1 / (n)_4 of a pair with n valid samples (n <= the population's size <= rtab_n)
   const double* rtab = reinterpret_cast<const double*>(smem + ba.rtab_off);

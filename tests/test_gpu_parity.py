@@ -279,10 +279,11 @@ def _two_key_subtiles(pos, md, key_of_pair, n_var):
 
 
 @pytest.mark.parametrize("key", ["bins", "windows"])
-@pytest.mark.parametrize("P,stat_set,missing,fp4", [(1, "moments", True, True), (1, "moments", False, True),
-                                                    (3, "self", True, True), (3, "self", False, False),
-                                                    (2, "moments", True, True), (2, "moments", False, False)])
-def test_band_sums_two_key_subtiles(key, P, stat_set, missing, fp4):
+@pytest.mark.parametrize("family,P,stat_set,missing,fp4", [
+    ("hap", 1, "moments", True, True), ("hap", 1, "moments", False, True), ("hap", 3, "self", True, True),
+    ("hap", 3, "self", False, False), ("hap", 2, "moments", True, True), ("hap", 2, "moments", False, False),
+    ("geno", 1, "moments", True, True), ("geno", 1, "moments", False, False)])
+def test_band_sums_two_key_subtiles(key, family, P, stat_set, missing, fp4):
     """Sub-tiles whose pairs fall in two consecutive bins or windows are
     summed into both keys at once: with SNPs 10 apart, bins 900 wide and
     windows 450 wide every sub-tile holds one or two keys, and 150 or more
@@ -291,9 +292,12 @@ def test_band_sums_two_key_subtiles(key, P, stat_set, missing, fp4):
     n_var = 1216
     rng = np.random.default_rng(900 + 10 * P + missing)
     pos = 10.0 * np.arange(n_var) + rng.integers(0, 4, n_var)
-    d = SY.neutral_haplotypes(30 * P + 11, n_var, rng, missing=0.03 if missing else 0.0)
+    if family == "hap":
+        d = SY.neutral_haplotypes(30 * P + 11, n_var, rng, missing=0.03 if missing else 0.0)
+    else:
+        d = SY.hwe_genotypes(41, n_var, rng, missing=0.03 if missing else 0.0)
     pops = [list(range(p, d.shape[0], P)) for p in range(P)]
-    packed = [band.pack_population(d, r, "hap", fp4=fp4) for r in pops]
+    packed = [band.pack_population(d, r, family, fp4=fp4) for r in pops]
     if key == "bins":
         edges = 905.0 + 900.0 * np.arange(14)
         md = float(edges[-1])
@@ -307,7 +311,7 @@ def test_band_sums_two_key_subtiles(key, P, stat_set, missing, fp4):
         kfun = lambda ii, jj: O.window_key(pos[ii], 0.0, width)  # noqa: E731
     assert _two_key_subtiles(pos, md, lambda i, j: kfun(np.array([i]), np.array([j]))[0], n_var) >= 150
     sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
-    stats, keys = _sums_oracle("hap", d, pos, pops, md, stat_set, kfun)
+    stats, keys = _sums_oracle(family, d, pos, pops, md, stat_set, kfun)
     _check_sums(sums, stats, keys, n_keys)
     np.testing.assert_array_equal(np_(kp), np.bincount(keys[(keys >= 0) & (keys < n_keys)], minlength=n_keys))
 
