@@ -160,7 +160,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   __shared__ int s_wkey[BT];
   __shared__ double s_edges[MAX_LDS_EDGES];
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // the wave index (and so the sub-tile's offsets) is wave-uniform: scalar
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   // The lean variant puts the operand buffer first, at a fixed LDS address
   // (its swizzled addresses then need no per-tile base), the reduction rows
   // after it.
@@ -210,7 +211,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     s_kc_j[tid] = v_kj;
     if (ba.key_mode == PGLD_KEY_WINDOW) {
       const double wk = floor((v_ki - ba.origin) / ba.width);
-      s_wkey[tid] = (wk >= 0.0 && wk < (double)ba.n_keys) ? (int)wk : -1;
+      // wk in [0, n_keys): compared as an int once it is known to fit one
+      s_wkey[tid] = (wk >= 0.0 && wk < 2147483648.0 && (int)wk < ba.n_keys) ? (int)wk : -1;
     }
   }
   if (edges_in_lds && tid <= ba.n_keys) s_edges[tid] = v_e;
