@@ -42,18 +42,18 @@ PEAK_HBM = 8.0e12        # bytes/s
 # HBM bytes per k_band_sums launch on the default workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters'
 # reading of a 1 GiB device copy (--copy-calibration: FETCH x 2.000, WRITE x
-# 1.000): FP4 planes 156,913 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
+# 1.000): FP4 planes 158,541 + 3,908 KiB, int8 planes 769,840 + 3,908 KiB.
 # Summary and commands: profiles/k_band_sums_hap2pop_windows_100k_snps_2000_haps.md
-TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3254e9, "int8": 1.581e9}
+TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3287e9, "int8": 1.581e9}
 # Issue counters of the same launch (SQ_INSTS_VALU, which counts the MFMAs
 # too, and SQ_INSTS_MFMA per dispatch): the VALU-issue bound of the kernel.
-WINDOWED_SQ = {"fp4": {"valu_insts": 3.02461e8, "mfma_insts": 1.44901e7}}
+WINDOWED_SQ = {"fp4": {"valu_insts": 3.01273e8, "mfma_insts": 1.44901e7}}
 # The same for k_batch_sums on the 512-region ragged batch (FP4 planes, the
-# autotuner's lean variant): FETCH_SIZE 2.60426e8 KiB x 2.000 + WRITE_SIZE
-# 1.25658e6 KiB x 1.000 per dispatch, and its issue counters.
+# autotuner's lean variant): FETCH_SIZE 2.32834e8 KiB x 2.000 + WRITE_SIZE
+# 1.24309e6 KiB x 1.000 per dispatch, and its issue counters.
 # Summary and commands: profiles/k_batch_sums_ragged_512_regions.md
-RAGGED_TRAFFIC_BYTES = {"fp4": 534.6e9}
-RAGGED_SQ = {"fp4": {"valu_insts": 1.04372e11, "mfma_insts": 6.82821e9}}
+RAGGED_TRAFFIC_BYTES = {"fp4": 478.1e9}
+RAGGED_SQ = {"fp4": {"valu_insts": 9.98258e10, "mfma_insts": 6.82821e9}}
 CLOCK_HZ = 2.4e9         # MI355X peak engine clock
 N_SIMD = 256 * 4         # 256 CUs x 4 SIMDs
 VALU_ISSUE_CYCLES = 4    # a wave64 VALU instruction (FP64 included) holds its SIMD's issue for 4 cycles

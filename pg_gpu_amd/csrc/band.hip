@@ -654,7 +654,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band_counts(BandArgs ba, PopAr
   constexpr int NC = PL::NC;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const BandTile tile = ba.tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int ri = tile.i0 + wr, rj = tile.j0 + wc;
   const bool wave_live = (int64_t)rj < ba.n_var && (int64_t)ri < ba.n_var && rj + 31 > ri;

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2(R2Args ra, const in
   __shared__ int s_rcnt[2][BT], s_ccnt[2][BT];
 
   const BandTile tile = ra.tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int hw = ra.n_hist + 1;
   const int h_n = ra.hist ? tile.n_slots * hw : 0;
   for (int t = tid; t < h_n; t += BAND_THREADS) s_hist[t] = 0;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(BAND_THREADS, 2) void k_band_r2_wide(R2Args ra, con
   __shared__ int s_hist[R2_MAX_HIST_LDS];
 
   const SuperTile st = sts[xcd_tile(blockIdx.x, gridDim.x)];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int hw = ra.n_hist + 1;
   const int own = w == 0 ? st.t[0] : w == 1 ? st.t[1] : w == 2 ? st.t[2] : st.t[3];
   const BandTile tile = own >= 0 ? ra.tiles[own] : BandTile{};
