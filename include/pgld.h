@@ -198,7 +198,7 @@ int pgld_band_plan_destroy(pgld_band_plan* plan);
 int pgld_band_plan_set_timing(pgld_band_plan* plan, int32_t enable);
 int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
 
-#define PGLD_SET_MOMENTS 0 /* ld_names(n_pops): 3 / 15 / 45 statistics for 1 / 2 / 3 pops */
+#define PGLD_SET_MOMENTS 0 /* ld_names(n_pops): 3 / 15 / 45 / 105 statistics for 1 / 2 / 3 / 4 pops */
 #define PGLD_SET_SELF 1    /* DD_i_i, Dz_i_i_i, pi2_i_i_i_i per population (3 * n_pops) */
 
 /* Fused band pass: int8 MFMA counts for every pair of the band, every
@@ -210,8 +210,8 @@ int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
  * (device pointers for planes and marg); n_samples <= 32767 and k_pad <= 32768
  * per population (as packed).  sums (device, n_keys x n_stats) is
  * overwritten; key_pairs (device, n_keys, may be NULL) receives the number of
- * pairs per key.  Haplotypes: moments set for n_pops <= 3, self set for
- * n_pops <= 4; genotypes: n_pops <= 2.  Replaces the
+ * pairs per key.  Haplotypes: n_pops <= 4 (the 105 four-population moments
+ * are summed in three column groups); genotypes: n_pops <= 2.  Replaces the
  * chunk loop of moments_ld._compute_ld_sums (moments_ld.py:248-343) and
  * HaplotypeMatrix.compute_ld_statistics_gpu_* (haplotype_matrix.py:
  * 1592-1866). */

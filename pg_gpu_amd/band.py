@@ -294,12 +294,13 @@ FUSED_MAX_SAMPLES = 32767  # int16 count stash and 24-bit products of the fused 
 
 def fused_supported(family, n_pops, stat_set, pop_sizes):
     """Whether the fused band kernels cover this statistic set: haplotype
-    moments sets for 1-3 populations, within-population sets for 1-4;
-    genotype sets for 1-2; at most FUSED_MAX_SAMPLES samples per population."""
+    sets for 1-4 populations (the 105 four-population moments in three column
+    groups); genotype sets for 1-2 (three or four populations' parked nine
+    counts exceed LDS); at most FUSED_MAX_SAMPLES samples per population."""
     if max(pop_sizes) > FUSED_MAX_SAMPLES:
         return False
     if family == "hap":
-        return n_pops <= (4 if stat_set == "self" else 3)
+        return n_pops <= 4
     return n_pops <= 2
 
 

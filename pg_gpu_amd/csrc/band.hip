@@ -21,6 +21,7 @@
 // identical from run to run.
 #include <algorithm>
 #include <array>
+#include <utility>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -69,6 +70,45 @@ PGLD_STATSETS(2)
 PGLD_STATSETS(3)
 PGLD_STATSETS(4)
 #undef PGLD_STATSETS
+
+// The fused kernel sums a statistic set in groups of consecutive columns,
+// one pass over the sub-tile's pairs per group, so that a group's running
+// sums stay in registers: one group except for the 105 four-population
+// haplotype moments (three groups of 35).
+template <int FAMILY, int P, int SET>
+struct StatGroups {
+  using SS = StatSet<FAMILY, P, SET>;
+  static constexpr int NG = 1;
+  template <int G>
+  static constexpr int off() { return 0; }
+  template <int G>
+  static constexpr int count() { return SS::N; }
+  template <int G, typename V>
+  __device__ static void eval(const V* v, double* out) { SS::eval(v, out); }
+};
+template <>
+struct StatGroups<PGLD_FAMILY_HAP, 4, PGLD_SET_MOMENTS> {
+  static constexpr int NG = EPI_HAP_MOMENTS_4_NG;
+  static_assert(NG == 3, "three generated groups");
+  template <int G>
+  static constexpr int off() { return EPI_HAP_MOMENTS_4_GOFF[G]; }
+  template <int G>
+  static constexpr int count() { return EPI_HAP_MOMENTS_4_GOFF[G + 1] - EPI_HAP_MOMENTS_4_GOFF[G]; }
+  template <int G>
+  __device__ static void eval(const HapP* v, double* out) {
+    if constexpr (G == 0)
+      epi_hap_moments_4_g0(v, out);
+    else if constexpr (G == 1)
+      epi_hap_moments_4_g1(v, out);
+    else
+      epi_hap_moments_4_g2(v, out);
+  }
+};
+
+template <typename F, int... G>
+__device__ __forceinline__ void for_each_group(std::integer_sequence<int, G...>, F&& f) {
+  (f(std::integral_constant<int, G>{}), ...);
+}
 
 template <int FAMILY>
 struct PopVal;
@@ -540,8 +580,15 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         for (int q = 0; q < NC; ++q) stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane] = (int16_t)c[q];
       }
     }
+    using SG = StatGroups<FAMILY, P, SET>;
+    // with several groups the first group's pass records the pair keys and
+    // the others reuse them
     if (wave_live)
-      epilogue(std::integral_constant<int, SS::N>{}, 0, 1, true, 0, [&](int r, int64_t i, double* out) {
+      for_each_group(std::make_integer_sequence<int, SG::NG>{}, [&](auto g_tag) {
+      constexpr int G = decltype(g_tag)::value;
+      const int kmode = SG::NG == 1 || tile.n_slots > 255 ? 0 : (G == 0 ? 1 : 2);
+      epilogue(std::integral_constant<int, SG::template count<G>()>{}, SG::template off<G>(), 1, G == 0, kmode,
+               [&](int r, int64_t i, double* out) {
         if constexpr (HAP_SELF && P == 1) {
           double n, a, b, s;
           hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
@@ -567,8 +614,9 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
           int c[NC];
           elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
           V[P - 1] = PopVal<FAMILY>::make(c);
-          SS::eval(V, out);
+          SG::template eval<G>(V, out);
         }
+      });
       });
   }
   __syncthreads();
@@ -1366,6 +1414,7 @@ static SumsLauncher pick_launcher(int P, int set) {
       if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
       if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS, NIB>;
       if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_MOMENTS, NIB>;
+      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_MOMENTS, NIB>;
     } else {
       if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
       if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF, NIB>;

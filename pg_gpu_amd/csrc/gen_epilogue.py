@@ -176,13 +176,17 @@ def call_expr(fam, t, q, sym):
     return "%s_pi2_%s(%s)" % (pre, case, args)
 
 
-def emit_hap_set(P, names, sym, fname):
+def emit_hap_set(P, names, sym, fname, first=0, count=None):
     """Haplotype statistics: each distinct numerator polynomial is evaluated
     once (p<k>), each distinct reciprocal product once (r<m>) and each distinct
     (reciprocal product x constant) once (k<m>).  A statistic whose terms share
     one reciprocal product and weight adds its (exact, integer-valued)
-    numerators before the single scaling."""
+    numerators before the single scaling.  With first / count only statistics
+    first .. first + count - 1 are written (to out[0 .. count - 1]), with only
+    the terms they use; the scale function always covers the whole set."""
     polys, dens, coefs, lines, body, scales = {}, {}, {}, [], [], []
+    if count is None:
+        count = len(names) - first
 
     def poly(e):
         if e not in polys:
@@ -215,23 +219,43 @@ def emit_hap_set(P, names, sym, fname):
         for w, t, q in terms_of(kind, p):
             key = hap_call(t, q, sym)
             weights[key] = weights.get(key, 0.0) + w
-        groups = {}  # (reciprocal product, weight x factor) -> numerators
+        groups = {}  # (reciprocal product, weight x factor) -> numerator expressions
         for (num, fac, den), w in weights.items():
-            groups.setdefault((" * ".join(sorted(den.split(" * "))), w * fac), []).append(poly(num))
+            groups.setdefault((" * ".join(sorted(den.split(" * "))), w * fac), []).append(num)
         # the smallest power-of-two factor of the statistic is left out of the
         # per-pair value and applied to the tile's partial sums (scales[s]);
         # scaling by a power of two commutes with FP64 rounding and addition,
         # so the sums are the same to the bit
         m = min(c for (_, c) in groups)
         scales.append(m)
+        if not first <= s < first + count:
+            continue
         parts = []
         for (den, c), nums in groups.items():
-            total = nums[0] if len(nums) == 1 else "(%s)" % " + ".join(nums)
+            names_ = [poly(x) for x in nums]
+            total = names_[0] if len(names_) == 1 else "(%s)" % " + ".join(names_)
             parts.append("%s * %s" % (total, coef(den, c / m)))
-        body.append("  out[%d] = %s;" % (s, " + ".join(parts)))
+        body.append("  out[%d] = %s;" % (s - first, " + ".join(parts)))
     head = "__device__ __forceinline__ void %s(const HapP* H, double* out) {" % fname
-    pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in range(P)]
-    return "\n".join([head] + pre + lines + body + ["}", scale_fn(fname, scales)]), len(names)
+    whole = (first, count) == (0, len(names))
+    used = [p for p in range(P) if whole or any("R[%d]" % p in ln for ln in lines + body)]
+    pre = ["  HapR R[%d];" % P] + ["  R[%d] = hap_recips(H[%d].n);" % (p, p) for p in used]
+    tail = ["}", scale_fn(fname, scales)] if whole else ["}"]
+    return "\n".join([head] + pre + lines + body + tail), len(names)
+
+
+def emit_hap_groups(P, names, sym, fname, n_groups):
+    """The set split into n_groups functions fname_g<k> of consecutive
+    statistics (the fused kernel sums one group at a time, so its running sums
+    stay in registers), with their first statistics in fname_GOFF."""
+    size = -(-len(names) // n_groups)
+    offs = [min(len(names), k * size) for k in range(n_groups + 1)]
+    parts = ["constexpr int %s_NG = %d;" % (fname.upper(), n_groups),
+             "constexpr int %s_GOFF[%d] = {%s};" % (fname.upper(), n_groups + 1, ", ".join(map(str, offs)))]
+    for k in range(n_groups):
+        code, _ = emit_hap_set(P, names, sym, "%s_g%d" % (fname, k), offs[k], offs[k + 1] - offs[k])
+        parts.append(code)
+    return "\n\n".join(parts)
 
 
 def scale_fn(fname, scales):
@@ -292,6 +316,8 @@ def main(out_path):
             code, n = emit_set(fam, P, ld_names(P), fam == "hap" and P == 2, "epi_%s_moments_%d" % (fam, P))
             parts.append("constexpr int EPI_%s_MOMENTS_%d_N = %d;" % (fam.upper(), P, n))
             parts.append(code)
+            if fam == "hap" and P == 4:
+                parts.append(emit_hap_groups(P, ld_names(P), False, "epi_hap_moments_4", 3))
             code, n = emit_set(fam, P, self_names(P), False, "epi_%s_self_%d" % (fam, P))
             parts.append("constexpr int EPI_%s_SELF_%d_N = %d;" % (fam.upper(), P, n))
             parts.append(code)

@@ -81,10 +81,11 @@ def test_moments_genotypes_im_example(method):
     np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
 
 
-@pytest.mark.parametrize("P,method", [(1, "auto"), (3, "auto"), (3, "plain"), (4, "auto")])
+@pytest.mark.parametrize("P,method", [(1, "auto"), (3, "auto"), (3, "plain"), (4, "auto"), (4, "plain")])
 def test_moments_hap_pop_counts(P, method):
-    """1, 3 and 4 populations on a synthetic matrix with missing data (4
-    populations take the per-pair path)."""
+    """1, 3 and 4 populations on a synthetic matrix with missing data, fused
+    (4 populations: the 105 statistics in three column groups) and per
+    pair."""
     rng = np.random.default_rng(40 + P)
     h = SY.neutral_haplotypes(24 * P, 300, rng, missing=0.03)
     pos = SY.uniform_positions(300, 60000, rng)

@@ -229,6 +229,7 @@ def _check_sums(got, stats, keys, n_keys):
 
 @pytest.mark.parametrize("P,missing,stat_set", [(1, True, "moments"), (2, True, "moments"),
                                                 (2, False, "moments"), (3, True, "moments"),
+                                                (4, True, "moments"), (4, False, "moments"),
                                                 (4, True, "self"), (2, True, "self")])
 def test_band_sums_windows_hap(P, missing, stat_set):
     d, pos = hap_data(40 + P, 40 * P + 17, 1300, 0.02 if missing else 0.0)
@@ -517,6 +518,7 @@ def test_pack_planes_fp4_definition(family, n_var, n_rows, missing):
     ("hap", 1, True, "moments", "window"), ("hap", 2, True, "moments", "window"),
     ("hap", 2, False, "moments", "bins"), ("hap", 3, True, "moments", "bins"),
     ("hap", 4, True, "self", "window"), ("hap", 2, True, "self", "bins"),
+    ("hap", 4, True, "moments", "window"), ("hap", 4, False, "moments", "window"),
     ("geno", 1, True, "moments", "bins"), ("geno", 1, False, "moments", "bins"),
     ("geno", 2, True, "moments", "window"), ("geno", 2, True, "self", "bins")])
 def test_band_sums_fp4_equal_int8(family, P, missing, stat_set, key):
