@@ -210,8 +210,10 @@ int pgld_band_plan_kernel_ms(const pgld_band_plan* plan, float* ms);
  * (device pointers for planes and marg); n_samples <= 32767 and k_pad <= 32768
  * per population (as packed).  sums (device, n_keys x n_stats) is
  * overwritten; key_pairs (device, n_keys, may be NULL) receives the number of
- * pairs per key.  Haplotypes: n_pops <= 4 (the 105 four-population moments
- * are summed in three column groups); genotypes: n_pops <= 2.  Replaces the
+ * pairs per key.  n_pops <= 4 (the 105 four-population moments are summed
+ * in three column groups; the 3-4 population genotype moments park their
+ * counts in a plan-owned global workspace, so they run on single bands only,
+ * not on region batches or caller-supplied workspaces).  Replaces the
  * chunk loop of moments_ld._compute_ld_sums (moments_ld.py:248-343) and
  * HaplotypeMatrix.compute_ld_statistics_gpu_* (haplotype_matrix.py:
  * 1592-1866). */

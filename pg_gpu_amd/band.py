@@ -293,15 +293,14 @@ FUSED_MAX_SAMPLES = 32767  # int16 count stash and 24-bit products of the fused 
 
 
 def fused_supported(family, n_pops, stat_set, pop_sizes):
-    """Whether the fused band kernels cover this statistic set: haplotype
-    sets for 1-4 populations (the 105 four-population moments in three column
-    groups); genotype sets for 1-2 (three or four populations' parked nine
-    counts exceed LDS); at most FUSED_MAX_SAMPLES samples per population."""
+    """Whether the fused band kernels cover this statistic set: haplotype and
+    genotype sets for 1-4 populations (the 105 four-population moments in
+    three column groups; the three- and four-population genotype moments park
+    their counts in a global workspace); at most FUSED_MAX_SAMPLES samples per
+    population."""
     if max(pop_sizes) > FUSED_MAX_SAMPLES:
         return False
-    if family == "hap":
-        return n_pops <= 4
-    return n_pops <= 2
+    return 1 <= n_pops <= 4
 
 
 def ld_sums(data, family, positions, rows, max_dist, *, edges=None, window=None, n_keys=None, origin=0.0,

@@ -105,6 +105,37 @@ struct StatGroups<PGLD_FAMILY_HAP, 4, PGLD_SET_MOMENTS> {
   }
 };
 
+template <>
+struct StatGroups<PGLD_FAMILY_GENO, 4, PGLD_SET_MOMENTS> {
+  static constexpr int NG = EPI_GENO_MOMENTS_4_NG;
+  static_assert(NG == 3, "three generated groups");
+  template <int G>
+  static constexpr int off() { return EPI_GENO_MOMENTS_4_GOFF[G]; }
+  template <int G>
+  static constexpr int count() { return EPI_GENO_MOMENTS_4_GOFF[G + 1] - EPI_GENO_MOMENTS_4_GOFF[G]; }
+  template <int G>
+  __device__ static void eval(const GenoP* v, double* out) {
+    if constexpr (G == 0)
+      epi_geno_moments_4_g0(v, out);
+    else if constexpr (G == 1)
+      epi_geno_moments_4_g1(v, out);
+    else
+      epi_geno_moments_4_g2(v, out);
+  }
+};
+
+// The joint genotype sets of three or four populations park the nine counts
+// of their populations, 72 KiB per population and workgroup: more than LDS
+// holds beside the operand rows, so they go to a global workspace (one slot
+// per workgroup of a launch; launches of at most GSTASH_TILES tiles).  The
+// last population is parked too, which frees its 144 accumulator registers
+// for the statistics.
+constexpr bool gstash_set(int family, int P, int set) {
+  return family == PGLD_FAMILY_GENO && P >= 3 && set == PGLD_SET_MOMENTS;
+}
+constexpr int64_t GSTASH_TILES = 1024;
+constexpr int64_t gstash_elems(int P) { return (int64_t)P * 4 * 9 * 16 * 64; }
+
 template <typename F, int... G>
 __device__ __forceinline__ void for_each_group(std::integer_sequence<int, G...>, F&& f) {
   (f(std::integral_constant<int, G>{}), ...);
@@ -209,7 +240,9 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
   constexpr int OP_FIRST_BYTES = OP_FIRST ? opbuf_bytes_swz<PL::N>() : 0;
   double* red = reinterpret_cast<double*>(smem + OP_FIRST_BYTES);
   const int red_n = 4 * tile.n_slots * NS;
-  int16_t* stash = reinterpret_cast<int16_t*>(smem + OP_FIRST_BYTES + ((red_n * 8 + 15) / 16) * 16);
+  constexpr bool GSTASH = gstash_set(FAMILY, P, SET);
+  unsigned char* after_red = smem + OP_FIRST_BYTES + ((red_n * 8 + 15) / 16) * 16;
+  int16_t* stash = GSTASH ? ba.gstash + (int64_t)blockIdx.x * gstash_elems(P) : reinterpret_cast<int16_t*>(after_red);
 
   // The tile's positions, key coordinates and bin edges are loaded into
   // registers, the population stream's first operand chunks right behind
@@ -545,7 +578,7 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         });
     }
   } else {
-    unsigned char* opbuf = reinterpret_cast<unsigned char*>(stash) + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
+    unsigned char* opbuf = GSTASH ? after_red : after_red + (size_t)(P - 1) * 4 * NC * 16 * 64 * 2;
     // with the load stream the next population's first operand chunks are in
     // flight while this one's counts are parked
 #pragma unroll 1
@@ -556,9 +589,11 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
       else
         block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
                                                     wave_live, opbuf, acc);
-      if (p == P - 1 || !wave_live) continue;
+      if ((p == P - 1 && !GSTASH) || !wave_live) continue;
       // all populations but the last park their counts in LDS as int16 (the
-      // two-population haplotype set parks the centred moments n, Sa, Sb, Sab)
+      // two-population haplotype set parks the centred moments n, Sa, Sb, Sab);
+      // with the global workspace the last one too, so that its accumulators
+      // are free during the statistics
       const int64_t jj = rj + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -605,15 +640,17 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
         } else {
           typename PopVal<FAMILY>::T V[P];
 #pragma unroll
-          for (int p = 0; p < P - 1; ++p) {
+          for (int p = 0; p < (GSTASH ? P : P - 1); ++p) {
             int c[NC];
 #pragma unroll
             for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
             V[p] = PopVal<FAMILY>::make(c);
           }
-          int c[NC];
-          elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
-          V[P - 1] = PopVal<FAMILY>::make(c);
+          if constexpr (!GSTASH) {
+            int c[NC];
+            elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
+            V[P - 1] = PopVal<FAMILY>::make(c);
+          }
           SG::template eval<G>(V, out);
         }
       });
@@ -641,7 +678,7 @@ template <int FAMILY, bool MISSING, int P, int SET, bool NIB, int KV = 0>
 __global__ __launch_bounds__(BAND_THREADS, band_min_waves(FAMILY, P, SET)) void k_band_sums(BandArgs ba,
                                                                                             PopArgs pa) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = ba.tile0 + xcd_tile(blockIdx.x, gridDim.x);
   const BandTile tile = ba.tiles[ba.order ? ba.order[t] : t];
   if (tile.n_slots <= 0) return;
   fill_recip_table<KV>(smem, ba);
@@ -1358,43 +1395,56 @@ static void launch_sums(const pgld_band_plan* plan, const BandArgs& ba_in, const
     }
   }
   if (plan->n_regions > 0) {
-    const int m = plan->order_mode;
-    const int64_t len = plan->order_len[m];
-    const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
-    const auto* rd = reinterpret_cast<const RegionDesc*>(plan->d_regions);
-    const int32_t* ord = plan->d_order + plan->order_off[m];
-    const int32_t xs = (int32_t)plan->xcd_split;
-    const dim3 g((unsigned)grid), b(BAND_THREADS);
-    auto go = [&](auto kv_tag) {
-      constexpr int KV = decltype(kv_tag)::value;
-      hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV>), g, b, lds, s, ba, rd, ord, len, xs);
-    };
-    if constexpr (VARIANTS) {
-      if (lean) {
-        if constexpr (TAB) {
-          if (tab) return go(std::integral_constant<int, KV_LEAN | KV_RTAB>{});
-        }
-        return go(std::integral_constant<int, KV_LEAN>{});
-      }
-    }
-    if constexpr (TAB) {
-      if (tab) {
-        if constexpr (VARIANTS) {
-          if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_RTAB | KV_PER_POP>{});
-          if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_RTAB | KV_ROLLED>{});
-        }
-        return go(std::integral_constant<int, KV_RTAB>{});
-      }
-      return go(std::integral_constant<int, 0>{});
+    if constexpr (gstash_set(FAMILY, P, SET)) {
+      return;  // run_sums refuses these sets for region batches
     } else {
+      const int m = plan->order_mode;
+      const int64_t len = plan->order_len[m];
+      const int64_t grid = plan->grid > 0 ? std::min<int64_t>(plan->grid, len) : len;
+      const auto* rd = reinterpret_cast<const RegionDesc*>(plan->d_regions);
+      const int32_t* ord = plan->d_order + plan->order_off[m];
+      const int32_t xs = (int32_t)plan->xcd_split;
+      const dim3 g((unsigned)grid), b(BAND_THREADS);
+      auto go = [&](auto kv_tag) {
+        constexpr int KV = decltype(kv_tag)::value;
+        hipLaunchKernelGGL((k_batch_sums<FAMILY, MISSING, P, SET, NIB, KV>), g, b, lds, s, ba, rd, ord, len, xs);
+      };
       if constexpr (VARIANTS) {
-        if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_PER_POP>{});
-        if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_ROLLED>{});
+        if (lean) {
+          if constexpr (TAB) {
+            if (tab) return go(std::integral_constant<int, KV_LEAN | KV_RTAB>{});
+          }
+          return go(std::integral_constant<int, KV_LEAN>{});
+        }
       }
-      return go(std::integral_constant<int, 0>{});
+      if constexpr (TAB) {
+        if (tab) {
+          if constexpr (VARIANTS) {
+            if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_RTAB | KV_PER_POP>{});
+            if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_RTAB | KV_ROLLED>{});
+          }
+          return go(std::integral_constant<int, KV_RTAB>{});
+        }
+        return go(std::integral_constant<int, 0>{});
+      } else {
+        if constexpr (VARIANTS) {
+          if (plan->kvar == KV_PER_POP) return go(std::integral_constant<int, KV_PER_POP>{});
+          if (plan->kvar == KV_ROLLED) return go(std::integral_constant<int, KV_ROLLED>{});
+        }
+        return go(std::integral_constant<int, 0>{});
+      }
     }
   } else {
     const dim3 g((unsigned)plan->n_tiles), b(BAND_THREADS);
+    if constexpr (gstash_set(FAMILY, P, SET)) {
+      // one workspace slot per workgroup of a launch of at most GSTASH_TILES
+      for (int64_t t0 = 0; t0 < plan->n_tiles; t0 += GSTASH_TILES) {
+        ba.tile0 = t0;
+        const dim3 gc((unsigned)std::min<int64_t>(GSTASH_TILES, plan->n_tiles - t0));
+        hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB>), gc, b, lds, s, ba, pa);
+      }
+      return;
+    }
     if constexpr (TAB) {
       if (tab) {
         hipLaunchKernelGGL((k_band_sums<FAMILY, MISSING, P, SET, NIB, KV_RTAB>), g, b, lds, s, ba, pa);
@@ -1423,9 +1473,14 @@ static SumsLauncher pick_launcher(int P, int set) {
     }
   } else {
     if (P == 1) return launch_sums<FAMILY, MISSING, 1, PGLD_SET_MOMENTS, NIB>;
-    if (P == 2) {
-      if (set == PGLD_SET_MOMENTS) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS, NIB>;
-      return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF, NIB>;
+    if (set == PGLD_SET_MOMENTS) {
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_MOMENTS, NIB>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_MOMENTS, NIB>;
+      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_MOMENTS, NIB>;
+    } else {
+      if (P == 2) return launch_sums<FAMILY, MISSING, 2, PGLD_SET_SELF, NIB>;
+      if (P == 3) return launch_sums<FAMILY, MISSING, 3, PGLD_SET_SELF, NIB>;
+      if (P == 4) return launch_sums<FAMILY, MISSING, 4, PGLD_SET_SELF, NIB>;
     }
   }
   return nullptr;
@@ -1721,7 +1776,7 @@ int pgld_band_plan_destroy(pgld_band_plan* plan) {
   void* ptrs[] = {plan->d_pos,      plan->d_kc,       plan->d_edges,     plan->d_tiles,    plan->d_pair_base,
                   plan->d_key_off,  plan->d_key_list, plan->d_partials,  plan->d_regions,  plan->d_order,
                   plan->d_row_off,  plan->d_col_off,  plan->d_col_list,  plan->d_snp_part, plan->d_snp_cnt,
-                  plan->d_super,    plan->d_red_lo,   plan->d_red_off,  plan->d_sums_order};
+                  plan->d_super,    plan->d_red_lo,   plan->d_red_off,  plan->d_sums_order, plan->d_gstash};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : plan->ev)
@@ -1793,7 +1848,9 @@ static size_t sums_lds_bytes(const pgld_band_plan* plan, int32_t family, int32_t
   const size_t red_bytes = (((size_t)4 * plan->max_tile_slots * ns_total * 8 + 15) / 16) * 16;
   // per-population statistics need no parked counts (each population's
   // epilogue follows its own products)
-  const size_t stash_bytes = stat_set == PGLD_SET_SELF ? 0 : (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
+  const size_t stash_bytes = stat_set == PGLD_SET_SELF || gstash_set(family, n_pops, stat_set)
+                                 ? 0
+                                 : (size_t)(n_pops - 1) * 4 * nc * 16 * 64 * 2;
   const int n_planes_used = family == PGLD_FAMILY_HAP ? (miss ? 2 : 1) : (miss ? 3 : 2);
   const size_t op_bytes = (size_t)2 * n_planes_used * BT * OP_ROW;
   if (red_bytes > (size_t)RED_LDS_BYTES || red_bytes + stash_bytes + op_bytes + STATIC_LDS_BYTES > 160 * 1024)
@@ -1837,6 +1894,23 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
               n_stats);
     return PGLD_EUNSUPPORTED;
   }
+  int16_t* gstash = nullptr;
+  if (gstash_set(family, n_pops, stat_set) && pl->n_tiles > 0) {
+    if (pl->n_regions > 0 || work || !allow_alloc) {
+      set_error("band sums: the %d-population genotype moments need the plan's own workspace (single bands)",
+                n_pops);
+      return PGLD_EUNSUPPORTED;
+    }
+    const size_t elems = (size_t)gstash_elems(n_pops) * (size_t)std::min<int64_t>(GSTASH_TILES, pl->n_tiles);
+    if (pl->gstash_cap < elems) {
+      if (pl->d_gstash) PGLD_TRY_HIP(hipFree(pl->d_gstash), "band count workspace free");
+      pl->d_gstash = nullptr;
+      pl->gstash_cap = 0;
+      PGLD_TRY_HIP(hipMalloc((void**)&pl->d_gstash, elems * sizeof(int16_t)), "band count workspace");
+      pl->gstash_cap = elems;
+    }
+    gstash = pl->d_gstash;
+  }
   const size_t need = (size_t)(std::max<int64_t>(1, pl->n_slots) + pl->n_red_chunks) * ns_total;
   double* partials = pl->d_partials;
   if (work) {
@@ -1869,6 +1943,7 @@ static int run_sums(pgld_band_plan* pl, int32_t family, int32_t n_pops, const Po
   ba.n_keys = pl->n_keys_local;
   ba.partials = partials;
   ba.order = pl->n_regions == 0 ? pl->d_sums_order : nullptr;
+  ba.gstash = gstash;
   if (pl->n_tiles > 0) {
     if (pl->timing) PGLD_TRY_HIP(hipEventRecord(pl->ev[0], s), "band sums event");
     L(pl, ba, pa, lds, s);

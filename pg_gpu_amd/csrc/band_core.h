@@ -63,6 +63,8 @@ struct pgld_band_plan {
   int64_t n_super = 0;
   double* d_snp_part = nullptr;        // workspace: per tile 64 row + 64 column sums
   int32_t* d_snp_cnt = nullptr;
+  int16_t* d_gstash = nullptr;         // workspace: parked counts of the 3-4 population genotype sets
+  size_t gstash_cap = 0;               // int16 elements
 };
 
 // Address space 1 (global memory) on AMDGPU.
@@ -869,6 +871,11 @@ struct BandArgs {
   // data: 1 / (n)_4 for n = 0 .. rtab_n as doubles at byte rtab_off of the
   // dynamic LDS (kernel variants with KV_RTAB only)
   int32_t rtab_n, rtab_off;
+  // fused sums of a single band: the launch covers tiles tile0 .. tile0 +
+  // grid - 1 of the launch order; gstash (3-4 population genotype moments)
+  // holds each workgroup's parked counts, GSTASH_ELEMS int16 per workgroup
+  int64_t tile0;
+  int16_t* gstash;
 };
 
 }  // namespace pgld

@@ -244,7 +244,7 @@ def emit_hap_set(P, names, sym, fname, first=0, count=None):
     return "\n".join([head] + pre + lines + body + tail), len(names)
 
 
-def emit_hap_groups(P, names, sym, fname, n_groups):
+def emit_groups(fam, P, names, sym, fname, n_groups):
     """The set split into n_groups functions fname_g<k> of consecutive
     statistics (the fused kernel sums one group at a time, so its running sums
     stay in registers), with their first statistics in fname_GOFF."""
@@ -253,7 +253,7 @@ def emit_hap_groups(P, names, sym, fname, n_groups):
     parts = ["constexpr int %s_NG = %d;" % (fname.upper(), n_groups),
              "constexpr int %s_GOFF[%d] = {%s};" % (fname.upper(), n_groups + 1, ", ".join(map(str, offs)))]
     for k in range(n_groups):
-        code, _ = emit_hap_set(P, names, sym, "%s_g%d" % (fname, k), offs[k], offs[k + 1] - offs[k])
+        code, _ = emit_set(fam, P, names, sym, "%s_g%d" % (fname, k), offs[k], offs[k + 1] - offs[k])
         parts.append(code)
     return "\n\n".join(parts)
 
@@ -274,14 +274,18 @@ def scale_fn(fname, scales):
     return "\n".join(out)
 
 
-def emit_set(fam, P, names, sym, fname):
+def emit_set(fam, P, names, sym, fname, first=0, count=None):
     if fam == "hap":
-        return emit_hap_set(P, names, sym, fname)
+        return emit_hap_set(P, names, sym, fname, first, count)
+    if count is None:
+        count = len(names) - first
     T = "GenoP"
     V = "G"
     calls, lines = {}, []
     body = []
     for s, (kind, p) in enumerate(names):
+        if not first <= s < first + count:
+            continue
         parts = []
         for w, t, q in terms_of(kind, p):
             e = call_expr(fam, t, q, sym)
@@ -294,14 +298,16 @@ def emit_set(fam, P, names, sym, fname):
             merged[v] = merged.get(v, 0.0) + w
         items = sorted(merged.items(), key=lambda kv: int(kv[0][1:]))
         if len(items) == 1 and items[0][1] == 1.0:
-            body.append("  out[%d] = %s;" % (s, items[0][0]))
+            body.append("  out[%d] = %s;" % (s - first, items[0][0]))
         else:
             expr = " + ".join("%r * %s" % (w, v) for v, w in items)
-            body.append("  out[%d] = %s;" % (s, expr))
+            body.append("  out[%d] = %s;" % (s - first, expr))
     head = "__device__ __forceinline__ void %s(const %s* %s, double* out) {" % (fname, T, V)
     used = [p for p in range(P) if any("GR[%d]" % p in ln for ln in lines)]
     pre = ["  HapR GR[%d];" % P] + ["  GR[%d] = hap_recips(G[%d].n);" % (p, p) for p in used] if used else []
-    return "\n".join([head] + pre + lines + body + ["}", scale_fn(fname, [1.0] * len(names))]), len(names)
+    whole = (first, count) == (0, len(names))
+    tail = ["}", scale_fn(fname, [1.0] * len(names))] if whole else ["}"]
+    return "\n".join([head] + pre + lines + body + tail), len(names)
 
 
 def self_names(P):
@@ -316,8 +322,8 @@ def main(out_path):
             code, n = emit_set(fam, P, ld_names(P), fam == "hap" and P == 2, "epi_%s_moments_%d" % (fam, P))
             parts.append("constexpr int EPI_%s_MOMENTS_%d_N = %d;" % (fam.upper(), P, n))
             parts.append(code)
-            if fam == "hap" and P == 4:
-                parts.append(emit_hap_groups(P, ld_names(P), False, "epi_hap_moments_4", 3))
+            if P == 4:
+                parts.append(emit_groups(fam, P, ld_names(P), False, "epi_%s_moments_4" % fam, 3))
             code, n = emit_set(fam, P, self_names(P), False, "epi_%s_self_%d" % (fam, P))
             parts.append("constexpr int EPI_%s_SELF_%d_N = %d;" % (fam.upper(), P, n))
             parts.append(code)

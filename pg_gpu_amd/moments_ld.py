@@ -2,8 +2,8 @@
 pg_gpu.moments_ld.compute_ld_statistics (moments_ld.py:40-388).
 
 The per-bin sums come from one fused int8-MFMA band pass (band.BandPlan)
-when a fused statistic set exists for the request (haplotypes with up to 4
-populations, genotypes with up to 2); otherwise from the plain path (pair
+when a fused statistic set exists for the request (haplotypes and genotypes
+with up to 4 populations); otherwise from the plain path (pair
 chunks -> per-pair counts -> per-pair statistics -> keyed sums), which is
 also what ``method="plain"`` selects.  Both run entirely in libpgld.
 """

@@ -102,12 +102,13 @@ def test_moments_hap_pop_counts(P, method):
     np.testing.assert_allclose(res["sums"][-1], [H[k] for k in res["stats"][1]], rtol=1e-13)
 
 
-@pytest.mark.parametrize("P", [3, 4])
-def test_moments_genotype_three_and_four_pops(P):
+@pytest.mark.parametrize("P,method", [(3, "auto"), (4, "auto"), (3, "plain"), (4, "plain")])
+def test_moments_genotype_three_and_four_pops(P, method):
     """Genotype moments statistics for 3 and 4 populations (45 / 105
     statistics, every case class of genotype_kernels.py:27-405 including the
     all-distinct pi2 at P = 4) through moments_ld.compute_ld_statistics,
-    against the oracle; missing data and the union biallelic filter."""
+    fused and per pair, against the oracle; missing data and the union
+    biallelic filter."""
     rng = np.random.default_rng(60 + P)
     g = SY.hwe_genotypes(16 * P, 220, rng, missing=0.04)
     g[:, 5] = 0   # monomorphic: dropped by the union filter
@@ -117,7 +118,7 @@ def test_moments_genotype_three_and_four_pops(P):
     gm = GenotypeMatrix(g, pos, sample_sets=sets)
     bins = np.array([0.0, 400.0, 1500.0, 6000.0])
     res = moments_ld.compute_ld_statistics(genotype_matrix=gm, pops=names, bp_bins=bins, use_genotypes=True,
-                                           report=False)
+                                           report=False, method=method)
     rows = [sets[n] for n in names]
     want = O.moments_ld_sums(g, pos, rows, bins, use_genotypes=True)
     scale = O.moments_ld_sums(g, pos, rows, bins, use_genotypes=True, absolute=True)

@@ -214,6 +214,9 @@ def _sums_oracle(family, d, pos, pops, md, stat_set, key):
             stats = O.two_pop_stats(cl[0], cl[1])
         else:
             stats = O.multi_pop_stats_hap(cl, [None] * P)
+    elif stat_set == "self" and P > 1:
+        specs = [(f"{t}", [(1.0, t, (p,) * k)]) for t, k in (("dd", 2), ("dz", 3), ("pi2", 4)) for p in range(P)]
+        stats = O.multi_pop_stats_geno(cl, [None] * P, specs)
     else:
         stats = O.multi_pop_stats_geno(cl, [None] * P)
     keys = key(ii, jj)
@@ -317,19 +320,46 @@ def test_band_sums_two_key_subtiles(key, family, P, stat_set, missing, fp4):
     np.testing.assert_array_equal(np_(kp), np.bincount(keys[(keys >= 0) & (keys < n_keys)], minlength=n_keys))
 
 
-@pytest.mark.parametrize("P,missing", [(1, True), (1, False), (2, True)])
-def test_band_sums_geno(P, missing):
+@pytest.mark.parametrize("P,missing,stat_set,fp4", [
+    (1, True, "moments", False), (1, False, "moments", False), (2, True, "moments", False),
+    (3, True, "moments", True), (3, False, "moments", False), (4, True, "moments", True),
+    (4, False, "moments", True), (3, True, "self", True), (4, True, "self", False)])
+def test_band_sums_geno(P, missing, stat_set, fp4):
+    """Genotype sets of 1-4 populations against the oracle; the three- and
+    four-population moments (45 / 105 statistics, every case class of
+    genotype_kernels.py:27-405) park their counts in the global workspace."""
     d, pos = geno_data(70 + P, 60 * P + 5, 800, 0.05 if missing else 0.0)
     n = d.shape[0]
     pops = [list(range(p, n, P)) for p in range(P)]
     edges = np.logspace(0, 5, 17)
     md = float(edges[-1])
     plan = band.BandPlan(pos, md, edges=edges)
-    packed = [band.pack_population(d, r, "geno") for r in pops]
-    sums = plan.sums(packed)
-    stats, keys = _sums_oracle("geno", d, pos, pops, md, "moments",
+    packed = [band.pack_population(d, r, "geno", fp4=fp4) for r in pops]
+    sums, kp = plan.sums(packed, stat_set=stat_set, want_pairs=True)
+    stats, keys = _sums_oracle("geno", d, pos, pops, md, stat_set,
                                lambda ii, jj: O.bin_index(pos[jj] - pos[ii], edges))
     _check_sums(sums, stats, keys, len(edges) - 1)
+    nb = len(edges) - 1
+    np.testing.assert_array_equal(np_(kp), np.bincount(keys[(keys >= 0) & (keys < nb)], minlength=nb))
+
+
+def test_geno_three_pop_moments_chunked_launches_equal_fp4_and_int8():
+    """More tiles than one launch of the workspace-parked genotype moments
+    covers (1,024): the launches continue the tile order, so FP4 and int8
+    planes give the same sums bit for bit, and the sums match the per-pair
+    path."""
+    d, pos = geno_data(83, 3 * 24, 10000, 0.04)
+    pops = [list(range(p, 72, 3)) for p in range(3)]
+    plan = band.BandPlan(pos, 2.5e5, window=3.0e5)
+    assert plan.n_tiles > 4 * 1024
+    s8, k8 = plan.sums([band.pack_population(d, r, "geno") for r in pops], want_pairs=True)
+    s4, k4 = plan.sums([band.pack_population(d, r, "geno", fp4=True) for r in pops], want_pairs=True)
+    np.testing.assert_array_equal(np_(s4), np_(s8))
+    np.testing.assert_array_equal(np_(k4), np_(k8))
+    ref, kref = band.ld_sums(d, "geno", pos, pops, 2.5e5, window=3.0e5, n_keys=plan.n_keys, method="plain")
+    np.testing.assert_array_equal(np_(k8), np_(kref))
+    ref = np_(ref)
+    assert np.all(np.abs(np_(s8) - ref) <= 1e-10 * np.maximum(np.abs(ref), 1e-300) + 1e-9)
 
 
 def test_band_sums_deterministic():
