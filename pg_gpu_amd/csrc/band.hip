@@ -889,6 +889,15 @@ __device__ __forceinline__ void pack_tile(const int8_t* __restrict__ data, int64
         const int64_t at = base + v0 + 4 * q;
         if (WORDS && v0 + 4 * q + 3 < n_var) {
           word = *reinterpret_cast<const uint32_t*>(data + at);
+        } else if (v0 + 4 * q + 3 < n_var) {
+          // a row that is not 4-byte aligned: the two aligned dwords that
+          // hold bytes at .. at + 3, joined with v_alignbyte (each holds a
+          // byte of the row, so neither read leaves the row's pages)
+          const uintptr_t a = reinterpret_cast<uintptr_t>(data + at);
+          const uint32_t* pa = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+          const uint32_t o = (uint32_t)(a & 3);
+          const uint32_t lo = pa[0];
+          word = o ? __builtin_amdgcn_alignbyte(pa[1], lo, o) : lo;  // bytes o .. o + 3 of {pa[1], lo}
         } else {
 #pragma unroll
           for (int b = 0; b < 4; ++b) {
