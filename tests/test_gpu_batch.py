@@ -267,3 +267,36 @@ def test_pack_batch_equals_pack_population(family, fp4):
         check()
         out[0][0].marg.fill_(11)  # eager work between replays
     assert int(np_(ref[2][0].flags)[1]) == 1 and int(np_(ref[1][0].flags)[0]) == 0
+
+
+def test_batch_four_population_moments_and_refused_sets():
+    """The 105 four-population haplotype moments over a region batch equal
+    each region's own band pass bit for bit and the oracle within 1e-12 of
+    sum |terms|; the three-population genotype moments, which park their
+    counts in a plan's own workspace, are refused for region batches."""
+    edges = np.array([0.0, 1e3, 1e4, 1e5, 1e6])
+    datas, pos, rows = SY.ragged_regions(seed=33, n_regions=4, m_lo=60, m_hi=250, s_lo=60, s_hi=140, n_pops=4,
+                                         span=40_000)
+    dev = L.device()
+    d_dev = [torch.as_tensor(d, device=dev) for d in datas]
+    packed = batch.pack_regions(d_dev, rows, fp4=True)
+    b = batch.RegionBatch(pos, 1e6, edges).bind(packed, stat_set="moments")
+    got = np_(b.sums(True))
+    assert got.shape == (4, 4, 105)
+    for r in range(4):
+        single = band.BandPlan(pos[r], 1e6, edges=edges)
+        np.testing.assert_array_equal(got[r], np_(single.sums(packed[r], stat_set="moments", with_missing=True)))
+        ii, jj = O.pairs_within_distance(pos[r], np.inf)
+        cl = [O.hap_counts(datas[r], ii, jj, rw) for rw in rows[r]]
+        st = O.multi_pop_stats_hap([c for c, _ in cl], [nv for _, nv in cl])
+        keys = O.bin_index(pos[r][jj] - pos[r][ii], edges)
+        want, scale = O.keyed_sums(keys, st, 4), O.keyed_sums(keys, np.abs(st), 4)
+        assert np.all(np.abs(got[r] - want) <= 1e-12 * np.maximum(scale, 1e-300) + 1e-300), f"region {r}"
+    rng = np.random.default_rng(6)
+    gl = [torch.as_tensor(SY.hwe_genotypes(60, len(p), rng, missing=0.04), device=dev) for p in pos]
+    g3 = [[list(range(k, 60, 3)) for k in range(3)] for _ in pos]
+    with pytest.raises(L.PgldError) as e:
+        gb = batch.RegionBatch(pos, 1e6, edges).bind(batch.pack_regions(gl, g3, family="geno", fp4=True),
+                                                     stat_set="moments")
+        gb.sums(True)
+    assert e.value.status == L.EUNSUPPORTED
