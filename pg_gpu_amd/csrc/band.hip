@@ -554,7 +554,8 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     for (int p = 0; p < P; ++p) {
       if constexpr ((KV & KV_LEAN) != 0)
         block_products<FAMILY, MISSING, false, NIB, 1, false, true>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0,
-                                                                    tile.j0, wr, wc, wave_live, opbuf, acc);
+                                                                    tile.j0, wr, wc, wave_live, opbuf, acc,
+                                                                    pa.n_samples[p]);
       else if constexpr ((KV & KV_PER_POP) != 0)
         block_products<FAMILY, MISSING, false, NIB>(pa.planes[p], ba.n_var, pa.k_pad[p], tile.i0, tile.j0, wr, wc,
                                                     wave_live, opbuf, acc);

@@ -277,7 +277,7 @@ template <int FAMILY, bool MISSING, bool POPC = false, bool NIB = false, int PFS
 __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes, int64_t n_var, int64_t k_pad,
                                                int i0, int j0, int wr, int wc, bool wave_live,
                                                unsigned char* __restrict__ opbuf,
-                                               WaveAcc<Planes<FAMILY, MISSING>::N, NIB>& acc) {
+                                               WaveAcc<Planes<FAMILY, MISSING>::N, NIB>& acc, int n_samples = -1) {
   static_assert(!(POPC && NIB), "bit planes and FP4 planes are separate layouts");
   static_assert(!(POPC && SWZ), "the popcount form reads the padded layout");
   constexpr int RS = SWZ ? OP_ROW_SWZ : OP_ROW;  // LDS row stride
@@ -331,6 +331,12 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
   const int ra = wr + (lane & 31), rb = wc + (lane & 31), kh = 16 * (lane >> 5);
   const int c16 = ((lane & 7) << 4) ^ kh;
   const int swz_a = (ra * RS) ^ c16, swz_b = ((N * BT + rb) * RS) ^ c16;
+  // FP4 rows without the register-stage double buffer of the fragments: a
+  // sub-step whose 64 samples all lie past the population's last sample
+  // (given n_samples >= 0) multiplies padding nibbles, which are 0, so its
+  // fragment reads and MFMAs are skipped -- except the first chunk's first
+  // sub-step, which starts the accumulators
+  const int kvalid = (NIB && !FDB && n_samples >= 0) ? 32 * ((n_samples + 63) / 64) : kp;
   auto step = [&](auto first_tag, v4i* pre, int k0) {
     constexpr bool FIRST = decltype(first_tag)::value;
     __syncthreads();  // the previous chunk has been read by every wave
@@ -399,6 +405,9 @@ __device__ __forceinline__ void block_products(const int8_t* __restrict__ planes
       if constexpr (FDB) frag(0, A[0], B[0]);
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
+        if constexpr (!FDB) {
+          if (!(FIRST && st == 0) && k0 + 32 * st >= kvalid) break;  // wave-uniform
+        }
         if constexpr (FDB) {
           if (st < 3) frag(st + 1, A[(st + 1) % NF], B[(st + 1) % NF]);
         } else {
