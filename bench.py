@@ -49,11 +49,11 @@ TRAFFIC_BYTES_PER_LAUNCH = {"fp4": 0.3287e9, "int8": 1.581e9}
 # too, and SQ_INSTS_MFMA per dispatch): the VALU-issue bound of the kernel.
 WINDOWED_SQ = {"fp4": {"valu_insts": 3.01273e8, "mfma_insts": 1.44901e7}}
 # The same for k_batch_sums on the 512-region ragged batch (FP4 planes, the
-# autotuner's lean variant): FETCH_SIZE 2.32834e8 KiB x 2.000 + WRITE_SIZE
-# 1.24309e6 KiB x 1.000 per dispatch, and its issue counters.
+# autotuner's lean variant): FETCH_SIZE 2.3243e8 KiB x 2.000 + WRITE_SIZE
+# 1.23857e6 KiB x 1.000 per dispatch, and its issue counters.
 # Summary and commands: profiles/k_batch_sums_ragged_512_regions.md
-RAGGED_TRAFFIC_BYTES = {"fp4": 478.1e9}
-RAGGED_SQ = {"fp4": {"valu_insts": 9.98258e10, "mfma_insts": 6.82821e9}}
+RAGGED_TRAFFIC_BYTES = {"fp4": 477.3e9}
+RAGGED_SQ = {"fp4": {"valu_insts": 9.83314e10, "mfma_insts": 6.01159e9}}
 CLOCK_HZ = 2.4e9         # MI355X peak engine clock
 N_SIMD = 256 * 4         # 256 CUs x 4 SIMDs
 VALU_ISSUE_CYCLES = 4    # a wave64 VALU instruction (FP64 included) holds its SIMD's issue for 4 cycles
