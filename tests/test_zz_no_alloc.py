@@ -73,3 +73,18 @@ def test_noalloc_rejects_plans_beyond_the_fixed_workspace():
     rc, _, _ = _call_sums("pgld_band_sums_fp4_noalloc", plan, pops, L.SET_MOMENTS, False)
     assert rc == -3
     assert b"fixed" in L.lib().pgld_last_error()
+
+
+def test_noalloc_refuses_the_genotype_sets_that_park_counts_globally():
+    # the 3-population genotype moments need the plan's own parked-count
+    # workspace: the allocation-free entry point returns PGLD_EUNSUPPORTED,
+    # the allocating one computes them
+    import torch
+    plan, pops = _case(13, 90, 300, "geno", 3, L.SET_MOMENTS, True)
+    print("noalloc refusal: geno P=3 moments", flush=True)
+    rc, _, _ = _call_sums("pgld_band_sums_fp4_noalloc", plan, pops, L.SET_MOMENTS, True)
+    assert rc == L.EUNSUPPORTED
+    assert b"workspace" in L.lib().pgld_last_error()
+    rc0, s0, k0 = _call_sums("pgld_band_sums_fp4", plan, pops, L.SET_MOMENTS, True)
+    torch.cuda.synchronize()
+    assert rc0 == 0 and k0.cpu().numpy().sum() == plan.n_pairs
