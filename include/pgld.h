@@ -237,7 +237,9 @@ int pgld_band_sums_fp4(const pgld_band_plan* plan, int32_t family, int32_t n_pop
  * partial-row workspace is a fixed 32 MiB device array (2^22 doubles), one
  * copy per GPU held until the process ends, instead of the plan's own buffer;
  * plans needing more ((n_slots + reduction chunks) x (n_stats + 1) doubles)
- * return PGLD_ECAPACITY.  Every call on one GPU shares that array: calls that
+ * return PGLD_ECAPACITY, and the 3-4 population genotype moments (which
+ * park counts in the plan's own workspace) PGLD_EUNSUPPORTED.  Every call on
+ * one GPU shares that array: calls that
  * overlap on one GPU (other streams, other threads) race.  Nothing in the
  * array needs clearing between calls (every used element is written before
  * it is read).  Same arguments and results as pgld_band_sums_fp4. */
