@@ -621,40 +621,40 @@ __device__ __forceinline__ void band_sums_tile(const BandArgs& ba, const PopArgs
     // the others reuse them
     if (wave_live)
       for_each_group(std::make_integer_sequence<int, SG::NG>{}, [&](auto g_tag) {
-      constexpr int G = decltype(g_tag)::value;
-      const int kmode = SG::NG == 1 || tile.n_slots > 255 ? 0 : (G == 0 ? 1 : 2);
-      epilogue(std::integral_constant<int, SG::template count<G>()>{}, SG::template off<G>(), 1, G == 0, kmode,
-               [&](int r, int64_t i, double* out) {
-        if constexpr (HAP_SELF && P == 1) {
-          double n, a, b, s;
-          hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
-          hap_self_stats(n, a, b, s, self_recip(n), out);
-        } else if constexpr (HAP2) {
-          double n[2], a[2], b[2], s[2];
-          const int16_t* st = stash + (((w * NC) * 16 + r) << 6) + lane;
-          n[0] = st[0];
-          a[0] = st[16 * 64];
-          b[0] = st[2 * 16 * 64];
-          s[0] = st[3 * 16 * 64];
-          hap_moments<MISSING>(acc, r, pa.marg[1], i, j, pa.n_samples[1], n[1], a[1], b[1], s[1]);
-          hap_two_pop_stats(n, a, b, s, out);
-        } else {
-          typename PopVal<FAMILY>::T V[P];
+        constexpr int G = decltype(g_tag)::value;
+        const int kmode = SG::NG == 1 || tile.n_slots > 255 ? 0 : (G == 0 ? 1 : 2);
+        epilogue(std::integral_constant<int, SG::template count<G>()>{}, SG::template off<G>(), 1, G == 0, kmode,
+                 [&](int r, int64_t i, double* out) {
+          if constexpr (HAP_SELF && P == 1) {
+            double n, a, b, s;
+            hap_moments<MISSING>(acc, r, pa.marg[0], i, j, pa.n_samples[0], n, a, b, s);
+            hap_self_stats(n, a, b, s, self_recip(n), out);
+          } else if constexpr (HAP2) {
+            double n[2], a[2], b[2], s[2];
+            const int16_t* st = stash + (((w * NC) * 16 + r) << 6) + lane;
+            n[0] = st[0];
+            a[0] = st[16 * 64];
+            b[0] = st[2 * 16 * 64];
+            s[0] = st[3 * 16 * 64];
+            hap_moments<MISSING>(acc, r, pa.marg[1], i, j, pa.n_samples[1], n[1], a[1], b[1], s[1]);
+            hap_two_pop_stats(n, a, b, s, out);
+          } else {
+            typename PopVal<FAMILY>::T V[P];
 #pragma unroll
-          for (int p = 0; p < (GSTASH ? P : P - 1); ++p) {
-            int c[NC];
+            for (int p = 0; p < (GSTASH ? P : P - 1); ++p) {
+              int c[NC];
 #pragma unroll
-            for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
-            V[p] = PopVal<FAMILY>::make(c);
+              for (int q = 0; q < NC; ++q) c[q] = stash[((((p * 4 + w) * NC + q) * 16 + r) << 6) + lane];
+              V[p] = PopVal<FAMILY>::make(c);
+            }
+            if constexpr (!GSTASH) {
+              int c[NC];
+              elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
+              V[P - 1] = PopVal<FAMILY>::make(c);
+            }
+            SG::template eval<G>(V, out);
           }
-          if constexpr (!GSTASH) {
-            int c[NC];
-            elem_counts<FAMILY, MISSING>(acc, r, pa.marg[P - 1], ba.n_var, i, j, pa.n_samples[P - 1], c);
-            V[P - 1] = PopVal<FAMILY>::make(c);
-          }
-          SG::template eval<G>(V, out);
-        }
-      });
+        });
       });
   }
   __syncthreads();
