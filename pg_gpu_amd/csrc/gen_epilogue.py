@@ -10,7 +10,10 @@ two haplotype populations the terms follow the CuPy fast-path forms
 (ld_pipeline.py:271-322), i.e. pi2(i,i,k,k) symmetrised over the loci.
 
 "self" sets list only the within-population statistics DD_i_i, Dz_i_i_i,
-pi2_i_i_i_i per population (3P values).
+pi2_i_i_i_i per population (3P values).  The 105 four-population moments are
+also emitted as three functions of 35 consecutive statistics each
+(<name>_g0 .. _g2, first statistics in <NAME>_GOFF), which the fused kernel
+sums one group at a time.
 
 Run by the build before hipcc.
 """
