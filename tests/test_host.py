@@ -205,3 +205,42 @@ def test_window_layers_are_disjoint_and_cover():
             assert stop[a] <= start[b]
     assert _layers(start, stop, []) == []
 
+
+
+@pytest.mark.parametrize("fam", ["hap", "geno"])
+def test_grouped_four_population_statistics_cover_the_set_once(fam):
+    """The generated three-group form of the 105 four-population moments:
+    group k writes out[0 .. n_k - 1] once each, the groups' first statistics
+    are GOFF, together they cover ld_names(4) in order, and each group states
+    the same expression for a statistic as the one-function form (up to the
+    names of its shared terms)."""
+    import importlib.util
+    path = os.path.join(os.path.dirname(__file__), "..", "pg_gpu_amd", "csrc", "gen_epilogue.py")
+    spec = importlib.util.spec_from_file_location("gen_epilogue_under_test", path)
+    G = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(G)
+    names = G.ld_names(4)
+    assert len(names) == 105
+    text = G.emit_groups(fam, 4, names, False, "f", 3)
+    goff = [int(x) for x in re.search(r"F_GOFF\[4\] = \{([^}]*)\}", text).group(1).split(",")]
+    assert goff == [0, 35, 70, 105]
+    full, _ = G.emit_set(fam, 4, names, False, "f")
+
+    def expanded(code):
+        # statement per output index with shared-term names replaced by their definitions
+        defs = dict(re.findall(r"const double (\w+) = (.*);", code))
+        out = {}
+        for k, e in re.findall(r"out\[(\d+)\] = (.*);", code):
+            for _ in range(4):
+                e = re.sub(r"\b([ptrk]\d+)\b", lambda m: "(" + defs.get(m.group(1), m.group(1)) + ")", e)
+            out[int(k)] = e
+        return out
+
+    whole = expanded(full)
+    assert sorted(whole) == list(range(105))
+    for g in range(3):
+        body = text.split("void f_g%d(" % g)[1].split("\n}\n")[0]
+        part = expanded(body)
+        assert sorted(part) == list(range(goff[g + 1] - goff[g]))
+        for k, e in part.items():
+            assert e == whole[goff[g] + k], (g, k)
