@@ -343,13 +343,14 @@ def test_band_sums_geno(P, missing, stat_set, fp4):
     np.testing.assert_array_equal(np_(kp), np.bincount(keys[(keys >= 0) & (keys < nb)], minlength=nb))
 
 
-def test_geno_three_pop_moments_chunked_launches_equal_fp4_and_int8():
+@pytest.mark.parametrize("P", [3, 4])
+def test_geno_moments_chunked_launches_equal_fp4_and_int8(P):
     """More tiles than one launch of the workspace-parked genotype moments
     covers (1,024): the launches continue the tile order, so FP4 and int8
     planes give the same sums bit for bit, and the sums match the per-pair
     path."""
-    d, pos = geno_data(83, 3 * 24, 10000, 0.04)
-    pops = [list(range(p, 72, 3)) for p in range(3)]
+    d, pos = geno_data(83, 72, 10000, 0.04)
+    pops = [list(range(p, 72, P)) for p in range(P)]
     plan = band.BandPlan(pos, 2.5e5, window=3.0e5)
     assert plan.n_tiles > 4 * 1024
     s8, k8 = plan.sums([band.pack_population(d, r, "geno") for r in pops], want_pairs=True)
